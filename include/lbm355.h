@@ -1,0 +1,123 @@
+/*
+ * lbm355 -- C ABI of the MI355X (gfx950) lattice Boltzmann stream-collide sweep.
+ *
+ * This library replaces the compiled sweep that lbmpy obtains from
+ * pystencils: create_lb_function() -> ast.compile() (lbmpy
+ * src/lbmpy/creationfunctions.py:543-559, kernel built at :562-582), whose
+ * generated C/CUDA function takes the raw PDF field pointers, the field
+ * shape/strides and the free parameters (relaxation rates).  Here the same
+ * information travels as plain pointers plus one POD descriptor; the method
+ * (moment bases, equilibrium, relaxation rates, force terms) arrives as a
+ * device-resident numeric table built on the host from lbmpy's LBMethod.
+ *
+ * Conventions
+ *  - all pointers are device pointers (HIP), except `desc`, which is host memory;
+ *  - element strides; x is contiguous (stride 1), "fzyx" layout;
+ *  - every call is asynchronous on `stream` (a hipStream_t, 0 = null stream);
+ *  - return value 0 = success, otherwise an LBM355_ERR_* code, with a message
+ *    available from lbm355_last_error();
+ *  - the sweep allocates nothing and never blocks.
+ */
+#ifndef LBM355_H
+#define LBM355_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* stencils ('walberla' direction order, lbmpy stencils.py:133-135,194-198,221-227) */
+#define LBM355_D2Q9 0
+#define LBM355_D3Q19 1
+#define LBM355_D3Q27 2
+
+/* collision families */
+#define LBM355_FAMILY_MOMENT 0   /* SRT / TRT / MRT (population or raw-moment space) */
+#define LBM355_FAMILY_CENTRAL 1  /* central-moment methods */
+#define LBM355_FAMILY_CUMULANT 2 /* cumulant methods (default polynomial cumulant set) */
+
+/* streaming patterns (lbmpy fieldaccess.py:66-75 pull, :136-157 AA) */
+#define LBM355_PULL 0
+#define LBM355_AA_EVEN 1
+#define LBM355_AA_ODD 2
+
+/* element types */
+#define LBM355_F64 0
+#define LBM355_F32 1
+#define LBM355_F16 2
+
+/* treatment of the two faces of an axis */
+#define LBM355_FACE_GHOST 0    /* read/write the ghost layer as is (needs ghost_layers >= 1) */
+#define LBM355_FACE_PERIODIC 1 /* periodic wrap inside the kernel */
+#define LBM355_FACE_NOSLIP 2   /* half-way bounce-back walls just outside the interior */
+
+/* flags */
+#define LBM355_FLAG_ZERO_CENTERED 1 /* PDFs stored as deviation from the lattice weights */
+#define LBM355_FLAG_COMPRESSIBLE 2  /* u = j / rho (else j / rho_0) */
+#define LBM355_FLAG_EQ_SHIFT 4      /* equilibrium velocity shifted by F / (2 rho_ref) */
+#define LBM355_FLAG_FORCE 8         /* a force model is present (macroscopic velocity shift) */
+#define LBM355_FLAG_SMAGORINSKY 16  /* Smagorinsky subgrid model on the shear rate */
+
+/* error codes */
+#define LBM355_OK 0
+#define LBM355_ERR_ARG 1
+#define LBM355_ERR_UNSUPPORTED 2
+#define LBM355_ERR_LAUNCH 3
+
+typedef struct lbm355_sweep_desc {
+  int32_t stencil;       /* LBM355_D2Q9 ... */
+  int32_t family;        /* LBM355_FAMILY_* */
+  int32_t pattern;       /* LBM355_PULL / LBM355_AA_EVEN / LBM355_AA_ODD */
+  int32_t store_dtype;   /* element type of the PDF arrays */
+  int32_t compute_dtype; /* arithmetic type: LBM355_F64 or LBM355_F32 */
+  int32_t nx, ny, nz;    /* interior cells (nz = 1 for 2-D) */
+  int32_t ghost_layers;  /* 0 or 1 */
+  int32_t face[3];       /* LBM355_FACE_* per axis */
+  int64_t stride_y, stride_z, stride_q;          /* PDF strides (elements) */
+  int64_t out_stride_y, out_stride_z, out_stride_c; /* density/velocity output strides */
+  int32_t flags;         /* LBM355_FLAG_* */
+  int32_t n_active;      /* moment family: number of individually relaxed moments */
+  double force[3];       /* constant body force */
+  double smagorinsky_constant; /* Cs, used with LBM355_FLAG_SMAGORINSKY */
+  double omega_shear;    /* molecular shear relaxation rate (Smagorinsky) */
+  const void* tables;    /* device pointer: method table, compute_dtype elements */
+} lbm355_sweep_desc;
+
+/* One stream-collide time step over the interior.
+ * Replaces the compiled kernel returned by lbmpy create_lb_function
+ * (creationfunctions.py:554) and called per step by
+ * LatticeBoltzmannStep.time_step (lbstep.py:271-282).
+ * pull: reads pdf_src, writes pdf_dst (caller swaps); AA: pdf_dst must equal pdf_src.
+ * density_out / velocity_out may be NULL (lbmpy LBMConfig.output, creationfunctions.py:299-305). */
+int lbm355_sweep(const lbm355_sweep_desc* desc, const void* pdf_src, void* pdf_dst, void* density_out,
+                 void* velocity_out, void* stream);
+
+/* Number of compute_dtype elements the method table of (stencil, family) holds. */
+int64_t lbm355_table_elements(int32_t stencil, int32_t family);
+
+/* Half-way bounce-back for an index list of boundary links, the NoSlip boundary kernel
+ * (lbmpy boundaries/boundaryconditions.py:104-139, built by boundaries/boundaryhandling.py:159-199).
+ * links: n_links x 4 int32 (x, y, z, dir) of fluid cells in ghost-layer coordinates.
+ * pattern LBM355_PULL: pdf[x + c_d][inv d] = pdf[x][d]                 (before a pull sweep)
+ * pattern LBM355_AA_EVEN: after an EVEN sweep, before ODD: pdf[x + c_d][d] = pdf[x][inv d]
+ * pattern LBM355_AA_ODD: after an ODD sweep, before EVEN: pdf[x][inv d] = pdf[x + c_d][d]
+ * (advanced_streaming/indexing.py:12-181 BetweenTimestepsIndexing). */
+int lbm355_noslip_links(int32_t stencil, int32_t pattern, int32_t dtype, void* pdf, int64_t stride_y,
+                        int64_t stride_z, int64_t stride_q, const int32_t* links, int64_t n_links, void* stream);
+
+/* Conserved-quantity reduction in FP64 over the interior: out[0] = sum(rho), out[1..3] = sum(j).
+ * out is a device buffer of 4 doubles (overwritten). Stored zero-centred PDFs add 1 per cell to rho. */
+int lbm355_reduce_conserved(const lbm355_sweep_desc* desc, const void* pdf, double* out, void* stream);
+
+/* Last error message of the calling thread (static storage, never NULL). */
+const char* lbm355_last_error(void);
+
+/* ABI version. */
+int32_t lbm355_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LBM355_H */

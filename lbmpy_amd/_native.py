@@ -1,0 +1,82 @@
+"""ctypes binding of the C-ABI sweep library ``liblbm355.so`` (``include/lbm355.h``).
+
+torch is imported first so that the library's ``libamdhip64.so.7`` dependency
+resolves to the HIP runtime torch already loaded: PDF tensors, streams and the
+kernels then live in one runtime.  There is no fallback: if the library is
+missing the import raises.
+"""
+import ctypes
+import os
+
+import torch  # noqa: F401  (must precede the library load, see module docstring)
+
+from .csrc.build import library_path
+
+c_int32, c_int64, c_double, c_void_p = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
+
+D2Q9, D3Q19, D3Q27 = 0, 1, 2
+FAMILY_MOMENT, FAMILY_CENTRAL, FAMILY_CUMULANT = 0, 1, 2
+PULL, AA_EVEN, AA_ODD = 0, 1, 2
+F64, F32, F16 = 0, 1, 2
+FACE_GHOST, FACE_PERIODIC, FACE_NOSLIP = 0, 1, 2
+FLAG_ZERO_CENTERED, FLAG_COMPRESSIBLE, FLAG_EQ_SHIFT, FLAG_FORCE, FLAG_SMAGORINSKY = 1, 2, 4, 8, 16
+
+DTYPE_CODES = {"float64": F64, "float32": F32, "float16": F16}
+
+
+class SweepDesc(ctypes.Structure):
+    _fields_ = [("stencil", c_int32), ("family", c_int32), ("pattern", c_int32), ("store_dtype", c_int32),
+                ("compute_dtype", c_int32), ("nx", c_int32), ("ny", c_int32), ("nz", c_int32),
+                ("ghost_layers", c_int32), ("face", c_int32 * 3),
+                ("stride_y", c_int64), ("stride_z", c_int64), ("stride_q", c_int64),
+                ("out_stride_y", c_int64), ("out_stride_z", c_int64), ("out_stride_c", c_int64),
+                ("flags", c_int32), ("n_active", c_int32),
+                ("force", c_double * 3), ("smagorinsky_constant", c_double), ("omega_shear", c_double),
+                ("tables", c_void_p)]
+
+
+EXPORTED = ("lbm355_sweep", "lbm355_table_elements", "lbm355_noslip_links", "lbm355_reduce_conserved",
+            "lbm355_last_error", "lbm355_version")
+
+_lib = None
+
+
+def load(path=None):
+    """Load the library (once) and declare the function signatures."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    path = path or library_path()
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with `python -m lbmpy_amd.csrc.build` "
+                          "(or __graft_entry__.build())")
+    lib = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+    lib.lbm355_sweep.argtypes = [ctypes.POINTER(SweepDesc), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
+    lib.lbm355_sweep.restype = c_int32
+    lib.lbm355_table_elements.argtypes = [c_int32, c_int32]
+    lib.lbm355_table_elements.restype = c_int64
+    lib.lbm355_noslip_links.argtypes = [c_int32, c_int32, c_int32, c_void_p, c_int64, c_int64, c_int64, c_void_p,
+                                        c_int64, c_void_p]
+    lib.lbm355_noslip_links.restype = c_int32
+    lib.lbm355_reduce_conserved.argtypes = [ctypes.POINTER(SweepDesc), c_void_p, c_void_p, c_void_p]
+    lib.lbm355_reduce_conserved.restype = c_int32
+    lib.lbm355_last_error.argtypes = []
+    lib.lbm355_last_error.restype = ctypes.c_char_p
+    lib.lbm355_version.argtypes = []
+    lib.lbm355_version.restype = c_int32
+    if path == library_path():
+        _lib = lib
+    return lib
+
+
+class LbmError(RuntimeError):
+    pass
+
+
+def check(rc):
+    if rc != 0:
+        raise LbmError(f"lbm355 error {rc}: {load().lbm355_last_error().decode()}")
+
+
+def current_stream_handle():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)

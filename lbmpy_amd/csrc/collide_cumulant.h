@@ -1,0 +1,292 @@
+// D3Q27 cumulant collision (lbmpy methods/cumulantbased/cumulantbasedmethod.py:296-402 with the
+// default polynomial cumulants of methods/default_moment_sets.py:106-145, CentralMoment forcing
+// forcemodels.py:213-232, optional Smagorinsky turbulence_models.py:25-84).
+//
+// The algorithm is lbmpy's (raw moments -> central moments -> cumulants -> relax -> back), but
+// every quantity is carried as its deviation from the rest-state Gaussian so that FP32 never
+// holds an O(1) constant next to an O(u^2) signal:
+//   f~        stored deviation PDFs (f - w)
+//   kt_e      central moments of f~ about u            (binomial shift of raw moments of f~)
+//   beta_e    central moments of w about u minus gamma  (closed form in u, no cancellation)
+//   mu'_e     = (kt_e + beta_e - drho*gamma_e) / rho    normalised central-moment deviation
+//   k'_e      cumulant deviations: k = gamma_k + k', gamma_k = 1/3 on the diagonal
+// where gamma_e = prod_a {1, 0, 1/3}[e_a] are the Gaussian (c_s^2 = 1/3) moments.  The
+// Gaussian has zero cumulants above second order, so the k'(mu') and mu'(k') relations carry
+// no constant terms.  Cumulant equilibria (rho * Gaussian cumulants) are exactly the
+// reference, so relaxation is C'* = (1 - r) C' for every non-conserved polynomial C.
+// First-order central moments vanish after the "force before" step (kappa_100 = j - rho u + F/2
+// = 0), which is used to drop them from the cumulant relations.
+#pragma once
+#include "lbm_common.h"
+
+namespace lbm355 {
+
+// beta_e = prod_a (gamma_a + delta_a) - prod_a gamma_a, gamma = {1, 0, 1/3}, delta = {0, -u, u^2}
+template <int E, class T>
+__device__ __forceinline__ T gam() {
+  return E == 0 ? T(1) : (E == 1 ? T(0) : T(1.0 / 3.0));
+}
+template <int E, class T>
+__device__ __forceinline__ T del(T u, T u2) {
+  return E == 0 ? T(0) : (E == 1 ? -u : u2);
+}
+template <int EX, int EY, int EZ, class T>
+__device__ __forceinline__ T beta(const T (&u)[3], const T (&u2)[3]) {
+  // (gx+dx)(gy+dy)(gz+dz) - gx gy gz = dx (gy+dy)(gz+dz) + gx [dy (gz+dz) + gy dz]
+  const T dx = del<EX, T>(u[0], u2[0]), dy = del<EY, T>(u[1], u2[1]), dz = del<EZ, T>(u[2], u2[2]);
+  const T gx = gam<EX, T>(), gy = gam<EY, T>(), gz = gam<EZ, T>();
+  const T pz = EZ == 0 ? T(1) : gz + dz;
+  const T py = EY == 0 ? T(1) : gy + dy;
+  T yz = T(0);
+  if (EY != 0) yz += dy * pz;
+  if (EZ != 0) yz += gy * dz;
+  T r = T(0);
+  if (EX != 0) r += dx * (py * pz);
+  if (EX != 1) r += gx * yz;
+  return r;
+}
+
+// raw chimera along one axis: values at c = -1, 0, +1  ->  sums of c^0, c^1, c^2
+template <class T>
+__device__ __forceinline__ void chim_fwd(T& a, T& b, T& c) {
+  const T s = a + b + c, d = c - a, t = c + a;
+  a = s;
+  b = d;
+  c = t;
+}
+// inverse: moments (m0, m1, m2) -> values at c = 0 (in b), +1 (in c), -1 (in a)
+template <class T>
+__device__ __forceinline__ void chim_bwd(T& m0, T& m1, T& m2) {
+  const T f0 = m0 - m2, fp = T(0.5) * (m2 + m1), fm = T(0.5) * (m2 - m1);
+  m0 = fm;
+  m1 = f0;
+  m2 = fp;
+}
+// central shift by -u: raw (m0,m1,m2) -> central (k0,k1,k2)
+template <class T>
+__device__ __forceinline__ void shift_fwd(T& m0, T& m1, T& m2, T u) {
+  const T k1 = m1 - u * m0;
+  const T k2 = m2 - u * (m1 + k1);
+  m1 = k1;
+  m2 = k2;
+}
+// inverse shift by +u: central (k0,k1,k2) -> raw (m0,m1,m2)
+template <class T>
+__device__ __forceinline__ void shift_bwd(T& k0, T& k1, T& k2, T u) {
+  const T m1 = k1 + u * k0;
+  const T m2 = k2 + u * (k1 + m1);
+  k1 = m1;
+  k2 = m2;
+}
+
+// cumulant-table layout: 27 per-polynomial relaxation rates in the order of the default
+// polynomial cumulant set (density, x, y, z, xy, xz, yz, x2-y2, x2-z2, x2+y2+z2, ...).
+struct CumulantTab {
+  static constexpr int RATES = 0, SIZE = 27;
+};
+
+template <class T>
+__device__ __forceinline__ void collide_cumulant_d3q27(T (&f)[27], const SweepArgs<T>& a, T& rho_out_v,
+                                                       T (&u_out_v)[3]) {
+  using S = D3Q27;
+  const T* __restrict__ r = a.tab + CumulantTab::RATES;
+  T m[3][3][3];  // [x][y][z]: first values at c=-1,0,+1, then exponents 0,1,2
+#pragma unroll
+  for (int cx = -1; cx <= 1; ++cx)
+#pragma unroll
+    for (int cy = -1; cy <= 1; ++cy)
+#pragma unroll
+      for (int cz = -1; cz <= 1; ++cz) m[cx + 1][cy + 1][cz + 1] = f[dir_index<S>(cx, cy, cz)];
+
+  // raw moments of f~ by the chimera transform (z, y, x passes)
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) chim_fwd(m[i][j][0], m[i][j][1], m[i][j][2]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) chim_fwd(m[i][0][k], m[i][1][k], m[i][2][k]);
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) chim_fwd(m[0][j][k], m[1][j][k], m[2][j][k]);
+
+  // density and velocity (conservedquantitycomputation.py:218-249, CentralMoment shift)
+  const T drho = m[0][0][0];
+  const T rho = T(1) + drho;
+  const T irho = T(1) / rho;
+  const T hF[3] = {T(0.5) * a.force[0], T(0.5) * a.force[1], T(0.5) * a.force[2]};
+  T u[3];
+  u[0] = (m[1][0][0] + hF[0]) * irho;
+  u[1] = (m[0][1][0] + hF[1]) * irho;
+  u[2] = (m[0][0][1] + hF[2]) * irho;
+  rho_out_v = rho;
+  u_out_v[0] = u[0];
+  u_out_v[1] = u[1];
+  u_out_v[2] = u[2];
+  const T u2[3] = {u[0] * u[0], u[1] * u[1], u[2] * u[2]};
+
+  // central moments of f~ about u
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) shift_fwd(m[0][j][k], m[1][j][k], m[2][j][k], u[0]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) shift_fwd(m[i][0][k], m[i][1][k], m[i][2][k], u[1]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) shift_fwd(m[i][j][0], m[i][j][1], m[i][j][2], u[2]);
+
+  // normalised central-moment deviations mu'
+#define LBM_MU(EX, EY, EZ) \
+  const T mu##EX##EY##EZ = (m[EX][EY][EZ] + beta<EX, EY, EZ>(u, u2) - drho * (gam<EX, T>() * gam<EY, T>() * gam<EZ, T>())) * irho
+  LBM_MU(2, 0, 0); LBM_MU(0, 2, 0); LBM_MU(0, 0, 2);
+  LBM_MU(1, 1, 0); LBM_MU(1, 0, 1); LBM_MU(0, 1, 1);
+  LBM_MU(1, 2, 0); LBM_MU(1, 0, 2); LBM_MU(2, 1, 0); LBM_MU(0, 1, 2); LBM_MU(2, 0, 1); LBM_MU(0, 2, 1);
+  LBM_MU(1, 1, 1);
+  LBM_MU(2, 2, 0); LBM_MU(2, 0, 2); LBM_MU(0, 2, 2);
+  LBM_MU(2, 1, 1); LBM_MU(1, 2, 1); LBM_MU(1, 1, 2);
+  LBM_MU(2, 2, 1); LBM_MU(2, 1, 2); LBM_MU(1, 2, 2);
+  LBM_MU(2, 2, 2);
+#undef LBM_MU
+  constexpr T third = T(1.0 / 3.0);
+  constexpr T ninth = T(1.0 / 9.0);
+
+  // shear relaxation rate, possibly replaced by the Smagorinsky rate
+  T rs[5] = {r[4], r[5], r[6], r[7], r[8]};
+  if (a.flags & LBM355_FLAG_SMAGORINSKY) {
+    // Pi_neq_ab = rho mu'_ab - (u_a F_b + u_b F_a) / 2  (second moment of f - f_eq)
+    const T pxx = rho * mu200 - u[0] * a.force[0];
+    const T pyy = rho * mu020 - u[1] * a.force[1];
+    const T pzz = rho * mu002 - u[2] * a.force[2];
+    const T pxy = rho * mu110 - T(0.5) * (u[0] * a.force[1] + u[1] * a.force[0]);
+    const T pxz = rho * mu101 - T(0.5) * (u[0] * a.force[2] + u[2] * a.force[0]);
+    const T pyz = rho * mu011 - T(0.5) * (u[1] * a.force[2] + u[2] * a.force[1]);
+    const T pi = sqrt(T(2) * (pxx * pxx + pyy * pyy + pzz * pzz + T(2) * (pxy * pxy + pxz * pxz + pyz * pyz))) * irho;
+    const T tau0 = T(1) / a.omega_shear;
+    const T w = T(2) / (tau0 + sqrt(a.smag_cs2x18 * pi + tau0 * tau0));
+    rs[0] = rs[1] = rs[2] = rs[3] = rs[4] = w;
+  }
+
+  // ---- cumulant deviations (second/third order equal the central moments)
+  const T k220 = mu220 - (mu020 + mu200) * third - mu020 * mu200 - T(2) * mu110 * mu110;
+  const T k202 = mu202 - (mu002 + mu200) * third - mu002 * mu200 - T(2) * mu101 * mu101;
+  const T k022 = mu022 - (mu002 + mu020) * third - mu002 * mu020 - T(2) * mu011 * mu011;
+  const T k211 = mu211 - mu011 * third - mu200 * mu011 - T(2) * mu101 * mu110;
+  const T k121 = mu121 - mu101 * third - mu020 * mu101 - T(2) * mu011 * mu110;
+  const T k112 = mu112 - mu110 * third - mu002 * mu110 - T(2) * mu011 * mu101;
+  const T k221 = mu221 - (mu021 + mu201) * third - mu020 * mu201 - mu200 * mu021 - T(4) * mu110 * mu111 -
+                 T(2) * (mu011 * mu210 + mu101 * mu120);
+  const T k212 = mu212 - (mu012 + mu210) * third - mu002 * mu210 - mu200 * mu012 - T(4) * mu101 * mu111 -
+                 T(2) * (mu011 * mu201 + mu102 * mu110);
+  const T k122 = mu122 - (mu102 + mu120) * third - mu002 * mu120 - mu020 * mu102 - T(4) * mu011 * mu111 -
+                 T(2) * (mu012 * mu110 + mu021 * mu101);
+  const T k222 = mu222 - (mu202 + mu220 + mu022) * third + (mu200 + mu020 + mu002) * ninth -
+                 (mu002 * mu220 + mu020 * mu202 + mu200 * mu022) +
+                 T(2.0 / 3.0) * (mu002 * mu020 + mu002 * mu200 + mu020 * mu200) + T(2) * mu002 * mu020 * mu200 +
+                 T(4) * (mu002 * mu110 * mu110 + mu200 * mu011 * mu011 + mu020 * mu101 * mu101) +
+                 T(4.0 / 3.0) * (mu011 * mu011 + mu101 * mu101 + mu110 * mu110) + T(16) * mu011 * mu101 * mu110 -
+                 T(4) * (mu011 * mu211 + mu101 * mu121 + mu110 * mu112) -
+                 T(2) * (mu012 * mu210 + mu021 * mu201 + mu102 * mu120) - T(4) * mu111 * mu111;
+
+  // ---- relaxation of the polynomial cumulants (equilibrium deviation is zero)
+  const T c110 = (T(1) - rs[0]) * mu110, c101 = (T(1) - rs[1]) * mu101, c011 = (T(1) - rs[2]) * mu011;
+  const T d1 = (T(1) - rs[3]) * (mu200 - mu020);
+  const T d2 = (T(1) - rs[4]) * (mu200 - mu002);
+  const T tr = (T(1) - r[9]) * (mu200 + mu020 + mu002);
+  const T c200 = (d1 + d2 + tr) * third;
+  const T c020 = (tr - T(2) * d1 + d2) * third;
+  const T c002 = (tr + d1 - T(2) * d2) * third;
+  const T ax = (T(1) - r[10]) * (mu120 + mu102), bx = (T(1) - r[13]) * (mu120 - mu102);
+  const T ay = (T(1) - r[11]) * (mu210 + mu012), by = (T(1) - r[14]) * (mu210 - mu012);
+  const T az = (T(1) - r[12]) * (mu201 + mu021), bz = (T(1) - r[15]) * (mu201 - mu021);
+  const T c120 = T(0.5) * (ax + bx), c102 = T(0.5) * (ax - bx);
+  const T c210 = T(0.5) * (ay + by), c012 = T(0.5) * (ay - by);
+  const T c201 = T(0.5) * (az + bz), c021 = T(0.5) * (az - bz);
+  const T c111 = (T(1) - r[16]) * mu111;
+  const T p1 = (T(1) - r[17]) * (k220 - T(2) * k202 + k022);
+  const T p2 = (T(1) - r[18]) * (k220 + k202 - T(2) * k022);
+  const T s4 = (T(1) - r[19]) * (k220 + k202 + k022);
+  const T c220 = (s4 + p1 + p2) * third, c202 = (s4 - p1) * third, c022 = (s4 - p2) * third;
+  const T c211 = (T(1) - r[20]) * k211, c121 = (T(1) - r[21]) * k121, c112 = (T(1) - r[22]) * k112;
+  const T c221 = (T(1) - r[23]) * k221, c212 = (T(1) - r[24]) * k212, c122 = (T(1) - r[25]) * k122;
+  const T c222 = (T(1) - r[26]) * k222;
+
+  // ---- back to normalised central-moment deviations
+  const T n220 = c220 + (c020 + c200) * third + c020 * c200 + T(2) * c110 * c110;
+  const T n202 = c202 + (c002 + c200) * third + c002 * c200 + T(2) * c101 * c101;
+  const T n022 = c022 + (c002 + c020) * third + c002 * c020 + T(2) * c011 * c011;
+  const T n211 = c211 + c011 * third + c200 * c011 + T(2) * c101 * c110;
+  const T n121 = c121 + c101 * third + c020 * c101 + T(2) * c011 * c110;
+  const T n112 = c112 + c110 * third + c002 * c110 + T(2) * c011 * c101;
+  const T n221 = c221 + (c021 + c201) * third + c020 * c201 + c200 * c021 + T(4) * c110 * c111 +
+                 T(2) * (c011 * c210 + c101 * c120);
+  const T n212 = c212 + (c012 + c210) * third + c002 * c210 + c200 * c012 + T(4) * c101 * c111 +
+                 T(2) * (c011 * c201 + c102 * c110);
+  const T n122 = c122 + (c102 + c120) * third + c002 * c120 + c020 * c102 + T(4) * c011 * c111 +
+                 T(2) * (c012 * c110 + c021 * c101);
+  const T n222 = c222 + (c202 + c220 + c022) * third + (c200 + c020 + c002) * ninth +
+                 (c002 * c220 + c020 * c202 + c200 * c022) +
+                 (c002 * c020 + c002 * c200 + c020 * c200) * third + c002 * c020 * c200 +
+                 T(2) * (c002 * c110 * c110 + c200 * c011 * c011 + c020 * c101 * c101) +
+                 T(2.0 / 3.0) * (c011 * c011 + c101 * c101 + c110 * c110) + T(8) * c011 * c101 * c110 +
+                 T(4) * (c011 * c211 + c101 * c121 + c110 * c112) +
+                 T(2) * (c012 * c210 + c021 * c201 + c102 * c120) + T(4) * c111 * c111;
+
+  // ---- central moments of the post-collision deviation f~* about u
+#define LBM_KT(EX, EY, EZ, V) \
+  m[EX][EY][EZ] = rho * (V) + drho * (gam<EX, T>() * gam<EY, T>() * gam<EZ, T>()) - beta<EX, EY, EZ>(u, u2)
+  m[0][0][0] = drho;
+  m[1][0][0] = hF[0] + u[0];
+  m[0][1][0] = hF[1] + u[1];
+  m[0][0][1] = hF[2] + u[2];
+  LBM_KT(2, 0, 0, c200); LBM_KT(0, 2, 0, c020); LBM_KT(0, 0, 2, c002);
+  LBM_KT(1, 1, 0, c110); LBM_KT(1, 0, 1, c101); LBM_KT(0, 1, 1, c011);
+  LBM_KT(1, 2, 0, c120); LBM_KT(1, 0, 2, c102); LBM_KT(2, 1, 0, c210);
+  LBM_KT(0, 1, 2, c012); LBM_KT(2, 0, 1, c201); LBM_KT(0, 2, 1, c021);
+  LBM_KT(1, 1, 1, c111);
+  LBM_KT(2, 2, 0, n220); LBM_KT(2, 0, 2, n202); LBM_KT(0, 2, 2, n022);
+  LBM_KT(2, 1, 1, n211); LBM_KT(1, 2, 1, n121); LBM_KT(1, 1, 2, n112);
+  LBM_KT(2, 2, 1, n221); LBM_KT(2, 1, 2, n212); LBM_KT(1, 2, 2, n122);
+  LBM_KT(2, 2, 2, n222);
+#undef LBM_KT
+
+  // raw moments (shift by +u), then populations (inverse chimera)
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) shift_bwd(m[i][j][0], m[i][j][1], m[i][j][2], u[2]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) shift_bwd(m[i][0][k], m[i][1][k], m[i][2][k], u[1]);
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) shift_bwd(m[0][j][k], m[1][j][k], m[2][j][k], u[0]);
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) chim_bwd(m[0][j][k], m[1][j][k], m[2][j][k]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) chim_bwd(m[i][0][k], m[i][1][k], m[i][2][k]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) chim_bwd(m[i][j][0], m[i][j][1], m[i][j][2]);
+
+#pragma unroll
+  for (int cx = -1; cx <= 1; ++cx)
+#pragma unroll
+    for (int cy = -1; cy <= 1; ++cy)
+#pragma unroll
+      for (int cz = -1; cz <= 1; ++cz) f[dir_index<S>(cx, cy, cz)] = m[cx + 1][cy + 1][cz + 1];
+}
+
+}  // namespace lbm355

@@ -1,0 +1,451 @@
+// Fused stream-collide sweep for MI355X (gfx950) and its C ABI (include/lbm355.h).
+//
+// One thread per interior cell, x fastest (a 64-lane wavefront covers 64 consecutive cells of
+// one row, so each population is read and written as contiguous 256 B (FP32) segments).
+// Blocks span up to 256 cells of a row; the grid is (row blocks, ny, nz).  Streaming:
+//   pull    read  src[i][x - c_i],        write dst[i][x]           (fieldaccess.py:66-75)
+//   AA even read  f[i][x],                write f[inv i][x]         (fieldaccess.py:136-145)
+//   AA odd  read  f[inv i][x - c_i],      write f[i][x + c_i]       (fieldaccess.py:148-157)
+// Periodic axes wrap inside the index math (PeriodicTwoFieldsAccessor semantics,
+// fieldaccess.py:90-133), NoSlip faces bounce back in place of the separate boundary kernel
+// (boundaries/boundaryconditions.py:104-139): a pull read from a wall cell returns
+// src[inv i][x]; AA odd reads src[i][x] and redirects the write to f[inv i][x].
+#include <string>
+#include <cstring>
+#include <cstdio>
+
+#include "lbm_common.h"
+#include "collide_moment.h"
+#include "collide_cumulant.h"
+
+namespace lbm355 {
+
+template <class S, int FAM, class TC>
+struct Collider;
+
+template <class S, class TC>
+struct Collider<S, LBM355_FAMILY_MOMENT, TC> {
+  static constexpr bool deviation_form = false;
+  static __device__ __forceinline__ void run(TC (&f)[S::Q], const SweepArgs<TC>& a, TC& rho, TC (&u)[3]) {
+    collide_moment<S, TC>(f, a, rho, u);
+  }
+};
+
+template <class TC>
+struct Collider<D3Q27, LBM355_FAMILY_CUMULANT, TC> {
+  static constexpr bool deviation_form = true;
+  static __device__ __forceinline__ void run(TC (&f)[27], const SweepArgs<TC>& a, TC& rho, TC (&u)[3]) {
+    collide_cumulant_d3q27<TC>(f, a, rho, u);
+  }
+};
+
+template <class S, int FAM, int PAT, bool WALLS, class TS, class TC>
+__global__ void __launch_bounds__(256) sweep_kernel(SweepArgs<TC> a) {
+  constexpr int Q = S::Q, D = S::D;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= a.nx) return;
+  const int y = blockIdx.y;
+  const int z = blockIdx.z;
+
+  const AxisNb ax = axis_neighbours(x, a.nx, a.g, a.face[0]);
+  const AxisNb ay = axis_neighbours(y, a.ny, a.g, a.face[1]);
+  AxisNb az = {0, 0, false, false};
+  if (D == 3) az = axis_neighbours(z, a.nz, a.g, a.face[2]);
+  // in-plane element offsets of the 3 x positions, 3 rows and 3 planes around the cell
+  const int sy = (int)a.sy, sz = (int)a.sz;
+  const int X[3] = {ax.lo, x + a.g, ax.hi};
+  const int Y[3] = {ay.lo * sy, (y + a.g) * sy, ay.hi * sy};
+  const int Z[3] = {D == 3 ? az.lo * sz : 0, D == 3 ? (z + a.g) * sz : 0, D == 3 ? az.hi * sz : 0};
+  const int center = X[1] + Y[1] + Z[1];
+
+  const TS* __restrict__ src = reinterpret_cast<const TS*>(a.src);
+  TS* __restrict__ dst = reinterpret_cast<TS*>(a.dst);
+  const int64_t sq = a.sq;
+
+  TC f[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const int cx = S::c[i][0], cy = S::c[i][1], cz = S::c[i][2];
+    const int ii = S::inv[i];
+    if (PAT == LBM355_AA_EVEN) {
+      f[i] = load_as<TS, TC>(src + i * sq + center);
+    } else {
+      // neighbour x - c_i
+      const int off = X[1 - cx] + Y[1 - cy] + Z[1 - cz];
+      const int plane = (PAT == LBM355_PULL) ? i : ii;
+      if (WALLS) {
+        const bool wall = (cx == 1 && ax.wall_lo) || (cx == -1 && ax.wall_hi) || (cy == 1 && ay.wall_lo) ||
+                          (cy == -1 && ay.wall_hi) || (cz == 1 && az.wall_lo) || (cz == -1 && az.wall_hi);
+        // bounce-back: pull reads f_{inv i}(x) from the previous step, AA odd finds it in slot i
+        const int wplane = (PAT == LBM355_PULL) ? ii : i;
+        const TS* p = wall ? (src + wplane * sq + center) : (src + plane * sq + off);
+        f[i] = load_as<TS, TC>(p);
+      } else {
+        f[i] = load_as<TS, TC>(src + plane * sq + off);
+      }
+    }
+  }
+
+  const bool absolute = !(a.flags & LBM355_FLAG_ZERO_CENTERED);
+  if (Collider<S, FAM, TC>::deviation_form && absolute) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] -= TC(weight<S>(i));
+  }
+
+  TC rho, u[3];
+  Collider<S, FAM, TC>::run(f, a, rho, u);
+
+  if (Collider<S, FAM, TC>::deviation_form && absolute) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] += TC(weight<S>(i));
+  }
+
+  if (a.rho_out || a.u_out) {
+    const int64_t o = (int64_t)(x + a.g) + (int64_t)(y + a.g) * a.osy + (D == 3 ? (int64_t)(z + a.g) * a.osz : 0);
+    if (a.rho_out) a.rho_out[o] = rho;
+    if (a.u_out) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) a.u_out[o + d * a.osc] = u[d];
+    }
+  }
+
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const int cx = S::c[i][0], cy = S::c[i][1], cz = S::c[i][2];
+    const int ii = S::inv[i];
+    if (PAT == LBM355_PULL) {
+      store_as<TS, TC>(dst + i * sq + center, f[i]);
+    } else if (PAT == LBM355_AA_EVEN) {
+      store_as<TS, TC>(dst + ii * sq + center, f[i]);
+    } else {
+      const int off = X[1 + cx] + Y[1 + cy] + Z[1 + cz];
+      if (WALLS) {
+        const bool wall = (cx == -1 && ax.wall_lo) || (cx == 1 && ax.wall_hi) || (cy == -1 && ay.wall_lo) ||
+                          (cy == 1 && ay.wall_hi) || (cz == -1 && az.wall_lo) || (cz == 1 && az.wall_hi);
+        TS* p = wall ? (dst + ii * sq + center) : (dst + i * sq + off);
+        store_as<TS, TC>(p, f[i]);
+      } else {
+        store_as<TS, TC>(dst + i * sq + off, f[i]);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ NoSlip index list
+template <class S, int PAT, class T>
+__global__ void noslip_links_kernel(T* pdf, int64_t sy, int64_t sz, int64_t sq, const int32_t* links, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int32_t* l = links + 4 * t;
+  const int x = l[0], y = l[1], z = l[2], d = l[3];
+  if (d < 0 || d >= S::Q) return;
+  int cx = 0, cy = 0, cz = 0, id = 0;
+#pragma unroll
+  for (int i = 0; i < S::Q; ++i)
+    if (i == d) {
+      cx = S::c[i][0];
+      cy = S::c[i][1];
+      cz = S::c[i][2];
+      id = S::inv[i];
+    }
+  const int64_t cell = x + y * sy + (S::D == 3 ? z * sz : 0);
+  const int64_t nb = (x + cx) + (y + cy) * sy + (S::D == 3 ? (z + cz) * sz : 0);
+  if (PAT == LBM355_PULL) {
+    pdf[id * sq + nb] = pdf[d * sq + cell];
+  } else if (PAT == LBM355_AA_EVEN) {
+    pdf[d * sq + nb] = pdf[id * sq + cell];
+  } else {
+    pdf[id * sq + cell] = pdf[d * sq + nb];
+  }
+}
+
+// ------------------------------------------------------------------ FP64 conserved sums
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__global__ void zero4_kernel(double* out) {
+  if (threadIdx.x < 4) out[threadIdx.x] = 0.0;
+}
+
+template <class S, class TS>
+__global__ void __launch_bounds__(256) reduce_kernel(const TS* pdf, int nx, int ny, int nz, int g, int64_t sy,
+                                                     int64_t sz, int64_t sq, int zc, double* out) {
+  constexpr int Q = S::Q;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y, z = blockIdx.z;
+  double r = 0, jx = 0, jy = 0, jz = 0;
+  if (x < nx) {
+    const int64_t c = (x + g) + (y + g) * sy + (S::D == 3 ? (z + g) * sz : 0);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const double v = (double)pdf[i * sq + c];
+      r += v;
+      if (S::c[i][0]) jx += S::c[i][0] > 0 ? v : -v;
+      if (S::c[i][1]) jy += S::c[i][1] > 0 ? v : -v;
+      if (S::c[i][2]) jz += S::c[i][2] > 0 ? v : -v;
+    }
+    if (zc) r += 1.0;
+  }
+  r = wave_sum(r);
+  jx = wave_sum(jx);
+  jy = wave_sum(jy);
+  jz = wave_sum(jz);
+  __shared__ double part[4][4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    part[wid][0] = r;
+    part[wid][1] = jx;
+    part[wid][2] = jy;
+    part[wid][3] = jz;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int nw = (blockDim.x + 63) >> 6;
+    double s = 0;
+    for (int w = 0; w < nw; ++w) s += part[w][threadIdx.x];
+    atomicAdd(out + threadIdx.x, s);
+  }
+}
+
+}  // namespace lbm355
+
+// ====================================================================== host side
+using namespace lbm355;
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+template <class TC>
+using KernelFn = void (*)(SweepArgs<TC>);
+
+struct Launch {
+  dim3 grid, block;
+};
+
+Launch geometry(int nx, int ny, int nz) {
+  int bx = nx >= 256 ? 256 : ((nx + 63) / 64) * 64;
+  Launch l;
+  l.block = dim3(bx, 1, 1);
+  l.grid = dim3((nx + bx - 1) / bx, ny, nz);
+  return l;
+}
+
+template <class S, int FAM, class TS, class TC>
+void* pick_pattern(int pat, bool walls) {
+  switch (pat) {
+    case LBM355_PULL:
+      return walls ? (void*)sweep_kernel<S, FAM, LBM355_PULL, true, TS, TC>
+                   : (void*)sweep_kernel<S, FAM, LBM355_PULL, false, TS, TC>;
+    case LBM355_AA_EVEN:
+      return (void*)sweep_kernel<S, FAM, LBM355_AA_EVEN, false, TS, TC>;
+    case LBM355_AA_ODD:
+      return walls ? (void*)sweep_kernel<S, FAM, LBM355_AA_ODD, true, TS, TC>
+                   : (void*)sweep_kernel<S, FAM, LBM355_AA_ODD, false, TS, TC>;
+  }
+  return nullptr;
+}
+
+template <class S, int FAM, class TC>
+void* pick_store(int store, int pat, bool walls) {
+  if (store == LBM355_F64) return pick_pattern<S, FAM, double, TC>(pat, walls);
+  if (store == LBM355_F32) return pick_pattern<S, FAM, float, TC>(pat, walls);
+  if (store == LBM355_F16 && sizeof(TC) == 4) return pick_pattern<S, FAM, _Float16, TC>(pat, walls);
+  return nullptr;
+}
+
+template <class TC>
+void* pick_kernel(const lbm355_sweep_desc* d, bool walls) {
+  if (d->family == LBM355_FAMILY_MOMENT) {
+    if (d->stencil == LBM355_D2Q9) return pick_store<D2Q9, LBM355_FAMILY_MOMENT, TC>(d->store_dtype, d->pattern, walls);
+    if (d->stencil == LBM355_D3Q19)
+      return pick_store<D3Q19, LBM355_FAMILY_MOMENT, TC>(d->store_dtype, d->pattern, walls);
+    if (d->stencil == LBM355_D3Q27)
+      return pick_store<D3Q27, LBM355_FAMILY_MOMENT, TC>(d->store_dtype, d->pattern, walls);
+  } else if (d->family == LBM355_FAMILY_CUMULANT) {
+    if (d->stencil == LBM355_D3Q27)
+      return pick_store<D3Q27, LBM355_FAMILY_CUMULANT, TC>(d->store_dtype, d->pattern, walls);
+  }
+  return nullptr;
+}
+
+int stencil_q(int s) { return s == LBM355_D2Q9 ? 9 : (s == LBM355_D3Q19 ? 19 : (s == LBM355_D3Q27 ? 27 : 0)); }
+int stencil_d(int s) { return s == LBM355_D2Q9 ? 2 : 3; }
+
+int validate(const lbm355_sweep_desc* d) {
+  if (!d) return fail(LBM355_ERR_ARG, "desc is NULL");
+  const int Q = stencil_q(d->stencil);
+  if (!Q) return fail(LBM355_ERR_ARG, "unknown stencil");
+  const int D = stencil_d(d->stencil);
+  if (d->nx <= 0 || d->ny <= 0 || d->nz <= 0) return fail(LBM355_ERR_ARG, "domain sizes must be positive");
+  if (D == 2 && d->nz != 1) return fail(LBM355_ERR_ARG, "2-D stencil needs nz == 1");
+  if (d->ghost_layers < 0 || d->ghost_layers > 1) return fail(LBM355_ERR_ARG, "ghost_layers must be 0 or 1");
+  if (d->ny > 65535 || d->nz > 65535) return fail(LBM355_ERR_ARG, "ny and nz must be <= 65535");
+  for (int a = 0; a < D; ++a) {
+    if (d->face[a] < 0 || d->face[a] > 2) return fail(LBM355_ERR_ARG, "bad face mode");
+    if (d->face[a] == LBM355_FACE_GHOST && d->ghost_layers < 1)
+      return fail(LBM355_ERR_ARG, "FACE_GHOST needs a ghost layer");
+  }
+  const int64_t g2 = 2 * d->ghost_layers;
+  if (d->stride_y < d->nx + g2) return fail(LBM355_ERR_ARG, "stride_y smaller than a row");
+  if (D == 3 && d->stride_z < d->stride_y * (d->ny + g2)) return fail(LBM355_ERR_ARG, "stride_z smaller than a plane");
+  const int64_t plane = D == 3 ? d->stride_z * (d->nz + g2) : d->stride_y * (d->ny + g2);
+  if (d->stride_q < plane) return fail(LBM355_ERR_ARG, "stride_q smaller than one population");
+  if (plane >= (int64_t(1) << 31)) return fail(LBM355_ERR_ARG, "one population must hold < 2^31 elements");
+  if (d->compute_dtype != LBM355_F64 && d->compute_dtype != LBM355_F32)
+    return fail(LBM355_ERR_ARG, "compute dtype must be F64 or F32");
+  if (d->store_dtype < 0 || d->store_dtype > 2) return fail(LBM355_ERR_ARG, "bad store dtype");
+  if (d->pattern < 0 || d->pattern > 2) return fail(LBM355_ERR_ARG, "bad pattern");
+  if (!d->tables) return fail(LBM355_ERR_ARG, "method table pointer is NULL");
+  if (d->family == LBM355_FAMILY_MOMENT && (d->n_active < 0 || d->n_active > Q))
+    return fail(LBM355_ERR_ARG, "n_active out of range");
+  return LBM355_OK;
+}
+
+template <class TC>
+int launch_sweep(const lbm355_sweep_desc* d, const void* src, void* dst, void* rho, void* u, hipStream_t s) {
+  bool walls = false;
+  for (int a = 0; a < stencil_d(d->stencil); ++a) walls |= d->face[a] == LBM355_FACE_NOSLIP;
+  void* k = pick_kernel<TC>(d, walls);
+  if (!k) return fail(LBM355_ERR_UNSUPPORTED, "no kernel for this stencil/family/dtype combination");
+  SweepArgs<TC> a;
+  a.src = src;
+  a.dst = dst;
+  a.rho_out = (TC*)rho;
+  a.u_out = (TC*)u;
+  a.tab = (const TC*)d->tables;
+  a.nx = d->nx;
+  a.ny = d->ny;
+  a.nz = d->nz;
+  a.g = d->ghost_layers;
+  a.sy = d->stride_y;
+  a.sz = d->stride_z;
+  a.sq = d->stride_q;
+  a.osy = d->out_stride_y;
+  a.osz = d->out_stride_z;
+  a.osc = d->out_stride_c;
+  for (int i = 0; i < 3; ++i) {
+    a.face[i] = d->face[i];
+    a.force[i] = (TC)d->force[i];
+  }
+  a.flags = d->flags;
+  a.nactive = d->n_active;
+  a.smag_cs2x18 = (TC)(18.0 * d->smagorinsky_constant * d->smagorinsky_constant);
+  a.omega_shear = (TC)d->omega_shear;
+  Launch l = geometry(d->nx, d->ny, d->nz);
+  hipLaunchKernelGGL((KernelFn<TC>)k, l.grid, l.block, 0, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(LBM355_ERR_LAUNCH, std::string("launch failed: ") + hipGetErrorString(e));
+  return LBM355_OK;
+}
+
+template <class S, class T>
+int launch_links(int pattern, void* pdf, int64_t sy, int64_t sz, int64_t sq, const int32_t* links, int64_t n,
+                 hipStream_t s) {
+  const int bs = 256;
+  const int64_t nb = (n + bs - 1) / bs;
+  if (pattern == LBM355_PULL)
+    hipLaunchKernelGGL((noslip_links_kernel<S, LBM355_PULL, T>), dim3(nb), dim3(bs), 0, s, (T*)pdf, sy, sz, sq, links, n);
+  else if (pattern == LBM355_AA_EVEN)
+    hipLaunchKernelGGL((noslip_links_kernel<S, LBM355_AA_EVEN, T>), dim3(nb), dim3(bs), 0, s, (T*)pdf, sy, sz, sq, links,
+                       n);
+  else
+    hipLaunchKernelGGL((noslip_links_kernel<S, LBM355_AA_ODD, T>), dim3(nb), dim3(bs), 0, s, (T*)pdf, sy, sz, sq, links,
+                       n);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(LBM355_ERR_LAUNCH, std::string("launch failed: ") + hipGetErrorString(e));
+  return LBM355_OK;
+}
+
+template <class S>
+int links_dtype(int dtype, int pattern, void* pdf, int64_t sy, int64_t sz, int64_t sq, const int32_t* links,
+                int64_t n, hipStream_t s) {
+  if (dtype == LBM355_F64) return launch_links<S, double>(pattern, pdf, sy, sz, sq, links, n, s);
+  if (dtype == LBM355_F32) return launch_links<S, float>(pattern, pdf, sy, sz, sq, links, n, s);
+  if (dtype == LBM355_F16) return launch_links<S, _Float16>(pattern, pdf, sy, sz, sq, links, n, s);
+  return fail(LBM355_ERR_ARG, "bad dtype");
+}
+
+template <class S, class TS>
+int launch_reduce(const lbm355_sweep_desc* d, const void* pdf, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(zero4_kernel, dim3(1), dim3(64), 0, s, out);
+  Launch l = geometry(d->nx, d->ny, d->nz);
+  hipLaunchKernelGGL((reduce_kernel<S, TS>), l.grid, l.block, 0, s, (const TS*)pdf, d->nx, d->ny, d->nz,
+                     d->ghost_layers, d->stride_y, d->stride_z, d->stride_q,
+                     (d->flags & LBM355_FLAG_ZERO_CENTERED) ? 1 : 0, out);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(LBM355_ERR_LAUNCH, std::string("launch failed: ") + hipGetErrorString(e));
+  return LBM355_OK;
+}
+
+template <class S>
+int reduce_dtype(const lbm355_sweep_desc* d, const void* pdf, double* out, hipStream_t s) {
+  if (d->store_dtype == LBM355_F64) return launch_reduce<S, double>(d, pdf, out, s);
+  if (d->store_dtype == LBM355_F32) return launch_reduce<S, float>(d, pdf, out, s);
+  if (d->store_dtype == LBM355_F16) return launch_reduce<S, _Float16>(d, pdf, out, s);
+  return fail(LBM355_ERR_ARG, "bad dtype");
+}
+}  // namespace
+
+extern "C" {
+
+int lbm355_sweep(const lbm355_sweep_desc* d, const void* pdf_src, void* pdf_dst, void* density_out,
+                 void* velocity_out, void* stream) {
+  int rc = validate(d);
+  if (rc) return rc;
+  if (!pdf_src || !pdf_dst) return fail(LBM355_ERR_ARG, "PDF pointers must not be NULL");
+  if (d->pattern != LBM355_PULL && pdf_src != pdf_dst) return fail(LBM355_ERR_ARG, "AA needs pdf_src == pdf_dst");
+  if (d->pattern == LBM355_PULL && pdf_src == pdf_dst) return fail(LBM355_ERR_ARG, "pull needs two arrays");
+  if (d->family == LBM355_FAMILY_CUMULANT && d->stencil != LBM355_D3Q27)
+    return fail(LBM355_ERR_UNSUPPORTED, "cumulant sweep is implemented for D3Q27");
+  if (d->family == LBM355_FAMILY_CENTRAL) return fail(LBM355_ERR_UNSUPPORTED, "central-moment sweep not available");
+  if ((d->flags & LBM355_FLAG_SMAGORINSKY) && d->family != LBM355_FAMILY_CUMULANT)
+    return fail(LBM355_ERR_UNSUPPORTED, "Smagorinsky is implemented for the cumulant sweep");
+  hipStream_t s = (hipStream_t)stream;
+  if (d->compute_dtype == LBM355_F64) return launch_sweep<double>(d, pdf_src, pdf_dst, density_out, velocity_out, s);
+  return launch_sweep<float>(d, pdf_src, pdf_dst, density_out, velocity_out, s);
+}
+
+int64_t lbm355_table_elements(int32_t stencil, int32_t family) {
+  const int Q = stencil_q(stencil);
+  if (!Q) return -1;
+  if (family == LBM355_FAMILY_MOMENT) return 16 * Q + 2 + 2 * Q * Q;
+  if (family == LBM355_FAMILY_CUMULANT) return CumulantTab::SIZE;
+  return -1;
+}
+
+int lbm355_noslip_links(int32_t stencil, int32_t pattern, int32_t dtype, void* pdf, int64_t stride_y,
+                        int64_t stride_z, int64_t stride_q, const int32_t* links, int64_t n_links, void* stream) {
+  if (n_links == 0) return LBM355_OK;
+  if (!pdf || !links || n_links < 0) return fail(LBM355_ERR_ARG, "bad pointer or link count");
+  if (pattern < 0 || pattern > 2) return fail(LBM355_ERR_ARG, "bad pattern");
+  hipStream_t s = (hipStream_t)stream;
+  if (stencil == LBM355_D2Q9) return links_dtype<D2Q9>(dtype, pattern, pdf, stride_y, stride_z, stride_q, links, n_links, s);
+  if (stencil == LBM355_D3Q19)
+    return links_dtype<D3Q19>(dtype, pattern, pdf, stride_y, stride_z, stride_q, links, n_links, s);
+  if (stencil == LBM355_D3Q27)
+    return links_dtype<D3Q27>(dtype, pattern, pdf, stride_y, stride_z, stride_q, links, n_links, s);
+  return fail(LBM355_ERR_ARG, "unknown stencil");
+}
+
+int lbm355_reduce_conserved(const lbm355_sweep_desc* d, const void* pdf, double* out, void* stream) {
+  if (!d || !pdf || !out) return fail(LBM355_ERR_ARG, "NULL argument");
+  hipStream_t s = (hipStream_t)stream;
+  if (d->stencil == LBM355_D2Q9) return reduce_dtype<D2Q9>(d, pdf, out, s);
+  if (d->stencil == LBM355_D3Q19) return reduce_dtype<D3Q19>(d, pdf, out, s);
+  if (d->stencil == LBM355_D3Q27) return reduce_dtype<D3Q27>(d, pdf, out, s);
+  return fail(LBM355_ERR_ARG, "unknown stencil");
+}
+
+const char* lbm355_last_error(void) { return g_err.c_str(); }
+
+int32_t lbm355_version(void) { return 1; }
+
+}  // extern "C"

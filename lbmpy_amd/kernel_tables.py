@@ -1,0 +1,229 @@
+"""Numeric method tables for the HIP sweep kernels.
+
+Turns a method description (``methods.py``) plus bound relaxation-rate values
+into the flat table a kernel family reads (layouts in ``csrc/collide_moment.h``
+and ``csrc/collide_cumulant.h``).  All derivation is exact (Fractions) until the
+relaxation rates enter; from there numpy float64 is used, and the table is
+cast to the kernel's compute type at upload.
+"""
+from fractions import Fraction
+
+import numpy as np
+
+from ._poly import Poly, MOMENT_VARS
+from .equilibrium import EQ_VARS
+from .moments import mat_inv, moment_matrix, cascaded_moment_sets_literature, get_order
+
+FAMILY_MOMENT, FAMILY_CENTRAL, FAMILY_CUMULANT = 0, 1, 2
+STENCIL_CODES = {"D2Q9": 0, "D3Q19": 1, "D3Q27": 2}
+
+
+class UnsupportedMethod(NotImplementedError):
+    pass
+
+
+def bind_rate(r, bindings):
+    """Numeric value of a relaxation rate (number or SymPy expression with free symbols)."""
+    if hasattr(r, "free_symbols") and r.free_symbols:
+        missing = [s.name for s in r.free_symbols if s.name not in bindings]
+        if missing:
+            raise ValueError(f"relaxation rate symbol(s) {missing} must be passed to the kernel call")
+        return float(r.subs({s: bindings[s.name] for s in r.free_symbols}))
+    return float(r)
+
+
+def family_of(method):
+    return {"moment": FAMILY_MOMENT, "central_moment": FAMILY_CENTRAL, "cumulant": FAMILY_CUMULANT}[method.kind]
+
+
+# ------------------------------------------------------------------------------- moment family
+
+def _storage_equilibrium(method):
+    """Per-direction equilibrium polynomials in storage form (zero-centred or absolute).
+
+    Raw-moment space with a non-delta equilibrium subtracts the background moments
+    before the backward transform (rawmomenttransforms.py:345-413); that is the same
+    as relaxing towards the delta equilibrium, so the storage form is
+    f_eq_full - w (zero-centred) or f_eq_full (absolute)."""
+    st = method.stencil
+    eq = method.equilibrium_distribution
+    M = moment_matrix(list(method.moments), st)
+    Minv = mat_inv(M)
+    meq = list(method.moment_equilibrium_values)
+    w = method.weights
+    zc = method.zero_centered_pdfs
+    if zc and not eq.deviation_only:
+        bg = [eq.background_moment(p) for p in method.moments]
+        meq = [m - b for m, b in zip(meq, bg)]
+    elif not zc and eq.deviation_only:
+        raise UnsupportedMethod("delta equilibrium requires zero-centred storage")
+    feq = []
+    for i in range(st.Q):
+        p = Poly(EQ_VARS)
+        for k in range(st.Q):
+            if Minv[i][k] != 0:
+                p = p + meq[k] * Minv[i][k]
+        feq.append(p)
+    return feq, M, Minv
+
+
+def _split_equilibrium(feq_i, compressible):
+    """f_eq = A + B*drho + lam * (L.u + Qd.u^2 + Qo.uu), lam = rho (compressible) or 1."""
+    # express rho as 1 + drho
+    rho = Poly.const(EQ_VARS, 1) + Poly.var(EQ_VARS, "drho")
+    p = feq_i.subs({"rho": rho})
+    A = p.coeff((0, 0, 0, 0, 0))
+    B = p.coeff((0, 1, 0, 0, 0))
+    lin = [p.coeff((0, 0) + tuple(int(a == d) for a in range(3))) for d in range(3)]
+    quad_d = [p.coeff((0, 0) + tuple(2 * int(a == d) for a in range(3))) for d in range(3)]
+    pairs = [(0, 1), (0, 2), (1, 2)]
+    quad_o = [p.coeff((0, 0) + tuple(int(a in pr) for a in range(3))) for pr in pairs]
+    # verify the remaining terms are exactly (drho * the u-part) for compressible, none otherwise
+    rebuilt = Poly.const(EQ_VARS, A) + Poly.var(EQ_VARS, "drho") * B
+    upart = Poly(EQ_VARS)
+    for d in range(3):
+        e = [0, 0, 0, 0, 0]
+        e[2 + d] = 1
+        upart = upart + Poly.monomial(EQ_VARS, e, lin[d])
+        e = [0, 0, 0, 0, 0]
+        e[2 + d] = 2
+        upart = upart + Poly.monomial(EQ_VARS, e, quad_d[d])
+    for (a, b), c in zip(pairs, quad_o):
+        e = [0, 0, 0, 0, 0]
+        e[2 + a] = 1
+        e[2 + b] = 1
+        upart = upart + Poly.monomial(EQ_VARS, e, c)
+    lam = rho if compressible else Poly.const(EQ_VARS, 1)
+    if not (rebuilt + lam * upart - p).is_zero():
+        raise UnsupportedMethod("equilibrium is not of the form A + B drho + lam * quadratic(u)")
+    return A, B, lin, quad_d, quad_o
+
+
+def moment_table(method, bindings, dtype=np.float64):
+    """Flat table for the moment-family kernel (layout MomentTab in collide_moment.h)."""
+    st = method.stencil
+    Q, D = st.Q, st.D
+    feq, M, Minv = _storage_equilibrium(method)
+    comp = method.compressible
+    A = np.zeros(Q)
+    B = np.zeros(Q)
+    L = np.zeros((Q, 3))
+    QD = np.zeros((Q, 3))
+    QO = np.zeros((Q, 3))
+    for i in range(Q):
+        a, b, lin, qd, qo = _split_equilibrium(feq[i], comp)
+        c = st[i] + (0,) * (3 - D)
+        for d in range(3):
+            if lin[d] != 0 and c[d] == 0:
+                raise UnsupportedMethod("equilibrium has a linear term the kernel skips")
+        for k, (x, y) in enumerate([(0, 1), (0, 2), (1, 2)]):
+            if qo[k] != 0 and (c[x] == 0 or c[y] == 0):
+                raise UnsupportedMethod("equilibrium has a cross term the kernel skips")
+        if D == 2 and (qo[1] != 0 or qo[2] != 0):
+            raise UnsupportedMethod("2-D equilibrium with z terms")
+        A[i], B[i] = float(a), float(b)
+        L[i] = [float(v) for v in lin]
+        QD[i] = [float(v) for v in qd]
+        QO[i] = [float(v) for v in qo]
+
+    rates = np.array([bind_rate(r, bindings) for r in method.relaxation_rates])
+    polys = list(method.moments)
+    if not all(p.parity_pure() for p in polys):
+        raise UnsupportedMethod("moment polynomials of mixed parity")
+    even = np.array([p.is_even() for p in polys])
+    order = np.array([get_order(p) for p in polys])
+    conserved = np.array([all(sum(e) <= 1 for e in p.terms) for p in polys])
+
+    def base(mask):
+        vals = rates[mask & ~conserved]
+        if len(vals) == 0:
+            vals = rates[mask]
+        uniq, counts = np.unique(vals, return_counts=True)
+        return float(uniq[np.argmax(counts)])
+
+    r_even, r_odd = base(even), base(~even)
+    Mf = np.array([[float(v) for v in r] for r in M])
+    Minvf = np.array([[float(v) for v in r] for r in Minv])
+    Dm = np.diag(rates)
+
+    # force: affine coefficients of the force term in u, and the constant contributions of
+    # conserved moments whose (m_eq - m) is known exactly
+    G0 = np.zeros(Q)
+    G = np.zeros((Q, 3))
+    fm = method.force_model
+    F = np.zeros(3)
+    if fm is not None:
+        F[:D] = fm.force
+        t0, tu = fm.population_terms(method)
+        t0 = np.array([float(v) for v in t0])
+        tu = np.array([[float(v) for v in r] for r in tu])
+        if type(fm).__name__ == "Guo":
+            Acorr = Minvf @ (np.eye(Q) - 0.5 * Dm) @ Mf
+            t0 = Acorr @ t0
+            tu = Acorr @ tu
+        G0 += t0
+        G[:, :D] += tu
+
+    active = []
+    for k in range(Q):
+        rb = r_even if even[k] else r_odd
+        dr = rates[k] - rb
+        if dr == 0.0:
+            continue
+        if conserved[k]:
+            # m_eq - m for a conserved moment: 0 for the density part; F/2 for momentum when the
+            # equilibrium velocity is shifted by F/(2 rho_ref)
+            lin_coef = np.zeros(3)
+            for e, cf in polys[k].terms.items():
+                if sum(e) == 1:
+                    lin_coef[e.index(1)] = float(cf)
+            dm = 0.0
+            if fm is not None and fm.equilibrium_velocity_shift:
+                dm = float(lin_coef @ (0.5 * F))
+            G0 += dr * Minvf[:, k] * dm
+        else:
+            active.append((k, dr))
+    tab = np.zeros(16 * Q + 2 + 2 * Q * Q)
+    tab[0:Q] = A
+    tab[Q:2 * Q] = B
+    tab[2 * Q:5 * Q] = L.reshape(-1)
+    tab[5 * Q:8 * Q] = QD.reshape(-1)
+    tab[8 * Q:11 * Q] = QO.reshape(-1)
+    tab[11 * Q:12 * Q] = G0
+    tab[12 * Q:15 * Q] = G.reshape(-1)
+    tab[15 * Q] = r_even
+    tab[15 * Q + 1] = r_odd
+    for n, (k, dr) in enumerate(active):
+        tab[15 * Q + 2 + n] = dr
+        tab[16 * Q + 2 + n * Q: 16 * Q + 2 + (n + 1) * Q] = Mf[k]
+        tab[16 * Q + 2 + Q * Q + n * Q: 16 * Q + 2 + Q * Q + (n + 1) * Q] = Minvf[:, k]
+    return tab.astype(dtype), len(active)
+
+
+# ------------------------------------------------------------------------------- cumulant family
+
+def cumulant_table(method, bindings, dtype=np.float64):
+    """Per-polynomial rates in the order of the default D3Q27 polynomial cumulant set."""
+    st = method.stencil
+    if st.name != "D3Q27":
+        raise UnsupportedMethod("the cumulant sweep is implemented for D3Q27")
+    reference = [p for g in cascaded_moment_sets_literature(st) for p in g]
+    polys = list(method.cumulants)
+    if polys != reference:
+        raise UnsupportedMethod("the cumulant sweep implements the default polynomial cumulant set")
+    fm = method.force_model
+    if fm is not None and type(fm).__name__ != "CentralMoment":
+        raise UnsupportedMethod("cumulant sweep supports the CentralMoment force model")
+    rates = np.array([bind_rate(r, bindings) for r in method.relaxation_rates])
+    if rates[0] != 0.0:
+        raise UnsupportedMethod("the zeroth cumulant must not be relaxed")
+    return rates.astype(dtype), 0
+
+
+def method_table(method, bindings, dtype=np.float64):
+    fam = family_of(method)
+    if fam == FAMILY_MOMENT:
+        return moment_table(method, bindings, dtype)
+    if fam == FAMILY_CUMULANT:
+        return cumulant_table(method, bindings, dtype)
+    raise UnsupportedMethod("central-moment sweep is not available on the GPU path")

@@ -1,0 +1,239 @@
+"""The compiled-sweep callable returned by ``create_lb_function``.
+
+lbmpy's ``create_lb_function`` returns a pystencils kernel that is called as
+``kernel(src=array, dst=array, omega=1.8)`` -- fields by their symbolic names,
+free parameters (symbolic relaxation rates) by symbol name
+(``creationfunctions.py:543-559``, ``lbstep.py:269,273-280``).  ``SweepKernel``
+keeps that calling convention for PyTorch ROCm tensors and launches the
+MI355X kernel through the C ABI (``include/lbm355.h``).
+
+PDF arrays are indexed like lbmpy's numpy/cupy arrays, ``arr[x, y, (z,) q]``,
+with the "fzyx" layout (x contiguous); :func:`create_pdf_array` allocates one.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from .kernel_tables import method_table, family_of, STENCIL_CODES, UnsupportedMethod, bind_rate
+from .relaxationrates import get_shear_relaxation_rate
+from .enums import SubgridScaleModel, Target
+
+TORCH_DTYPES = {"float64": torch.float64, "float32": torch.float32, "float16": torch.float16}
+
+
+def create_pdf_array(domain_size, q, ghost_layers=1, dtype="float64", device="cuda", fill=0.0):
+    """Allocate a PDF field in fzyx layout; returns the ``[x, y, (z,) q]`` view."""
+    dims = [n + 2 * ghost_layers for n in domain_size]
+    storage = torch.full([q] + dims[::-1], fill, dtype=TORCH_DTYPES[str(dtype)], device=device)
+    return storage.permute(*range(len(dims), 0, -1), 0)
+
+
+def create_field_array(domain_size, values_per_cell, ghost_layers=1, dtype="float64", device="cuda", fill=0.0):
+    """Macroscopic field (density: values_per_cell=1, velocity: D) in the same layout."""
+    arr = create_pdf_array(domain_size, values_per_cell, ghost_layers, dtype, device, fill)
+    return arr[..., 0] if values_per_cell == 1 else arr
+
+
+class UpdateRule:
+    """Stand-in for lbmpy's update rule: the method plus the streaming/kernel specification."""
+
+    def __init__(self, method, lbm_config, lbm_optimisation, config):
+        self.method = method
+        self.lbm_config = lbm_config
+        self.lbm_optimisation = lbm_optimisation
+        self.config = config
+
+    def __repr__(self):
+        c = self.lbm_config
+        return (f"UpdateRule({self.method!r}, pattern={c.streaming_pattern}, timestep={c.timestep}, "
+                f"dtype={self.config.data_type})")
+
+
+class KernelAst:
+    """Stand-in for the pystencils AST: ``compile()`` yields the GPU sweep."""
+
+    def __init__(self, update_rule):
+        self.update_rule = update_rule
+        self.method = update_rule.method
+
+    def compile(self):
+        return SweepKernel(self.update_rule)
+
+
+class SweepKernel:
+    """One stream-collide step per call, on the current torch stream (asynchronous)."""
+
+    def __init__(self, update_rule):
+        self.update_rule = update_rule
+        cfg = update_rule.config
+        lc = update_rule.lbm_config
+        lo = update_rule.lbm_optimisation
+        m = update_rule.method
+        if cfg.target != Target.GPU:
+            raise NotImplementedError("this package implements the GPU (MI355X) sweep; "
+                                      "create the kernel with CreateKernelConfig(target=Target.GPU)")
+        if lc.kernel_type != 'default_stream_collide':
+            raise NotImplementedError(f"kernel_type {lc.kernel_type!r} is not implemented on the GPU path")
+        if lc.streaming_pattern not in ("pull", "aa"):
+            raise NotImplementedError(f"streaming pattern {lc.streaming_pattern!r} is not implemented")
+        if lc.streaming_pattern == "aa" and int(lc.timestep) not in (0, 1):
+            raise ValueError(f"Invalid timestep for streaming pattern aa: {lc.timestep}")
+        if lo.field_layout not in ("fzyx", "f", "reverse_numpy", "SoA"):
+            raise NotImplementedError("the GPU sweep uses the fzyx (SoA) layout")
+        if m.stencil.name not in STENCIL_CODES or not m.stencil.is_gpu_layout():
+            raise NotImplementedError(f"stencil {m.stencil.name} ({m.stencil.ordering}) has no GPU sweep")
+        if lc.velocity_input is not None or lc.density_input is not None:
+            raise NotImplementedError("density/velocity input fields are not supported by the GPU sweep")
+        self.method = m
+        self.lbm_config = lc
+        self.config = cfg
+        self.family = family_of(m)
+        self.store = nat.DTYPE_CODES[cfg.data_type]
+        self.compute = nat.DTYPE_CODES[cfg.compute_type]
+        if self.store == nat.F16 and self.compute != nat.F32:
+            raise ValueError("FP16 storage needs FP32 arithmetic")
+        if self.compute not in (nat.F32, nat.F64):
+            raise ValueError("compute type must be float32 or float64")
+        self.ghost_layers = int(cfg.ghost_layers)
+        self.pattern = nat.PULL if lc.streaming_pattern == "pull" else (
+            nat.AA_EVEN if int(lc.timestep) == 0 else nat.AA_ODD)
+        self.faces = [nat.FACE_PERIODIC if p else nat.FACE_GHOST for p in lo.builtin_periodicity[:m.dim]]
+        self.smagorinsky = None
+        sgs = lc.subgrid_scale_model
+        if sgs is not None:
+            model, const = (sgs if isinstance(sgs, tuple) else (sgs, None))
+            if model != SubgridScaleModel.SMAGORINSKY:
+                raise NotImplementedError(f"subgrid model {model} is not implemented on the GPU path")
+            self.smagorinsky = float(const) if const is not None else 0.12
+        self._tables = {}
+        self.free_symbols = m.relaxation_rate_symbols()
+        # validate the method once with dummy bindings so unsupported configurations fail here
+        method_table(m, {s: 1.0 for s in self.free_symbols})
+        nat.load()
+
+    # ------------------------------------------------------------------ configuration
+    def set_faces(self, faces):
+        """Per-axis face treatment: 'periodic', 'noslip' or 'ghost'."""
+        codes = {"periodic": nat.FACE_PERIODIC, "noslip": nat.FACE_NOSLIP, "ghost": nat.FACE_GHOST}
+        self.faces = [codes[f] if isinstance(f, str) else int(f) for f in faces]
+        return self
+
+    def table(self, bindings, device):
+        key = (tuple(sorted(bindings.items())), str(device))
+        t = self._tables.get(key)
+        if t is None:
+            arr, nact = method_table(self.method, bindings)
+            dt = torch.float64 if self.compute == nat.F64 else torch.float32
+            t = (torch.tensor(arr, dtype=dt, device=device), nact)
+            self._tables[key] = t
+        return t
+
+    def descriptor(self, arr, bindings, out_strides=(0, 0, 0)):
+        m = self.method
+        D, Q = m.dim, m.stencil.Q
+        g = self.ghost_layers
+        if arr.dim() != D + 1 or arr.shape[-1] != Q:
+            raise ValueError(f"PDF array must be indexed [x, y{', z' if D == 3 else ''}, q] with q = {Q}")
+        if arr.stride(0) != 1:
+            raise ValueError("PDF array must be in fzyx layout (x contiguous)")
+        sizes = [arr.shape[a] - 2 * g for a in range(D)]
+        if min(sizes) <= 0:
+            raise ValueError("PDF array smaller than its ghost layers")
+        tab, nact = self.table(bindings, arr.device)
+        d = nat.SweepDesc()
+        d.stencil = STENCIL_CODES[m.stencil.name]
+        d.family = self.family
+        d.pattern = self.pattern
+        d.store_dtype = self.store
+        d.compute_dtype = self.compute
+        d.nx, d.ny = sizes[0], sizes[1]
+        d.nz = sizes[2] if D == 3 else 1
+        d.ghost_layers = g
+        for a in range(3):
+            d.face[a] = self.faces[a] if a < D else nat.FACE_PERIODIC
+        d.stride_y = arr.stride(1)
+        d.stride_z = arr.stride(2) if D == 3 else 0
+        d.stride_q = arr.stride(D)
+        d.out_stride_y, d.out_stride_z, d.out_stride_c = out_strides
+        flags = 0
+        if m.zero_centered_pdfs:
+            flags |= nat.FLAG_ZERO_CENTERED
+        if m.compressible:
+            flags |= nat.FLAG_COMPRESSIBLE
+        fm = m.force_model
+        if fm is not None:
+            flags |= nat.FLAG_FORCE
+            if fm.equilibrium_velocity_shift:
+                flags |= nat.FLAG_EQ_SHIFT
+            for a in range(D):
+                d.force[a] = float(fm.force[a])
+        if self.smagorinsky is not None:
+            flags |= nat.FLAG_SMAGORINSKY
+            d.smagorinsky_constant = self.smagorinsky
+            d.omega_shear = bind_rate(get_shear_relaxation_rate(m), bindings)
+        d.flags = flags
+        d.n_active = nact
+        d.tables = tab.data_ptr()
+        return d
+
+    # ------------------------------------------------------------------ call
+    def __call__(self, **kwargs):
+        lc = self.lbm_config
+        src = kwargs.pop(lc.field_name)
+        dst = kwargs.pop(lc.temporary_field_name, None)
+        if self.pattern == nat.PULL:
+            if dst is None:
+                raise ValueError(f"pull streaming needs the temporary field '{lc.temporary_field_name}'")
+        else:
+            dst = src
+        bindings = {}
+        for s in self.free_symbols:
+            if s not in kwargs:
+                raise ValueError(f"missing value for free symbol '{s}'")
+            bindings[s] = float(kwargs.pop(s))
+        outputs = {}
+        for q, target in lc.output.items():
+            arr = kwargs.pop(target, None) if isinstance(target, str) else target
+            if arr is not None:
+                outputs[q] = arr
+        self._check_tensor(src)
+        self._check_tensor(dst)
+        if self.pattern == nat.PULL and src.shape != dst.shape:
+            raise ValueError("src and dst must have the same shape")
+        if self.pattern == nat.PULL and src.stride() != dst.stride():
+            raise ValueError("src and dst must have the same strides")
+        rho_o = outputs.get("density")
+        u_o = outputs.get("velocity")
+        out_strides = (0, 0, 0)
+        if rho_o is not None or u_o is not None:
+            ref = rho_o if rho_o is not None else u_o
+            D = self.method.dim
+            ct = torch.float64 if self.compute == nat.F64 else torch.float32
+            for o in (rho_o, u_o):
+                if o is not None and (o.dtype != ct or o.stride(0) != 1 or o.device != src.device):
+                    raise ValueError(f"output fields must be {ct} fzyx tensors on {src.device}")
+            if rho_o is not None and u_o is not None and rho_o.stride()[:D] != u_o.stride()[:D]:
+                raise ValueError("density and velocity outputs must share the spatial strides")
+            out_strides = (ref.stride(1), ref.stride(2) if D == 3 else 0, u_o.stride(D) if u_o is not None else 0)
+        d = self.descriptor(src, bindings, out_strides)
+        lib = nat.load()
+        nat.check(lib.lbm355_sweep(ctypes.byref(d), ctypes.c_void_p(src.data_ptr()),
+                                   ctypes.c_void_p(dst.data_ptr()),
+                                   ctypes.c_void_p(rho_o.data_ptr() if rho_o is not None else 0),
+                                   ctypes.c_void_p(u_o.data_ptr() if u_o is not None else 0),
+                                   nat.current_stream_handle()))
+
+    def _check_tensor(self, t):
+        if not isinstance(t, torch.Tensor) or not t.is_cuda:
+            raise TypeError("PDF fields must be torch tensors on the GPU")
+        if t.dtype != TORCH_DTYPES[self.config.data_type]:
+            raise TypeError(f"PDF field dtype {t.dtype} != kernel data type {self.config.data_type}")
+
+    @property
+    def parameters(self):
+        return list(self.free_symbols)
+
+    def __repr__(self):
+        return f"SweepKernel({self.update_rule!r})"

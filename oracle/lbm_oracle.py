@@ -1,0 +1,565 @@
+"""CPU FP64 oracle for the fused stream-collide sweep -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline
+leg may import this module; the product path (``lbmpy_amd``) never does.
+
+It restates lbmpy's sweep generically, in the matrix / generating-function
+form in which lbmpy derives it, vectorised over cells with numpy:
+
+* density/velocity: ``methods/conservedquantitycomputation.py:218-249``;
+* moment methods (SRT/TRT in population space, MRT in raw-moment space):
+  ``methods/momentbased/momentbasedmethod.py:320-416`` --
+  f* = f + M^-1 D (m_eq - M f) + force, or m* = m + D (m_eq - m) + force_m;
+* central-moment methods: ``methods/momentbased/centralmomentbasedmethod.py:305-394``
+  with the binomial shift of ``moment_transforms/centralmomenttransforms.py:168-483``;
+* cumulant methods: ``methods/cumulantbased/cumulantbasedmethod.py:296-402`` with
+  the cumulant generating function of ``moment_transforms/cumulanttransforms.py:46-220``
+  evaluated by the Leibniz recursion on K' = C' K (no hand-expanded formulas);
+* force models: ``forcemodels.py:179-316`` (Simple, Luo, Guo, CentralMoment);
+* streaming: pull ``fieldaccess.py:66-75`` and AA ``fieldaccess.py:136-157`` on
+  arrays with one ghost layer, periodic ghost copies (``lbstep.py:148-151``)
+  and the NoSlip index-list boundary kernel
+  (``boundaries/boundaryconditions.py:104-139``, ``boundaries/boundaryhandling.py:159-199``)
+  executed between steps exactly as ``LatticeBoltzmannStep.time_step`` orders
+  them (``lbstep.py:271-282``).
+
+The method description (stencil, moment polynomials, relaxation rates,
+equilibrium values, weights) is taken from ``lbmpy_amd.methods`` -- the same
+numeric LBMethod the kernels are parameterised from; everything downstream of
+it is computed here independently.
+"""
+from fractions import Fraction
+from itertools import product
+from math import comb
+
+import numpy as np
+
+from lbmpy_amd.moments import moment_matrix, evaluate_moment, monomial_sort_key
+from lbmpy_amd.equilibrium import EQ_VARS
+
+
+# ------------------------------------------------------------------------ helpers
+
+def _rate(r, bindings):
+    if hasattr(r, "free_symbols"):
+        return float(r.subs({s: bindings[s.name] for s in r.free_symbols}))
+    return float(r)
+
+
+def _eval_eq_poly(p, rho, drho, u):
+    """Evaluate a polynomial in (rho, drho, u0, u1, u2) on cell arrays."""
+    vals = {"rho": rho, "drho": drho, "u0": u[0], "u1": u[1] if len(u) > 1 else 0.0,
+            "u2": u[2] if len(u) > 2 else 0.0}
+    out = np.zeros_like(rho)
+    for e, c in p.terms.items():
+        t = float(c)
+        term = np.full_like(rho, t)
+        for v, k in zip(EQ_VARS, e):
+            if k:
+                term = term * vals[v] ** k
+        out = out + term
+    return out
+
+
+def _mono_exponents(polys, dim):
+    monos = set()
+    for p in polys:
+        for e in p.terms:
+            monos.add(tuple(e[:dim]) + (0,) * (3 - dim))
+    return sorted(monos, key=monomial_sort_key)
+
+
+class OracleCollision:
+    """Collision operator of one method on arrays shaped (Q, ncells)."""
+
+    def __init__(self, method, bindings=None):
+        self.m = method
+        self.bind = dict(bindings or {})
+        self.stencil = method.stencil
+        self.Q = method.stencil.Q
+        self.D = method.stencil.D
+        self.c = np.array(method.stencil.stencil_entries, dtype=np.float64)          # (Q, D)
+        self.w = np.array([float(w) for w in method.weights])
+        self.rates = np.array([_rate(r, self.bind) for r in method.relaxation_rates])
+        self.polys = list(method.relaxation_rate_dict.keys())
+        fm = method.force_model
+        self.F = np.array(fm.force if fm is not None else (0.0,) * self.D, dtype=np.float64)
+        self.fm = fm
+        self.cqc = method.conserved_quantity_computation
+        self.eq = method.equilibrium_distribution
+        if method.kind == "moment":
+            self._setup_moment()
+        else:
+            self._setup_central()
+
+    # ---- density / velocity (CQC) -----------------------------------------------------
+    def macroscopic(self, f):
+        s = f.sum(axis=0)
+        zc = self.m.zero_centered_pdfs
+        rho = 1.0 + s if zc else s
+        drho = s if zc else s - 1.0
+        j = self.c.T @ f                                           # (D, n)
+        ref = rho if self.m.compressible else np.ones_like(rho)
+        u = j / ref
+        u_eq = u.copy()
+        u_out = u.copy()
+        if self.fm is not None:
+            shift = self.F[:, None] / (2.0 * ref[None, :])
+            if self.fm.equilibrium_velocity_shift:
+                u_eq = u + shift
+            u_out = u + shift
+        return rho, drho, u_eq, u_out
+
+    def _luo(self, u):
+        """Luo term w_i [F.(c-u)/cs2 + (c.F)(c.u)/cs2^2] per direction (forcemodels.py:235-256)."""
+        cs2 = 1.0 / 3.0
+        cF = self.c @ self.F                                       # (Q,)
+        cu = self.c @ u                                            # (Q, n)
+        Fu = self.F @ u                                            # (n,)
+        return self.w[:, None] * ((cF[:, None] - Fu[None, :]) / cs2 + cF[:, None] * cu / cs2 ** 2)
+
+    def _simple(self, n):
+        cs2 = 1.0 / 3.0
+        return np.repeat((self.w * (self.c @ self.F) / cs2)[:, None], n, axis=1)
+
+    def _pop_force_vector(self, u):
+        name = type(self.fm).__name__
+        if name == "Luo" or name == "Guo":
+            return self._luo(u)
+        if name == "Simple":
+            return self._simple(u.shape[1])
+        raise NotImplementedError(name)
+
+    # ---- moment methods ---------------------------------------------------------------
+    def _setup_moment(self):
+        M = moment_matrix(self.polys, self.stencil)
+        self.M = np.array([[float(v) for v in r] for r in M])
+        self.Minv = np.linalg.inv(self.M)
+        self.Dm = np.diag(self.rates)
+        self.meq_polys = list(self.m.moment_equilibrium_values)
+        self.raw_space = self.m.collision_space.name == "RAW_MOMENTS"
+        zc = self.m.zero_centered_pdfs
+        self.bg = None
+        if zc and not self.eq.deviation_only:
+            self.bg = np.array([float(self.eq.background_moment(p)) for p in self.polys])
+
+    def _collide_moment(self, f):
+        rho, drho, u, u_out = self.macroscopic(f)
+        n = f.shape[1]
+        meq = np.stack([_eval_eq_poly(p, rho, drho, list(u)) for p in self.meq_polys])
+        if self.raw_space:
+            m = self.M @ f
+            if self.bg is not None:
+                m = m + self.bg[:, None]
+            mpost = m + self.Dm @ (meq - m)
+            if self.fm is not None:
+                name = type(self.fm).__name__
+                fvec = self._pop_force_vector(u)
+                if name == "Guo":
+                    mpost = mpost + (np.eye(self.Q) - 0.5 * self.Dm) @ (self.M @ fvec)
+                else:
+                    mpost = mpost + self.M @ fvec
+            if self.bg is not None:
+                mpost = mpost - self.bg[:, None]
+            fpost = self.Minv @ mpost
+        else:
+            fpost = f + self.Minv @ (self.Dm @ (meq - self.M @ f))
+            if self.fm is not None:
+                name = type(self.fm).__name__
+                fvec = self._pop_force_vector(u)
+                if name == "Guo":
+                    if len(set(self.rates.tolist())) == 1:
+                        fvec = (1.0 - self.rates[0] / 2.0) * fvec
+                    else:
+                        fvec = self.Minv @ ((np.eye(self.Q) - 0.5 * self.Dm) @ (self.M @ fvec))
+                fpost = fpost + fvec
+        return fpost, rho, u_out
+
+    # ---- central-moment and cumulant methods ------------------------------------------
+    def _setup_central(self):
+        D = self.D
+        self.monos = _mono_exponents(self.polys, D)
+        if len(self.monos) != self.Q:
+            raise NotImplementedError("de-aliasing of central-moment polynomials is not restated")
+        Mm = moment_matrix(self.monos, self.stencil)
+        self.Mmono = np.array([[float(v) for v in r] for r in Mm])
+        self.Mmono_inv = np.linalg.inv(self.Mmono)
+        idx = {e: i for i, e in enumerate(self.monos)}
+        self.midx = idx
+        P = np.zeros((self.Q, self.Q))
+        for i, p in enumerate(self.polys):
+            for e, cf in p.terms.items():
+                P[i, idx[tuple(e[:D]) + (0,) * (3 - D)]] = float(cf)
+        self.P = P
+        self.Pinv = np.linalg.inv(P)
+        self.bg = None
+        if self.m.zero_centered_pdfs:
+            if self.m.kind == "cumulant" or not self.eq.deviation_only:
+                self.bg = np.array([_background_monomial(e, D) for e in self.monos])
+        # binomial shift pairs: kappa_e = sum_{r<=e} prod C(e_a,r_a) s_a^(e_a-r_a) m_r
+        self.shift_pairs = []
+        for e in self.monos:
+            for r in product(*(range(k + 1) for k in e)):
+                self.shift_pairs.append((idx[e], idx[r], tuple(comb(e[a], r[a]) for a in range(3)),
+                                         tuple(e[a] - r[a] for a in range(3))))
+        if self.m.kind == "central_moment":
+            self.keq_polys = list(self.m.moment_equilibrium_values)
+        else:
+            self.ceq_polys = list(self.m.cumulant_equilibrium_values)
+
+    def _shift(self, m, s):
+        """Binomial shift of monomial moments by velocity s (s=-u: raw->central, s=+u: central->raw)."""
+        out = np.zeros_like(m)
+        for i, r, binom, pw in self.shift_pairs:
+            coef = float(binom[0] * binom[1] * binom[2])
+            t = coef * m[r]
+            for a in range(3):
+                if pw[a]:
+                    t = t * s[a] ** pw[a]
+            out[i] = out[i] + t
+        return out
+
+    def _first_order_indices(self):
+        out = []
+        for a in range(self.D):
+            e = [0, 0, 0]
+            e[a] = 1
+            out.append(self.midx[tuple(e)])
+        return out
+
+    def _collide_central(self, f):
+        rho, drho, u, u_out = self.macroscopic(f)
+        n = f.shape[1]
+        u3 = [u[a] if a < self.D else np.zeros(n) for a in range(3)]
+        m = self.Mmono @ f
+        if self.bg is not None:
+            m = m + self.bg[:, None]
+        kappa = self._shift(m, [-x for x in u3])
+        fo = self._first_order_indices()
+        if self.m.kind == "central_moment":
+            K = self.P @ kappa
+            keq = np.stack([_eval_eq_poly(p, rho, drho, list(u)) for p in self.keq_polys])
+            Kp = K + np.diag(self.rates) @ (keq - K)
+            if self.fm is not None:
+                name = type(self.fm).__name__
+                fvec = self._pop_force_vector(u)
+                # central moments of the force vector about u: polynomial central moments of fvec
+                fk = self.P @ self._shift(self.Mmono @ fvec, [-x for x in u3])
+                if name == "Guo":
+                    fk = (np.eye(self.Q) - 0.5 * np.diag(self.rates)) @ fk
+                Kp = Kp + fk
+            kpost = self.Pinv @ Kp
+        else:
+            if self.fm is not None:
+                for a, i in enumerate(fo):
+                    kappa[i] = kappa[i] + 0.5 * self.F[a]
+            c = self._cumulants_from_central(kappa, rho, u3)
+            C = self.P @ c
+            ceq = np.zeros_like(C)
+            for i, p in enumerate(self.ceq_polys):
+                if p is not None:
+                    ceq[i] = _eval_eq_poly(p, rho, drho, list(u))
+            rates = self.rates
+            for i, p in enumerate(self.ceq_polys):
+                if p is None:
+                    assert rates[i] == 0.0, "the zeroth cumulant must not be relaxed"
+            Cp = C + rates[:, None] * (ceq - C)
+            cpost = self.Pinv @ Cp
+            kpost = self._central_from_cumulants(cpost, rho, u3, kappa[self.midx[(0, 0, 0)]])
+            if self.fm is not None:
+                for a, i in enumerate(fo):
+                    kpost[i] = kpost[i] + 0.5 * self.F[a]
+        mpost = self._shift(kpost, u3)
+        if self.bg is not None:
+            mpost = mpost - self.bg[:, None]
+        return self.Mmono_inv @ mpost, rho, u_out
+
+    def _leibniz_terms(self, e):
+        """Pick j with e_j > 0; return (j, list of (binomial, r)) for r <= e - e_j."""
+        j = next(a for a in range(3) if e[a] > 0)
+        em = list(e)
+        em[j] -= 1
+        em = tuple(em)
+        terms = []
+        for r in product(*(range(k + 1) for k in em)):
+            b = comb(em[0], r[0]) * comb(em[1], r[1]) * comb(em[2], r[2])
+            rj = list(r)
+            rj[j] += 1
+            rest = tuple(em[a] - r[a] for a in range(3))
+            terms.append((b, tuple(rj), rest))
+        return terms
+
+    def _cumulants_from_central(self, kappa, rho, u3):
+        """c_e = rho * d^e [Xi.u + log K(Xi)] at 0 (cumulanttransforms.py:157-181)."""
+        order = sorted(self.monos, key=lambda e: sum(e))
+        L = {}
+        k0 = kappa[self.midx[(0, 0, 0)]]
+        for e in order:
+            if sum(e) == 0:
+                continue
+            acc = kappa[self.midx[e]].copy()
+            for b, rj, rest in self._leibniz_terms(e):
+                if rj == e:
+                    continue
+                acc = acc - b * L[rj] * kappa[self.midx[rest]]
+            L[e] = acc / k0
+        c = np.zeros_like(kappa)
+        for e in self.monos:
+            i = self.midx[e]
+            if sum(e) == 0:
+                c[i] = rho * np.log(k0)
+            elif sum(e) == 1:
+                a = e.index(1)
+                c[i] = rho * (u3[a] + L[e])
+            else:
+                c[i] = rho * L[e]
+        return c
+
+    def _central_from_cumulants(self, cpost, rho, u3, k0):
+        """kappa_e = d^e exp(C(Xi) - Xi.u) at 0 with c -> c/rho (cumulanttransforms.py:183-214);
+        exp(c_000/rho) = exp(log kappa_000) = kappa_000."""
+        order = sorted(self.monos, key=lambda e: sum(e))
+        L = {}
+        for e in self.monos:
+            i = self.midx[e]
+            if sum(e) == 1:
+                a = e.index(1)
+                L[e] = cpost[i] / rho - u3[a]
+            elif sum(e) > 1:
+                L[e] = cpost[i] / rho
+        K = {(0, 0, 0): k0}
+        for e in order:
+            if sum(e) == 0:
+                continue
+            acc = np.zeros_like(k0)
+            for b, rj, rest in self._leibniz_terms(e):
+                acc = acc + b * L[rj] * K[rest]
+            K[e] = acc
+        out = np.zeros_like(cpost)
+        for e in self.monos:
+            out[self.midx[e]] = K[e]
+        return out
+
+    def __call__(self, f):
+        if self.m.kind == "moment":
+            return self._collide_moment(f)
+        return self._collide_central(f)
+
+
+def _background_monomial(e, dim):
+    """Raw monomial moment of the rest state rho=1, u=0 (Gaussian with cs2=1/3)."""
+    v = 1.0
+    for a in range(dim):
+        k = e[a]
+        if k % 2:
+            return 0.0
+        v *= (1.0 / 3.0) ** (k // 2) * (1 if k < 4 else 3)
+    return v
+
+
+# ------------------------------------------------------------------------ streaming driver
+
+class OracleDomain:
+    """A domain with one ghost layer, stepping the way ``LatticeBoltzmannStep`` does.
+
+    ``faces`` maps axis -> 'periodic' | 'noslip' | 'ghost' (both sides of the axis).
+    Arrays are numpy float64 with shape (Q, nz+2, ny+2, nx+2) for 3-D and
+    (Q, ny+2, nx+2) for 2-D (the ``fzyx`` layout)."""
+
+    def __init__(self, method, shape, faces=None, pattern="pull", bindings=None):
+        self.coll = OracleCollision(method, bindings)
+        self.method = method
+        self.D = method.stencil.D
+        self.Q = method.stencil.Q
+        self.shape = tuple(shape)                       # (nx, ny[, nz])
+        self.faces = faces or {a: "periodic" for a in range(self.D)}
+        self.pattern = pattern
+        self.c = np.array(method.stencil.stencil_entries, dtype=int)
+        self.inv = np.array(method.stencil.inverse_indices())
+        gshape = tuple(n + 2 for n in reversed(self.shape))
+        self.src = np.zeros((self.Q,) + gshape)
+        self.dst = np.zeros_like(self.src) if pattern == "pull" else None
+        self.aa_parity = 0            # number of executed AA steps mod 2 (0: next kernel is ODD)
+        self.rho_out = None
+        self.u_out = None
+
+    # numpy axis for lattice axis a (arrays are [q, z, y, x])
+    def _ax(self, a):
+        return self.D - a
+
+    def interior(self, arr=None):
+        arr = self.src if arr is None else arr
+        sl = (slice(None),) + (slice(1, -1),) * self.D
+        return arr[sl]
+
+    def _sl(self, a, idx):
+        s = [slice(None)] * (self.D + 1)
+        s[self._ax(a)] = idx
+        return tuple(s)
+
+    def sync_periodic(self, arr):
+        for a in range(self.D):
+            if self.faces.get(a) == "periodic":
+                arr[self._sl(a, 0)] = arr[self._sl(a, -2)]
+                arr[self._sl(a, -1)] = arr[self._sl(a, 1)]
+
+    def wall_links(self):
+        """Index list (x, y[, z], dir) in ghost coordinates of fluid cells whose neighbour in
+        direction dir is a wall cell (what pystencils' index-list creation yields for a NoSlip
+        flag on the ghost layer)."""
+        if hasattr(self, "_links"):
+            return self._links
+        n = self.shape
+        walls = [a for a in range(self.D) if self.faces.get(a) == "noslip"]
+        grids = np.meshgrid(*[np.arange(1, n[a] + 1) for a in range(self.D)], indexing="ij")
+        cells = np.stack([g.reshape(-1) for g in grids], axis=1)            # (ncells, D)
+        rows = []
+        if walls:
+            for d in range(self.Q):
+                nb = cells + self.c[d][None, :]
+                hit = np.zeros(len(cells), dtype=bool)
+                for a in walls:
+                    hit |= (nb[:, a] == 0) | (nb[:, a] == n[a] + 1)
+                sel = cells[hit]
+                rows.append(np.concatenate([sel, np.full((len(sel), 1), d)], axis=1))
+        self._links = np.concatenate(rows) if rows else np.zeros((0, self.D + 1), dtype=int)
+        return self._links
+
+    def noslip_pull(self, arr):
+        """f_in(inv(d)) at the wall neighbour := f_out(d) at the fluid cell (boundaryconditions.py:138)."""
+        links = self.wall_links()
+        if len(links) == 0:
+            return
+        d = links[:, self.D]
+        cell = [links[:, a] for a in range(self.D)]
+        nb = [links[:, a] + self.c[d, a] for a in range(self.D)]
+        arr[(self.inv[d],) + tuple(reversed(nb))] = arr[(d,) + tuple(reversed(cell))]
+
+    def _neighbor_view(self, arr, q, off):
+        """arr[q] at (x + off) over the interior."""
+        s = [q]
+        for a in reversed(range(self.D)):
+            lo = 1 + off[a]
+            hi = lo + self.shape[a]
+            s.append(slice(lo, hi))
+        return arr[tuple(s)]
+
+    def _collide_cells(self, fpre):
+        shape = fpre.shape[1:]
+        fpost, rho, u_out = self.coll(fpre.reshape(self.Q, -1))
+        self.rho_out = rho.reshape(shape)
+        self.u_out = u_out.reshape((self.D,) + shape)
+        return fpost.reshape((self.Q,) + shape)
+
+    def step(self):
+        if self.pattern == "pull":
+            self.sync_periodic(self.src)
+            self.noslip_pull(self.src)
+            fpre = np.stack([self._neighbor_view(self.src, i, -self.c[i]) for i in range(self.Q)])
+            fpost = self._collide_cells(fpre)
+            self.interior(self.dst)[...] = fpost
+            self.src, self.dst = self.dst, self.src
+        else:
+            if any(self.faces.get(a) != "periodic" for a in range(self.D)):
+                raise NotImplementedError("the AA oracle covers fully periodic domains")
+            if self.aa_parity == 0:
+                # ODD kernel: read f[x - c_i](inv i), write f[x + c_i](i)
+                self.sync_periodic(self.src)
+                fpre = np.stack([self._neighbor_view(self.src, self.inv[i], -self.c[i]) for i in range(self.Q)])
+                fpost = self._collide_cells(fpre)
+                for i in range(self.Q):
+                    self._neighbor_view(self.src, i, self.c[i])[...] = fpost[i]
+                self._fold_ghost_writes()
+            else:
+                # EVEN kernel: read f[x](i), write f[x](inv i)
+                fpre = self.interior().copy()
+                fpost = self._collide_cells(fpre)
+                out = self.interior()
+                for i in range(self.Q):
+                    out[self.inv[i]] = fpost[i]
+            self.aa_parity ^= 1
+
+    def _fold_ghost_writes(self):
+        """Values the ODD kernel wrote into ghost cells belong to the periodic images
+        (the in-place periodicity handling of ``advanced_streaming/communication.py``)."""
+        for i in range(self.Q):
+            ci = self.c[i]
+            arr = self.src[i]
+            # wrap each axis in turn; edges and corners are handled by the repeated wrapping
+            for a in range(self.D):
+                ax = self._ax(a) - 1
+                if ci[a] == 1:
+                    src_sl = [slice(None)] * self.D
+                    dst_sl = [slice(None)] * self.D
+                    src_sl[ax] = -1
+                    dst_sl[ax] = 1
+                    arr[tuple(dst_sl)] = arr[tuple(src_sl)]
+                elif ci[a] == -1:
+                    src_sl = [slice(None)] * self.D
+                    dst_sl = [slice(None)] * self.D
+                    src_sl[ax] = 0
+                    dst_sl[ax] = -2
+                    arr[tuple(dst_sl)] = arr[tuple(src_sl)]
+
+    # ---- setup / readout ----------------------------------------------------------------
+    def set_pdfs_interior(self, f):
+        self.interior()[...] = f
+
+    def pdfs_interior(self):
+        return self.interior().copy()
+
+    def pdfs_in_natural_layout(self):
+        """Interior PDFs in pull layout (slot i holds f_i); for AA after an EVEN kernel the
+        slots hold swapped values and are read back through the accessor."""
+        f = self.pdfs_interior()
+        if self.pattern == "aa" and self.aa_parity == 0:
+            # last kernel was EVEN (or initialisation): slot inv(i) holds f_i
+            return f[self.inv]
+        return f
+
+
+def equilibrium_pdfs(method, rho, u, bindings=None):
+    """Equilibrium populations for given (rho, u) fields, in the stored format
+    (``macroscopic_value_kernels.py:29-45`` via ``conservedquantitycomputation.py:251-272``):
+    the equilibrium velocity is u - F/(2 rho_ref) and collision with all rates 1
+    maps any state with these moments to the equilibrium."""
+    coll = OracleCollision(method, bindings)
+    shape = rho.shape
+    rho_f = rho.reshape(-1).astype(np.float64)
+    u_f = u.reshape(u.shape[0], -1).astype(np.float64)
+    ref = rho_f if method.compressible else np.ones_like(rho_f)
+    if method.force_model is not None:
+        u_f = u_f - coll.F[:, None] / (2.0 * ref[None, :])
+    drho = rho_f - 1.0
+    if method.kind == "moment":
+        meq = np.stack([_eval_eq_poly(p, rho_f, drho, list(u_f)) for p in coll.meq_polys])
+        if coll.bg is not None:
+            meq = meq - coll.bg[:, None]
+        feq = coll.Minv @ meq
+    else:
+        # continuous Maxwellian central moments (rho * Gaussian, truncated as in the method),
+        # shifted to raw moments and inverted: the equilibrium of the central/cumulant method
+        n = rho_f.shape[0]
+        u3 = [u_f[a] if a < coll.D else np.zeros(n) for a in range(3)]
+        eq = method.equilibrium_distribution
+        if method.kind == "cumulant":
+            keq = np.stack([_gauss_central(e, rho_f, coll.D) for e in coll.monos])
+        else:
+            from lbmpy_amd._poly import moment_poly
+            keq = np.stack([_eval_eq_poly(eq.monomial_central_moment(e), rho_f, drho, list(u_f))
+                            for e in coll.monos])
+        m = coll._shift(keq, u3)
+        if coll.bg is not None:
+            m = m - coll.bg[:, None]
+        feq = coll.Mmono_inv @ m
+    return feq.reshape((coll.Q,) + shape)
+
+
+def _gauss_central(e, rho, dim):
+    v = rho.copy()
+    for a in range(dim):
+        k = e[a]
+        if k % 2:
+            return np.zeros_like(rho)
+        v = v * (1.0 / 3.0) ** (k // 2) * (1 if k < 4 else 3)
+    return v
