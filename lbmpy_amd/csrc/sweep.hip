@@ -39,11 +39,36 @@ struct Collider<D3Q27, LBM355_FAMILY_CUMULANT, TC> {
   }
 };
 
-template <class S, int FAM, int PAT, bool WALLS, class TS, class TC>
-__global__ void __launch_bounds__(256) sweep_kernel(SweepArgs<TC> a) {
+// number of directions with a non-zero x component (those are shifted through LDS)
+template <class S>
+__host__ __device__ constexpr int n_xshift() {
+  int n = 0;
+  for (int i = 0; i < S::Q; ++i) n += S::c[i][0] != 0;
+  return n;
+}
+
+// Non-temporal accesses for aligned, streamed-once data (each PDF is read and written exactly
+// once per sweep); misaligned (x +- 1) accesses keep the default policy so that the partially
+// used cache lines are served from cache to the neighbouring wavefront.
+template <class TS, class TC>
+__device__ __forceinline__ TC load_nt(const TS* p) {
+  return (TC)__builtin_nontemporal_load(p);
+}
+template <class TS, class TC>
+__device__ __forceinline__ void store_nt(TS* p, TC v) {
+  __builtin_nontemporal_store((TS)v, p);
+}
+
+// ROWLDS: the block spans one whole row and x is periodic (blockDim.x == nx).  The +-1
+// x-shifts of pull / AA-odd streaming are then done in LDS, so every global load and store
+// is an aligned, contiguous, non-temporal 64-lane access:
+//   read : tile[j][x] = row(y - c_y, z - c_z)[x];  f_i(x) = tile[j][(x - c_x) mod nx]
+//   write: tile[j][x] = f_i(x);  row(y + c_y, z + c_z)[x] = tile[j][(x - c_x) mod nx]
+template <class S, int FAM, int PAT, bool WALLS, bool ROWLDS, class TS, class TC>
+__global__ void __launch_bounds__(512) sweep_kernel(SweepArgs<TC> a) {
   constexpr int Q = S::Q, D = S::D;
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= a.nx) return;
+  if (!ROWLDS && x >= a.nx) return;
   const int y = blockIdx.y;
   const int z = blockIdx.z;
 
@@ -57,32 +82,64 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs<TC> a) {
   const int Y[3] = {ay.lo * sy, (y + a.g) * sy, ay.hi * sy};
   const int Z[3] = {D == 3 ? az.lo * sz : 0, D == 3 ? (z + a.g) * sz : 0, D == 3 ? az.hi * sz : 0};
   const int center = X[1] + Y[1] + Z[1];
+  // LDS row positions of x - 1 and x + 1 (periodic)
+  const int XL[3] = {ax.lo - a.g, x, ax.hi - a.g};
 
   const TS* __restrict__ src = reinterpret_cast<const TS*>(a.src);
   TS* __restrict__ dst = reinterpret_cast<TS*>(a.dst);
   const int64_t sq = a.sq;
+  extern __shared__ unsigned char lds_raw[];
+  TS* tile = reinterpret_cast<TS*>(lds_raw);
+  const int nx = a.nx;
 
   TC f[Q];
+  if (PAT == LBM355_AA_EVEN) {
 #pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    const int cx = S::c[i][0], cy = S::c[i][1], cz = S::c[i][2];
-    const int ii = S::inv[i];
-    if (PAT == LBM355_AA_EVEN) {
-      f[i] = load_as<TS, TC>(src + i * sq + center);
-    } else {
-      // neighbour x - c_i
-      const int off = X[1 - cx] + Y[1 - cy] + Z[1 - cz];
+    for (int i = 0; i < Q; ++i) f[i] = load_nt<TS, TC>(src + i * sq + center);
+  } else {
+    // pull reads src[i] at x - c_i; AA odd reads slot inv(i) there.  At a wall the
+    // bounce-back value sits in the cell itself: pull slot inv(i), AA odd slot i.
+    int j = 0;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const int cx = S::c[i][0], cy = S::c[i][1], cz = S::c[i][2];
+      const int ii = S::inv[i];
       const int plane = (PAT == LBM355_PULL) ? i : ii;
-      if (WALLS) {
-        const bool wall = (cx == 1 && ax.wall_lo) || (cx == -1 && ax.wall_hi) || (cy == 1 && ay.wall_lo) ||
-                          (cy == -1 && ay.wall_hi) || (cz == 1 && az.wall_lo) || (cz == -1 && az.wall_hi);
-        // bounce-back: pull reads f_{inv i}(x) from the previous step, AA odd finds it in slot i
-        const int wplane = (PAT == LBM355_PULL) ? ii : i;
-        const TS* p = wall ? (src + wplane * sq + center) : (src + plane * sq + off);
-        f[i] = load_as<TS, TC>(p);
+      const int wplane = (PAT == LBM355_PULL) ? ii : i;
+      bool wall = false;
+      if (WALLS)
+        wall = (cx == 1 && ax.wall_lo) || (cx == -1 && ax.wall_hi) || (cy == 1 && ay.wall_lo) ||
+               (cy == -1 && ay.wall_hi) || (cz == 1 && az.wall_lo) || (cz == -1 && az.wall_hi);
+      if (ROWLDS && cx != 0) {
+        // x is periodic here, so `wall` is uniform over the row (block)
+        if (wall)
+          f[i] = load_nt<TS, TC>(src + wplane * sq + center);
+        else
+          tile[j * nx + x] = __builtin_nontemporal_load(src + plane * sq + X[1] + Y[1 - cy] + Z[1 - cz]);
+        ++j;
+      } else if (ROWLDS) {
+        const TS* p = wall ? (src + wplane * sq + center) : (src + plane * sq + X[1] + Y[1 - cy] + Z[1 - cz]);
+        f[i] = load_nt<TS, TC>(p);
       } else {
-        f[i] = load_as<TS, TC>(src + plane * sq + off);
+        const TS* p = wall ? (src + wplane * sq + center) : (src + plane * sq + X[1 - cx] + Y[1 - cy] + Z[1 - cz]);
+        f[i] = load_as<TS, TC>(p);
       }
+    }
+    if (ROWLDS) {
+      __syncthreads();
+      j = 0;
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        const int cx = S::c[i][0], cy = S::c[i][1], cz = S::c[i][2];
+        if (cx == 0) continue;
+        bool wall = false;
+        if (WALLS)
+          wall = (cy == 1 && ay.wall_lo) || (cy == -1 && ay.wall_hi) || (cz == 1 && az.wall_lo) ||
+                 (cz == -1 && az.wall_hi);
+        if (!wall) f[i] = (TC)tile[j * nx + XL[1 - cx]];
+        ++j;
+      }
+      __syncthreads();
     }
   }
 
@@ -109,23 +166,50 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs<TC> a) {
     }
   }
 
+  if (PAT == LBM355_PULL) {
 #pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    const int cx = S::c[i][0], cy = S::c[i][1], cz = S::c[i][2];
-    const int ii = S::inv[i];
-    if (PAT == LBM355_PULL) {
-      store_as<TS, TC>(dst + i * sq + center, f[i]);
-    } else if (PAT == LBM355_AA_EVEN) {
-      store_as<TS, TC>(dst + ii * sq + center, f[i]);
-    } else {
-      const int off = X[1 + cx] + Y[1 + cy] + Z[1 + cz];
-      if (WALLS) {
-        const bool wall = (cx == -1 && ax.wall_lo) || (cx == 1 && ax.wall_hi) || (cy == -1 && ay.wall_lo) ||
-                          (cy == 1 && ay.wall_hi) || (cz == -1 && az.wall_lo) || (cz == 1 && az.wall_hi);
-        TS* p = wall ? (dst + ii * sq + center) : (dst + i * sq + off);
-        store_as<TS, TC>(p, f[i]);
+    for (int i = 0; i < Q; ++i) store_nt<TS, TC>(dst + i * sq + center, f[i]);
+  } else if (PAT == LBM355_AA_EVEN) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) store_nt<TS, TC>(dst + S::inv[i] * sq + center, f[i]);
+  } else {
+    // AA odd writes f_i to x + c_i slot i; at a wall it is bounced into the cell's slot inv(i)
+    int j = 0;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const int cx = S::c[i][0], cy = S::c[i][1], cz = S::c[i][2];
+      const int ii = S::inv[i];
+      bool wall = false;
+      if (WALLS)
+        wall = (cx == -1 && ax.wall_lo) || (cx == 1 && ax.wall_hi) || (cy == -1 && ay.wall_lo) ||
+               (cy == 1 && ay.wall_hi) || (cz == -1 && az.wall_lo) || (cz == 1 && az.wall_hi);
+      if (ROWLDS && cx != 0) {
+        if (wall)
+          store_as<TS, TC>(dst + ii * sq + center, f[i]);
+        else
+          tile[j * nx + x] = (TS)f[i];
+        ++j;
+      } else if (ROWLDS) {
+        TS* p = wall ? (dst + ii * sq + center) : (dst + i * sq + X[1] + Y[1 + cy] + Z[1 + cz]);
+        store_nt<TS, TC>(p, f[i]);
       } else {
-        store_as<TS, TC>(dst + i * sq + off, f[i]);
+        TS* p = wall ? (dst + ii * sq + center) : (dst + i * sq + X[1 + cx] + Y[1 + cy] + Z[1 + cz]);
+        store_as<TS, TC>(p, f[i]);
+      }
+    }
+    if (ROWLDS) {
+      __syncthreads();
+      j = 0;
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        const int cx = S::c[i][0], cy = S::c[i][1], cz = S::c[i][2];
+        if (cx == 0) continue;
+        bool wall = false;
+        if (WALLS)
+          wall = (cy == -1 && ay.wall_lo) || (cy == 1 && ay.wall_hi) || (cz == -1 && az.wall_lo) ||
+                 (cz == 1 && az.wall_hi);
+        if (!wall) __builtin_nontemporal_store(tile[j * nx + XL[1 - cx]], dst + i * sq + X[1] + Y[1 + cy] + Z[1 + cz]);
+        ++j;
       }
     }
   }
@@ -230,48 +314,65 @@ struct Launch {
   dim3 grid, block;
 };
 
-Launch geometry(int nx, int ny, int nz) {
-  int bx = nx >= 256 ? 256 : ((nx + 63) / 64) * 64;
+Launch geometry(int nx, int ny, int nz, bool rowlds = false) {
+  int bx = rowlds ? nx : (nx >= 256 ? 256 : ((nx + 63) / 64) * 64);
   Launch l;
   l.block = dim3(bx, 1, 1);
   l.grid = dim3((nx + bx - 1) / bx, ny, nz);
   return l;
 }
 
+size_t elem_size(int dtype) { return dtype == LBM355_F64 ? 8 : (dtype == LBM355_F32 ? 4 : 2); }
+
+int xshift_dirs(int stencil) { return stencil == LBM355_D2Q9 ? 6 : (stencil == LBM355_D3Q19 ? 10 : 18); }
+
+// LDS row shifting applies when a block can span a whole periodic row
+bool use_rowlds(const lbm355_sweep_desc* d, size_t* lds_bytes) {
+  if (d->pattern == LBM355_AA_EVEN || d->face[0] != LBM355_FACE_PERIODIC) return false;
+  if (d->nx % 64 != 0 || d->nx > 512) return false;
+  const size_t bytes = (size_t)xshift_dirs(d->stencil) * d->nx * elem_size(d->store_dtype);
+  if (bytes > 65536) return false;
+  *lds_bytes = bytes;
+  return true;
+}
+
 template <class S, int FAM, class TS, class TC>
-void* pick_pattern(int pat, bool walls) {
+void* pick_pattern(int pat, bool walls, bool rowlds) {
+#define LBM_K(P, W, R) (void*)sweep_kernel<S, FAM, P, W, R, TS, TC>
   switch (pat) {
     case LBM355_PULL:
-      return walls ? (void*)sweep_kernel<S, FAM, LBM355_PULL, true, TS, TC>
-                   : (void*)sweep_kernel<S, FAM, LBM355_PULL, false, TS, TC>;
+      if (rowlds) return walls ? LBM_K(LBM355_PULL, true, true) : LBM_K(LBM355_PULL, false, true);
+      return walls ? LBM_K(LBM355_PULL, true, false) : LBM_K(LBM355_PULL, false, false);
     case LBM355_AA_EVEN:
-      return (void*)sweep_kernel<S, FAM, LBM355_AA_EVEN, false, TS, TC>;
+      return LBM_K(LBM355_AA_EVEN, false, false);
     case LBM355_AA_ODD:
-      return walls ? (void*)sweep_kernel<S, FAM, LBM355_AA_ODD, true, TS, TC>
-                   : (void*)sweep_kernel<S, FAM, LBM355_AA_ODD, false, TS, TC>;
+      if (rowlds) return walls ? LBM_K(LBM355_AA_ODD, true, true) : LBM_K(LBM355_AA_ODD, false, true);
+      return walls ? LBM_K(LBM355_AA_ODD, true, false) : LBM_K(LBM355_AA_ODD, false, false);
   }
+#undef LBM_K
   return nullptr;
 }
 
 template <class S, int FAM, class TC>
-void* pick_store(int store, int pat, bool walls) {
-  if (store == LBM355_F64) return pick_pattern<S, FAM, double, TC>(pat, walls);
-  if (store == LBM355_F32) return pick_pattern<S, FAM, float, TC>(pat, walls);
-  if (store == LBM355_F16 && sizeof(TC) == 4) return pick_pattern<S, FAM, _Float16, TC>(pat, walls);
+void* pick_store(int store, int pat, bool walls, bool rowlds) {
+  if (store == LBM355_F64) return pick_pattern<S, FAM, double, TC>(pat, walls, rowlds);
+  if (store == LBM355_F32) return pick_pattern<S, FAM, float, TC>(pat, walls, rowlds);
+  if (store == LBM355_F16 && sizeof(TC) == 4) return pick_pattern<S, FAM, _Float16, TC>(pat, walls, rowlds);
   return nullptr;
 }
 
 template <class TC>
-void* pick_kernel(const lbm355_sweep_desc* d, bool walls) {
+void* pick_kernel(const lbm355_sweep_desc* d, bool walls, bool rowlds) {
   if (d->family == LBM355_FAMILY_MOMENT) {
-    if (d->stencil == LBM355_D2Q9) return pick_store<D2Q9, LBM355_FAMILY_MOMENT, TC>(d->store_dtype, d->pattern, walls);
+    if (d->stencil == LBM355_D2Q9)
+      return pick_store<D2Q9, LBM355_FAMILY_MOMENT, TC>(d->store_dtype, d->pattern, walls, rowlds);
     if (d->stencil == LBM355_D3Q19)
-      return pick_store<D3Q19, LBM355_FAMILY_MOMENT, TC>(d->store_dtype, d->pattern, walls);
+      return pick_store<D3Q19, LBM355_FAMILY_MOMENT, TC>(d->store_dtype, d->pattern, walls, rowlds);
     if (d->stencil == LBM355_D3Q27)
-      return pick_store<D3Q27, LBM355_FAMILY_MOMENT, TC>(d->store_dtype, d->pattern, walls);
+      return pick_store<D3Q27, LBM355_FAMILY_MOMENT, TC>(d->store_dtype, d->pattern, walls, rowlds);
   } else if (d->family == LBM355_FAMILY_CUMULANT) {
     if (d->stencil == LBM355_D3Q27)
-      return pick_store<D3Q27, LBM355_FAMILY_CUMULANT, TC>(d->store_dtype, d->pattern, walls);
+      return pick_store<D3Q27, LBM355_FAMILY_CUMULANT, TC>(d->store_dtype, d->pattern, walls, rowlds);
   }
   return nullptr;
 }
@@ -313,7 +414,9 @@ template <class TC>
 int launch_sweep(const lbm355_sweep_desc* d, const void* src, void* dst, void* rho, void* u, hipStream_t s) {
   bool walls = false;
   for (int a = 0; a < stencil_d(d->stencil); ++a) walls |= d->face[a] == LBM355_FACE_NOSLIP;
-  void* k = pick_kernel<TC>(d, walls);
+  size_t lds = 0;
+  const bool rowlds = use_rowlds(d, &lds);
+  void* k = pick_kernel<TC>(d, walls, rowlds);
   if (!k) return fail(LBM355_ERR_UNSUPPORTED, "no kernel for this stencil/family/dtype combination");
   SweepArgs<TC> a;
   a.src = src;
@@ -339,8 +442,8 @@ int launch_sweep(const lbm355_sweep_desc* d, const void* src, void* dst, void* r
   a.nactive = d->n_active;
   a.smag_cs2x18 = (TC)(18.0 * d->smagorinsky_constant * d->smagorinsky_constant);
   a.omega_shear = (TC)d->omega_shear;
-  Launch l = geometry(d->nx, d->ny, d->nz);
-  hipLaunchKernelGGL((KernelFn<TC>)k, l.grid, l.block, 0, s, a);
+  Launch l = geometry(d->nx, d->ny, d->nz, rowlds);
+  hipLaunchKernelGGL((KernelFn<TC>)k, l.grid, l.block, rowlds ? lds : 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(LBM355_ERR_LAUNCH, std::string("launch failed: ") + hipGetErrorString(e));
   return LBM355_OK;

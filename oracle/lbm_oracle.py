@@ -460,18 +460,21 @@ class OracleDomain:
             self.interior(self.dst)[...] = fpost
             self.src, self.dst = self.dst, self.src
         else:
-            if any(self.faces.get(a) != "periodic" for a in range(self.D)):
-                raise NotImplementedError("the AA oracle covers fully periodic domains")
+            if any(self.faces.get(a) == "ghost" for a in range(self.D)):
+                raise NotImplementedError("the AA oracle covers periodic and NoSlip faces")
             if self.aa_parity == 0:
-                # ODD kernel: read f[x - c_i](inv i), write f[x + c_i](i)
+                # periodicity(EVEN), boundary(prev=EVEN), then the ODD kernel:
+                # read f[x - c_i](inv i), write f[x + c_i](i)
                 self.sync_periodic(self.src)
+                self.noslip_aa(self.src, prev_even=True)
                 fpre = np.stack([self._neighbor_view(self.src, self.inv[i], -self.c[i]) for i in range(self.Q)])
                 fpost = self._collide_cells(fpre)
                 for i in range(self.Q):
                     self._neighbor_view(self.src, i, self.c[i])[...] = fpost[i]
                 self._fold_ghost_writes()
             else:
-                # EVEN kernel: read f[x](i), write f[x](inv i)
+                # boundary(prev=ODD), then the EVEN kernel: read f[x](i), write f[x](inv i)
+                self.noslip_aa(self.src, prev_even=False)
                 fpre = self.interior().copy()
                 fpost = self._collide_cells(fpre)
                 out = self.interior()
@@ -479,14 +482,32 @@ class OracleDomain:
                     out[self.inv[i]] = fpost[i]
             self.aa_parity ^= 1
 
+    def noslip_aa(self, arr, prev_even):
+        """NoSlip between AA time steps (``advanced_streaming/indexing.py:12-181``): after an EVEN
+        step the population leaving x in direction d sits in slot inv(d) of x and is placed where
+        the ODD step reads it (slot d of the wall cell); after an ODD step the value written into
+        the wall cell (slot d) is returned to slot inv(d) of x, where the EVEN step reads it."""
+        links = self.wall_links()
+        if len(links) == 0:
+            return
+        d = links[:, self.D]
+        cell = tuple(reversed([links[:, a] for a in range(self.D)]))
+        nb = tuple(reversed([links[:, a] + self.c[d, a] for a in range(self.D)]))
+        if prev_even:
+            arr[(d,) + nb] = arr[(self.inv[d],) + cell]
+        else:
+            arr[(self.inv[d],) + cell] = arr[(d,) + nb]
+
     def _fold_ghost_writes(self):
         """Values the ODD kernel wrote into ghost cells belong to the periodic images
         (the in-place periodicity handling of ``advanced_streaming/communication.py``)."""
         for i in range(self.Q):
             ci = self.c[i]
             arr = self.src[i]
-            # wrap each axis in turn; edges and corners are handled by the repeated wrapping
+            # wrap each periodic axis in turn; edges and corners are handled by the repeated wrapping
             for a in range(self.D):
+                if self.faces.get(a) != "periodic":
+                    continue
                 ax = self._ax(a) - 1
                 if ci[a] == 1:
                     src_sl = [slice(None)] * self.D

@@ -84,3 +84,34 @@ def test_cumulant_d3q27_fp64_all_rates():
     ref, _ = oracle_run(m, shape, fdict, "pull", f0, 10)
     got = gpu_run(gpu_kernels(kw), m, shape, fdict, f0, 10)
     assert np.abs(got - ref).max() <= FP64_TOL
+
+
+# nx a multiple of 64 with periodic x selects the LDS row-shift kernels (blockDim.x == nx)
+ROWLDS_CASES = [
+    ("D2Q9", Method.TRT, dict(relaxation_rates=[1.8, 1.1], force=(1e-4, 0)), ("periodic", "noslip"), "pull", (64, 6)),
+    ("D2Q9", Method.SRT, dict(relaxation_rate=1.6), ("periodic", "periodic"), "aa", (128, 5)),
+    ("D3Q19", Method.TRT, dict(relaxation_rates=[1.97, 1.6], compressible=True, force=(1e-5, 0, 0)),
+     ("periodic", "noslip", "noslip"), "pull", (64, 5, 4)),
+    ("D3Q19", Method.MRT, dict(relaxation_rate=1.8, force=(1e-5, 0, 0)), ("periodic", "noslip", "periodic"), "aa",
+     (64, 4, 5)),
+    ("D3Q27", Method.CUMULANT, dict(relaxation_rate=1.9, compressible=True, force=(1e-5, 2e-5, 0)),
+     ("periodic", "periodic", "periodic"), "aa", (64, 4, 3)),
+    ("D3Q27", Method.CUMULANT, dict(relaxation_rate=1.7, compressible=True), ("periodic", "noslip", "periodic"),
+     "pull", (64, 3, 4)),
+    ("D3Q27", Method.CUMULANT, dict(relaxation_rate=1.7, compressible=True), ("periodic", "periodic", "noslip"),
+     "aa", (64, 3, 4)),
+]
+
+
+@pytest.mark.parametrize("case", ROWLDS_CASES, ids=lambda c: f"{c[0]}-{c[1].name}-{c[4]}-{'x'.join(map(str, c[5]))}")
+def test_row_lds_kernels_fp64(case):
+    st, meth, extra, faces, pattern, shape = case
+    kw = dict(stencil=LBStencil(Stencil[st]), method=meth, streaming_pattern=pattern, **extra)
+    m = create_lb_method(LBMConfig(**kw))
+    f0 = seeded_state(m, shape, seed=4)
+    steps = 16
+    fdict = {a: faces[a] for a in range(m.dim)}
+    ref, _ = oracle_run(m, shape, fdict, pattern, f0, steps)
+    print(f"rowlds case {st} {meth.name} {pattern} {shape}", flush=True)
+    got = gpu_run(gpu_kernels(kw), m, shape, fdict, f0, steps)
+    assert np.abs(got - ref).max() <= FP64_TOL
