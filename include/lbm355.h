@@ -111,6 +111,28 @@ int lbm355_noslip_links(int32_t stencil, int32_t pattern, int32_t dtype, void* p
  * out is a device buffer of 4 doubles (overwritten). Stored zero-centred PDFs add 1 per cell to rho. */
 int lbm355_reduce_conserved(const lbm355_sweep_desc* desc, const void* pdf, double* out, void* stream);
 
+/* Equilibrium initialisation from macroscopic fields (lbmpy macroscopic_value_kernels.py:29-45,
+ * pdf_initialization_assignments; run by LatticeBoltzmannStep.set_pdf_fields_from_macroscopic_values,
+ * lbstep.py:268-269).  density / velocity: FP64 device arrays laid out with desc->out_stride_*
+ * (velocity component stride out_stride_c) and the same ghost-layer offset as the PDFs.  The
+ * equilibrium velocity is u - F/(2 rho_ref) when a force model is present.  prev_pattern:
+ * LBM355_PULL writes slot i; LBM355_AA_EVEN writes slot inv(i) (state "after an EVEN step", so
+ * the next AA kernel is ODD).  table_f64: FP64 method table (moment family; ignored for cumulants).
+ * Computes in FP64 and rounds once to the storage type. */
+int lbm355_set_equilibrium(const lbm355_sweep_desc* desc, void* pdf, const double* density, const double* velocity,
+                           const double* table_f64, int32_t prev_pattern, void* stream);
+
+/* Density and velocity (u = j/rho_ref + F/(2 rho_ref)) of the populations the last kernel wrote,
+ * read through its write accessor (lbmpy macroscopic_value_kernels.py:48-61 with
+ * get_field_accesses :14-26): LBM355_PULL slot i at x, LBM355_AA_EVEN slot inv(i) at x,
+ * LBM355_AA_ODD slot i at x + c_i (bounced into slot inv(i) of x at a NoSlip face).
+ * Either output may be NULL.  FP64 outputs, strides as for lbm355_set_equilibrium. */
+int lbm355_get_macroscopic(const lbm355_sweep_desc* desc, const void* pdf, double* density, double* velocity,
+                           int32_t last_pattern, void* stream);
+
+/* sizeof(lbm355_sweep_desc), for FFI layout checks. */
+int64_t lbm355_desc_size(void);
+
 /* Last error message of the calling thread (static storage, never NULL). */
 const char* lbm355_last_error(void);
 

@@ -22,6 +22,7 @@ FACE_GHOST, FACE_PERIODIC, FACE_NOSLIP = 0, 1, 2
 FLAG_ZERO_CENTERED, FLAG_COMPRESSIBLE, FLAG_EQ_SHIFT, FLAG_FORCE, FLAG_SMAGORINSKY = 1, 2, 4, 8, 16
 
 DTYPE_CODES = {"float64": F64, "float32": F32, "float16": F16}
+STENCIL_NAMES = {"D2Q9": D2Q9, "D3Q19": D3Q19, "D3Q27": D3Q27}
 
 
 class SweepDesc(ctypes.Structure):
@@ -36,7 +37,8 @@ class SweepDesc(ctypes.Structure):
 
 
 EXPORTED = ("lbm355_sweep", "lbm355_table_elements", "lbm355_noslip_links", "lbm355_reduce_conserved",
-            "lbm355_last_error", "lbm355_version")
+            "lbm355_set_equilibrium", "lbm355_get_macroscopic", "lbm355_desc_size", "lbm355_last_error",
+            "lbm355_version")
 
 _lib = None
 
@@ -60,6 +62,16 @@ def load(path=None):
     lib.lbm355_noslip_links.restype = c_int32
     lib.lbm355_reduce_conserved.argtypes = [ctypes.POINTER(SweepDesc), c_void_p, c_void_p, c_void_p]
     lib.lbm355_reduce_conserved.restype = c_int32
+    lib.lbm355_set_equilibrium.argtypes = [ctypes.POINTER(SweepDesc), c_void_p, c_void_p, c_void_p, c_void_p,
+                                           c_int32, c_void_p]
+    lib.lbm355_set_equilibrium.restype = c_int32
+    lib.lbm355_get_macroscopic.argtypes = [ctypes.POINTER(SweepDesc), c_void_p, c_void_p, c_void_p, c_int32,
+                                           c_void_p]
+    lib.lbm355_get_macroscopic.restype = c_int32
+    lib.lbm355_desc_size.argtypes = []
+    lib.lbm355_desc_size.restype = c_int64
+    if lib.lbm355_desc_size() != ctypes.sizeof(SweepDesc):
+        raise ImportError("lbm355_sweep_desc layout mismatch between include/lbm355.h and the ctypes binding")
     lib.lbm355_last_error.argtypes = []
     lib.lbm355_last_error.restype = ctypes.c_char_p
     lib.lbm355_version.argtypes = []

@@ -13,6 +13,9 @@
 
 #include "../../include/lbm355.h"
 
+// records the calling thread's last error (returned by lbm355_last_error) and returns `code`
+int lbm355_set_error(int code, const char* msg);
+
 namespace lbm355 {
 
 // ---------------------------------------------------------------- stencils

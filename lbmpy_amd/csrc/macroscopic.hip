@@ -1,0 +1,218 @@
+// Macroscopic setter / getter on the GPU (lbmpy macroscopic_value_kernels.py:29-61), which the
+// reference's LatticeBoltzmannStep runs on the CPU after copying the field back (lbstep.py:311-314,
+// 444-449).
+//
+// setter: PDFs := equilibrium(rho, u - F/(2 rho_ref)) in storage form, written through the
+//         accessor of the *previous* time step (pull: slot i; AA with previous EVEN: slot inv(i)).
+// getter: rho, u = j / rho_ref + F/(2 rho_ref) from the populations the last kernel wrote,
+//         read through that kernel's write accessor (pull: slot i at x; AA EVEN: slot inv(i) at x;
+//         AA ODD: slot i at x + c_i, bounced back into slot inv(i) at a wall).
+// Both compute in FP64 regardless of the storage type.
+#include "lbm_common.h"
+#include "collide_moment.h"
+#include "collide_cumulant.h"
+
+namespace lbm355 {
+
+// equilibrium of the moment family in storage form, from the FP64 method table
+template <class S>
+__device__ __forceinline__ void moment_equilibrium(const double* tab, double drho, double lam, const double (&u)[3],
+                                                   double (&feq)[S::Q]) {
+  using L = MomentTab<S::Q>;
+#pragma unroll
+  for (int i = 0; i < S::Q; ++i) {
+    double p = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      p += tab[L::L + 3 * i + d] * u[d];
+      p += tab[L::QD + 3 * i + d] * (u[d] * u[d]);
+    }
+    p += tab[L::QO + 3 * i + 0] * (u[0] * u[1]) + tab[L::QO + 3 * i + 1] * (u[0] * u[2]) +
+         tab[L::QO + 3 * i + 2] * (u[1] * u[2]);
+    feq[i] = tab[L::A + i] + tab[L::B + i] * drho + lam * p;
+  }
+}
+
+// D3Q27 cumulant equilibrium: backward chimera of rho * Gaussian central moments about u:
+// f_i = rho * prod_a phi(c_ia, u_a), phi(0,u) = 2/3 - u^2, phi(+-1,u) = (1/3 + u^2 +- u) / 2
+__device__ __forceinline__ double phi(int c, double u) {
+  return c == 0 ? (2.0 / 3.0 - u * u) : 0.5 * (1.0 / 3.0 + u * u + c * u);
+}
+
+template <class S, int FAM, int PAT, class TS>
+__global__ void set_equilibrium_kernel(TS* pdf, const double* rho_in, const double* u_in, const double* tab, int nx,
+                                       int ny, int nz, int g, int64_t sy, int64_t sz, int64_t sq, int64_t osy,
+                                       int64_t osz, int64_t osc, int flags, double fx, double fy, double fz) {
+  constexpr int Q = S::Q, D = S::D;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= nx) return;
+  const int y = blockIdx.y, z = blockIdx.z;
+  const int64_t o = (int64_t)(x + g) + (int64_t)(y + g) * osy + (D == 3 ? (int64_t)(z + g) * osz : 0);
+  const double rho = rho_in[o];
+  const bool comp = flags & LBM355_FLAG_COMPRESSIBLE;
+  const double ref = comp ? rho : 1.0;
+  const double F[3] = {fx, fy, fz};
+  double u[3] = {0, 0, 0};
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    u[d] = u_in[o + d * osc];
+    if (flags & LBM355_FLAG_FORCE) u[d] -= 0.5 * F[d] / ref;
+  }
+  double feq[Q];
+  if (FAM == LBM355_FAMILY_MOMENT) {
+    moment_equilibrium<S>(tab, rho - 1.0, comp ? rho : 1.0, u, feq);
+  } else {
+    const bool zc = flags & LBM355_FLAG_ZERO_CENTERED;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const double fi = rho * phi(S::c[i][0], u[0]) * phi(S::c[i][1], u[1]) * phi(S::c[i][2], u[2]);
+      feq[i] = zc ? fi - weight<S>(i) : fi;
+    }
+  }
+  const int64_t c = (int64_t)(x + g) + (int64_t)(y + g) * sy + (D == 3 ? (int64_t)(z + g) * sz : 0);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const int slot = PAT == LBM355_PULL ? i : S::inv[i];
+    pdf[slot * sq + c] = (TS)feq[i];
+  }
+}
+
+template <class S, int PAT, class TS>
+__global__ void get_macroscopic_kernel(const TS* pdf, double* rho_out, double* u_out, int nx, int ny, int nz, int g,
+                                       int64_t sy, int64_t sz, int64_t sq, int64_t osy, int64_t osz, int64_t osc,
+                                       int fx_mode, int fy_mode, int fz_mode, int flags, double fx, double fy,
+                                       double fz) {
+  constexpr int Q = S::Q, D = S::D;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= nx) return;
+  const int y = blockIdx.y, z = blockIdx.z;
+  const AxisNb ax = axis_neighbours(x, nx, g, fx_mode);
+  const AxisNb ay = axis_neighbours(y, ny, g, fy_mode);
+  AxisNb az = {0, 0, false, false};
+  if (D == 3) az = axis_neighbours(z, nz, g, fz_mode);
+  const int64_t X[3] = {ax.lo, x + g, ax.hi};
+  const int64_t Y[3] = {ay.lo * sy, (y + g) * sy, ay.hi * sy};
+  const int64_t Z[3] = {D == 3 ? az.lo * sz : 0, D == 3 ? (z + g) * sz : 0, D == 3 ? az.hi * sz : 0};
+  const int64_t center = X[1] + Y[1] + Z[1];
+  double s = 0, j[3] = {0, 0, 0};
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const int cx = S::c[i][0], cy = S::c[i][1], cz = S::c[i][2];
+    double v;
+    if (PAT == LBM355_PULL) {
+      v = (double)pdf[i * sq + center];
+    } else if (PAT == LBM355_AA_EVEN) {
+      v = (double)pdf[S::inv[i] * sq + center];
+    } else {
+      const bool wall = (cx == -1 && ax.wall_lo) || (cx == 1 && ax.wall_hi) || (cy == -1 && ay.wall_lo) ||
+                        (cy == 1 && ay.wall_hi) || (cz == -1 && az.wall_lo) || (cz == 1 && az.wall_hi);
+      v = wall ? (double)pdf[S::inv[i] * sq + center] : (double)pdf[i * sq + X[1 + cx] + Y[1 + cy] + Z[1 + cz]];
+    }
+    s += v;
+    j[0] += cx * v;
+    j[1] += cy * v;
+    j[2] += cz * v;
+  }
+  const double rho = (flags & LBM355_FLAG_ZERO_CENTERED) ? 1.0 + s : s;
+  const double ref = (flags & LBM355_FLAG_COMPRESSIBLE) ? rho : 1.0;
+  const double F[3] = {fx, fy, fz};
+  const int64_t o = (int64_t)(x + g) + (int64_t)(y + g) * osy + (D == 3 ? (int64_t)(z + g) * osz : 0);
+  if (rho_out) rho_out[o] = rho;
+  if (u_out) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      double ud = j[d] / ref;
+      if (flags & LBM355_FLAG_FORCE) ud += 0.5 * F[d] / ref;
+      u_out[o + d * osc] = ud;
+    }
+  }
+}
+
+}  // namespace lbm355
+
+using namespace lbm355;
+
+namespace {
+int fail2(int code, const char* m) { return lbm355_set_error(code, m); }
+
+template <class S, int FAM, class TS>
+int set_eq_launch(const lbm355_sweep_desc* d, void* pdf, const double* rho, const double* u, const double* tab,
+                  int prev_pattern, hipStream_t s) {
+  const int bx = d->nx >= 256 ? 256 : ((d->nx + 63) / 64) * 64;
+  dim3 block(bx), grid((d->nx + bx - 1) / bx, d->ny, d->nz);
+  auto k = prev_pattern == LBM355_PULL ? set_equilibrium_kernel<S, FAM, LBM355_PULL, TS>
+                                       : set_equilibrium_kernel<S, FAM, LBM355_AA_EVEN, TS>;
+  hipLaunchKernelGGL(k, grid, block, 0, s, (TS*)pdf, rho, u, tab, d->nx, d->ny, d->nz, d->ghost_layers, d->stride_y,
+                     d->stride_z, d->stride_q, d->out_stride_y, d->out_stride_z, d->out_stride_c, d->flags,
+                     d->force[0], d->force[1], d->force[2]);
+  return hipGetLastError() == hipSuccess ? 0 : fail2(LBM355_ERR_LAUNCH, "setter launch failed");
+}
+
+template <class S, class TS>
+int get_launch(const lbm355_sweep_desc* d, const void* pdf, double* rho, double* u, int last_pattern,
+               hipStream_t s) {
+  const int bx = d->nx >= 256 ? 256 : ((d->nx + 63) / 64) * 64;
+  dim3 block(bx), grid((d->nx + bx - 1) / bx, d->ny, d->nz);
+  auto k = last_pattern == LBM355_PULL      ? get_macroscopic_kernel<S, LBM355_PULL, TS>
+           : last_pattern == LBM355_AA_EVEN ? get_macroscopic_kernel<S, LBM355_AA_EVEN, TS>
+                                            : get_macroscopic_kernel<S, LBM355_AA_ODD, TS>;
+  hipLaunchKernelGGL(k, grid, block, 0, s, (const TS*)pdf, rho, u, d->nx, d->ny, d->nz, d->ghost_layers, d->stride_y,
+                     d->stride_z, d->stride_q, d->out_stride_y, d->out_stride_z, d->out_stride_c, d->face[0],
+                     d->face[1], d->face[2], d->flags, d->force[0], d->force[1], d->force[2]);
+  return hipGetLastError() == hipSuccess ? 0 : fail2(LBM355_ERR_LAUNCH, "getter launch failed");
+}
+
+template <class S, int FAM>
+int set_eq_dtype(const lbm355_sweep_desc* d, void* pdf, const double* rho, const double* u, const double* tab,
+                 int prev, hipStream_t s) {
+  if (d->store_dtype == LBM355_F64) return set_eq_launch<S, FAM, double>(d, pdf, rho, u, tab, prev, s);
+  if (d->store_dtype == LBM355_F32) return set_eq_launch<S, FAM, float>(d, pdf, rho, u, tab, prev, s);
+  return set_eq_launch<S, FAM, _Float16>(d, pdf, rho, u, tab, prev, s);
+}
+
+template <class S>
+int get_dtype(const lbm355_sweep_desc* d, const void* pdf, double* rho, double* u, int last, hipStream_t s) {
+  if (d->store_dtype == LBM355_F64) return get_launch<S, double>(d, pdf, rho, u, last, s);
+  if (d->store_dtype == LBM355_F32) return get_launch<S, float>(d, pdf, rho, u, last, s);
+  return get_launch<S, _Float16>(d, pdf, rho, u, last, s);
+}
+}  // namespace
+
+extern "C" {
+
+int lbm355_set_equilibrium(const lbm355_sweep_desc* d, void* pdf, const double* density, const double* velocity,
+                           const double* table_f64, int32_t prev_pattern, void* stream) {
+  if (!d || !pdf || !density || !velocity) return fail2(LBM355_ERR_ARG, "NULL argument");
+  if (d->store_dtype < 0 || d->store_dtype > 2) return fail2(LBM355_ERR_ARG, "bad dtype");
+  if (prev_pattern != LBM355_PULL && prev_pattern != LBM355_AA_EVEN)
+    return fail2(LBM355_ERR_ARG, "setter writes through the pull or AA-EVEN accessor");
+  hipStream_t s = (hipStream_t)stream;
+  if (d->family == LBM355_FAMILY_MOMENT) {
+    if (!table_f64) return fail2(LBM355_ERR_ARG, "moment family needs the FP64 method table");
+    if (d->stencil == LBM355_D2Q9)
+      return set_eq_dtype<D2Q9, LBM355_FAMILY_MOMENT>(d, pdf, density, velocity, table_f64, prev_pattern, s);
+    if (d->stencil == LBM355_D3Q19)
+      return set_eq_dtype<D3Q19, LBM355_FAMILY_MOMENT>(d, pdf, density, velocity, table_f64, prev_pattern, s);
+    if (d->stencil == LBM355_D3Q27)
+      return set_eq_dtype<D3Q27, LBM355_FAMILY_MOMENT>(d, pdf, density, velocity, table_f64, prev_pattern, s);
+  } else if (d->family == LBM355_FAMILY_CUMULANT && d->stencil == LBM355_D3Q27) {
+    return set_eq_dtype<D3Q27, LBM355_FAMILY_CUMULANT>(d, pdf, density, velocity, table_f64, prev_pattern, s);
+  }
+  return fail2(LBM355_ERR_UNSUPPORTED, "no setter for this stencil/family");
+}
+
+int lbm355_get_macroscopic(const lbm355_sweep_desc* d, const void* pdf, double* density, double* velocity,
+                           int32_t last_pattern, void* stream) {
+  if (!d || !pdf) return fail2(LBM355_ERR_ARG, "NULL argument");
+  if (d->store_dtype < 0 || d->store_dtype > 2) return fail2(LBM355_ERR_ARG, "bad dtype");
+  if (last_pattern < 0 || last_pattern > 2) return fail2(LBM355_ERR_ARG, "bad pattern");
+  hipStream_t s = (hipStream_t)stream;
+  if (d->stencil == LBM355_D2Q9) return get_dtype<D2Q9>(d, pdf, density, velocity, last_pattern, s);
+  if (d->stencil == LBM355_D3Q19) return get_dtype<D3Q19>(d, pdf, density, velocity, last_pattern, s);
+  if (d->stencil == LBM355_D3Q27) return get_dtype<D3Q27>(d, pdf, density, velocity, last_pattern, s);
+  return fail2(LBM355_ERR_ARG, "unknown stencil");
+}
+
+int64_t lbm355_desc_size(void) { return (int64_t)sizeof(lbm355_sweep_desc); }
+
+}  // extern "C"

@@ -306,6 +306,11 @@ int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
+}  // namespace
+
+int lbm355_set_error(int code, const char* msg) { return fail(code, msg); }
+
+namespace {
 
 template <class TC>
 using KernelFn = void (*)(SweepArgs<TC>);

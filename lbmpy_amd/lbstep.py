@@ -1,0 +1,315 @@
+"""``LatticeBoltzmannStep`` on the MI355X sweep.
+
+Mirrors lbmpy ``src/lbmpy/lbstep.py:18-450``: same constructor parameters and
+defaults (D2Q9 / D3Q27 by dimension, one ghost layer, ``<name>_pdfSrc`` /
+``<name>_pdfTmp`` fields), ``run``, ``time_step``, ``pre_run``/``post_run``,
+``benchmark_run``/``benchmark`` (MLUPS = cells / seconds-per-step * 1e-6,
+lbstep.py:329-344), ``velocity``/``density`` slicing, ``boundary_handling``
+with ``set_boundary(NoSlip(), slice_from_direction(...))`` and
+``set_pdf_fields_from_macroscopic_values``.
+
+Differences, all deliberate: the target is always the GPU (this package has
+no CPU sweep); PDF fields are PyTorch tensors that stay resident in HBM; the
+macroscopic setter/getter run on the GPU (the reference compiles them for the
+CPU, lbstep.py:444-449); periodicity is built into the sweep instead of ghost
+copies; in-place AA streaming is available (``streaming_pattern='aa'``), which
+the reference's fixed time loop does not support.
+"""
+import ctypes
+import time
+from types import MappingProxyType
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from .boundaries import LatticeBoltzmannBoundaryHandling
+from .creationfunctions import (update_with_default_parameters, create_lb_function, Timestep, CreateKernelConfig)
+from .enums import Stencil, Target
+from .kernel_tables import method_table
+from .stencils import LBStencil
+from .sweep import create_pdf_array, TORCH_DTYPES
+
+
+class _SlicedGetter:
+    def __init__(self, fn):
+        self._fn = fn
+
+    def __getitem__(self, item):
+        return self._fn(item)
+
+
+class LatticeBoltzmannStep:
+
+    def __init__(self, domain_size=None, lbm_kernel=None, periodicity=False, kernel_params=MappingProxyType({}),
+                 data_handling=None, name="lbm", optimization=None, velocity_data_name=None, density_data_name=None,
+                 density_data_index=None, compute_velocity_in_every_step=False, compute_density_in_every_step=False,
+                 velocity_input_array_name=None, time_step_order='stream_collide', flag_interface=None,
+                 alignment_if_vectorized=64, fixed_loop_sizes=True, timeloop_creation_function=None,
+                 lbm_config=None, lbm_optimisation=None, config=None, **method_parameters):
+        if data_handling is not None:
+            raise NotImplementedError("pystencils data handlings are not used on the GPU path; pass domain_size")
+        if domain_size is None:
+            raise ValueError("Specify either domain_size or data_handling")
+        if time_step_order != 'stream_collide':
+            raise NotImplementedError("only the fused stream-collide order is implemented")
+        if velocity_input_array_name is not None:
+            raise NotImplementedError("velocity input arrays are not supported")
+        optimization = dict(optimization or {})
+        optimization["target"] = Target.GPU
+        domain_size = tuple(int(n) for n in domain_size)
+        dim = len(domain_size)
+        if lbm_config:
+            method_parameters['stencil'] = lbm_config.stencil
+        if 'stencil' not in method_parameters:
+            method_parameters['stencil'] = LBStencil(Stencil.D2Q9 if dim == 2 else Stencil.D3Q27)
+        if config is not None:
+            from dataclasses import replace
+            config = replace(config, target=Target.GPU)
+        lbm_config, lbm_optimisation, config = update_with_default_parameters(
+            method_parameters, optimization, lbm_config, lbm_optimisation, config)
+        if isinstance(periodicity, bool):
+            periodicity = (periodicity,) * dim
+        self.periodicity = tuple(bool(p) for p in periodicity)
+        self.name = name
+        self._pdf_arr_name = name + "_pdfSrc"
+        self._tmp_arr_name = name + "_pdfTmp"
+        self.velocity_data_name = name + "_velocity" if velocity_data_name is None else velocity_data_name
+        self.density_data_name = name + "_density" if density_data_name is None else density_data_name
+        self.kernel_params = dict(kernel_params)
+        self._domain_size = domain_size
+        self._lbm_config = lbm_config
+        self._lbm_optimisation = lbm_optimisation
+        self._config = config
+        self._pattern = lbm_config.streaming_pattern
+        if self._pattern not in ("pull", "aa"):
+            raise NotImplementedError(f"streaming pattern {self._pattern!r}")
+
+        from dataclasses import replace
+        lc = replace(lbm_config, field_name=self._pdf_arr_name, temporary_field_name=self._tmp_arr_name,
+                     lb_method=None, collision_rule=None, update_rule=None, ast=None)
+        cfg = replace(config, ghost_layers=1)
+        if lbm_kernel is not None:
+            self._kernels = [lbm_kernel]
+        elif self._pattern == "pull":
+            self._kernels = [create_lb_function(lbm_config=lc, lbm_optimisation=lbm_optimisation, config=cfg)]
+        else:
+            self._kernels = [create_lb_function(lbm_config=replace(lc, timestep=t), lbm_optimisation=lbm_optimisation,
+                                                config=cfg) for t in (Timestep.EVEN, Timestep.ODD)]
+        self.method = self._kernels[0].method
+        self.ast = self._kernels[0].ast
+        q = self.method.stencil.Q
+        dt = config.data_type
+        self._src = create_pdf_array(domain_size, q, 1, dt)
+        self._tmp = create_pdf_array(domain_size, q, 1, dt) if self._pattern == "pull" else None
+        # macroscopic fields: host numpy (interior) + FP64 device staging with the PDF ghost layout
+        self._rho_host = np.ones(domain_size)
+        self._u_host = np.zeros(domain_size + (dim,))
+        self._rho_dev = torch.zeros([n + 2 for n in domain_size][::-1], dtype=torch.float64, device="cuda")
+        self._u_dev = torch.zeros([dim] + [n + 2 for n in domain_size][::-1], dtype=torch.float64, device="cuda")
+        self._boundary_handling = LatticeBoltzmannBoundaryHandling(self.method, domain_size, self.periodicity)
+        self._aa_next_odd = True
+        self._links_dev = None
+        self._links_version = None
+        self.time_steps_run = 0
+        self.set_pdf_fields_from_macroscopic_values()
+
+    # ------------------------------------------------------------------ properties
+    @property
+    def boundary_handling(self):
+        return self._boundary_handling
+
+    @property
+    def data_handling(self):
+        return None
+
+    @property
+    def dim(self):
+        return len(self._domain_size)
+
+    @property
+    def domain_size(self):
+        return self._domain_size
+
+    @property
+    def number_of_cells(self):
+        return int(np.prod(self._domain_size))
+
+    @property
+    def pdf_array_name(self):
+        return self._pdf_arr_name
+
+    @property
+    def lbm_config(self):
+        return self._lbm_config
+
+    @property
+    def lbm_optimisation(self):
+        return self._lbm_optimisation
+
+    @property
+    def config(self):
+        return self._config
+
+    @property
+    def pdf_array(self):
+        """The current PDF field (torch, ``[x, y, (z,) q]`` with one ghost layer)."""
+        return self._src
+
+    @property
+    def velocity(self):
+        return _SlicedGetter(lambda s: self.velocity_slice(s, masked=False))
+
+    @property
+    def density(self):
+        return _SlicedGetter(lambda s: self.density_slice(s, masked=False))
+
+    # ------------------------------------------------------------------ fields
+    def _default_slice(self):
+        return (slice(None),) * 2 if self.dim == 2 else (slice(None), slice(None), self._domain_size[2] // 2)
+
+    def velocity_slice(self, slice_obj=None, masked=True):
+        res = self._u_host[self._default_slice() if slice_obj is None else slice_obj]
+        if masked:
+            mask = self._boundary_handling.get_mask(None, 'domain', True)
+            sl = self._default_slice() if slice_obj is None else slice_obj
+            m = mask[sl]
+            if m.ndim < res.ndim:
+                m = np.repeat(m[..., None], res.shape[-1], axis=-1)
+            res = np.ma.masked_array(res, m)
+        return res
+
+    def density_slice(self, slice_obj=None, masked=True):
+        res = self._rho_host[self._default_slice() if slice_obj is None else slice_obj]
+        if masked:
+            mask = self._boundary_handling.get_mask(None, 'domain', True)
+            res = np.ma.masked_array(res, mask[self._default_slice() if slice_obj is None else slice_obj])
+        return res
+
+    def _descriptor(self, arr=None):
+        k = self._kernels[0]
+        k.set_faces(self._boundary_handling.faces)
+        d = k.descriptor(self._src if arr is None else arr, self.kernel_params)
+        D = self.dim
+        d.out_stride_y = self._rho_dev.stride(D - 2)
+        d.out_stride_z = self._rho_dev.stride(0) if D == 3 else 0
+        d.out_stride_c = self._u_dev.stride(0)
+        return d
+
+    def set_pdf_fields_from_macroscopic_values(self):
+        """Equilibrium from the host density/velocity fields (lbstep.py:268-269), on the GPU."""
+        D = self.dim
+        inner = (slice(1, -1),) * D
+        self._rho_dev[inner] = torch.from_numpy(np.ascontiguousarray(self._rho_host.T)).cuda()
+        for a in range(D):
+            self._u_dev[a][inner] = torch.from_numpy(np.ascontiguousarray(self._u_host[..., a].T)).cuda()
+        d = self._descriptor()
+        tab, _ = method_table(self.method, self.kernel_params, np.float64)
+        tab_dev = torch.from_numpy(tab).cuda()
+        prev = nat.PULL if self._pattern == "pull" else nat.AA_EVEN
+        lib = nat.load()
+        nat.check(lib.lbm355_set_equilibrium(ctypes.byref(d), ctypes.c_void_p(self._src.data_ptr()),
+                                             ctypes.c_void_p(self._rho_dev.data_ptr()),
+                                             ctypes.c_void_p(self._u_dev.data_ptr()),
+                                             ctypes.c_void_p(tab_dev.data_ptr()), prev, nat.current_stream_handle()))
+        self._aa_next_odd = True
+        torch.cuda.synchronize()
+
+    def _links(self):
+        links = self._boundary_handling.links
+        if self._links_version is not links:
+            self._links_version = links
+            self._links_dev = torch.from_numpy(np.ascontiguousarray(links)).cuda() if len(links) else None
+        return self._links_dev
+
+    def _apply_links(self, pattern):
+        links = self._links()
+        if links is None:
+            return
+        a = self._src
+        lib = nat.load()
+        nat.check(lib.lbm355_noslip_links(nat.STENCIL_NAMES[self.method.stencil.name], pattern,
+                                          nat.DTYPE_CODES[self._config.data_type], ctypes.c_void_p(a.data_ptr()),
+                                          a.stride(1), a.stride(2) if self.dim == 3 else 0, a.stride(self.dim),
+                                          ctypes.c_void_p(links.data_ptr()), links.shape[0],
+                                          nat.current_stream_handle()))
+
+    # ------------------------------------------------------------------ stepping
+    def pre_run(self):
+        for k in self._kernels:
+            k.set_faces(self._boundary_handling.faces)
+
+    def time_step(self):
+        if self._pattern == "pull":
+            self._apply_links(nat.PULL)
+            self._kernels[0](**{self._pdf_arr_name: self._src, self._tmp_arr_name: self._tmp}, **self.kernel_params)
+            self._src, self._tmp = self._tmp, self._src
+        else:
+            even, odd = self._kernels
+            if self._aa_next_odd:
+                self._apply_links(nat.AA_EVEN)
+                odd(**{self._pdf_arr_name: self._src}, **self.kernel_params)
+            else:
+                self._apply_links(nat.AA_ODD)
+                even(**{self._pdf_arr_name: self._src}, **self.kernel_params)
+            self._aa_next_odd = not self._aa_next_odd
+
+    def post_run(self):
+        """Density and velocity of the latest populations into the host fields (lbstep.py:311-314)."""
+        d = self._descriptor()
+        if self._pattern == "pull":
+            last = nat.PULL
+        else:
+            last = nat.AA_EVEN if self._aa_next_odd else nat.AA_ODD
+        lib = nat.load()
+        nat.check(lib.lbm355_get_macroscopic(ctypes.byref(d), ctypes.c_void_p(self._src.data_ptr()),
+                                             ctypes.c_void_p(self._rho_dev.data_ptr()),
+                                             ctypes.c_void_p(self._u_dev.data_ptr()), last,
+                                             nat.current_stream_handle()))
+        D = self.dim
+        inner = (slice(1, -1),) * D
+        self._rho_host[...] = self._rho_dev[inner].cpu().numpy().T
+        for a in range(D):
+            self._u_host[..., a] = self._u_dev[a][inner].cpu().numpy().T
+
+    def run(self, time_steps):
+        self.pre_run()
+        for _ in range(time_steps):
+            self.time_step()
+        self.post_run()
+        self.time_steps_run += time_steps
+
+    def benchmark_run(self, time_steps, number_of_cells=None):
+        if number_of_cells is None:
+            number_of_cells = self.number_of_cells
+        self.pre_run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(time_steps):
+            self.time_step()
+        torch.cuda.synchronize()
+        duration = (time.perf_counter() - t0) / time_steps
+        self.time_steps_run += time_steps
+        self.post_run()
+        return number_of_cells / duration * 1e-6
+
+    def benchmark(self, time_for_benchmark=5, init_time_steps=2, number_of_time_steps_for_estimation='auto'):
+        self.pre_run()
+        for _ in range(init_time_steps):
+            self.time_step()
+        n_est = 20 if number_of_time_steps_for_estimation == 'auto' else number_of_time_steps_for_estimation
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n_est):
+            self.time_step()
+        torch.cuda.synchronize()
+        per = (time.perf_counter() - t0) / n_est
+        steps = max(2, int(time_for_benchmark / max(per, 1e-9)))
+        steps += steps % 2
+        return self.benchmark_run(steps)
+
+    def write_vtk(self):
+        raise NotImplementedError("VTK output is not part of the GPU path")
+
+    def run_iterative_initialization(self, *args, **kwargs):
+        raise NotImplementedError("iterative initialisation is not implemented on the GPU path")

@@ -72,8 +72,10 @@ def _mono_exponents(polys, dim):
 class OracleCollision:
     """Collision operator of one method on arrays shaped (Q, ncells)."""
 
-    def __init__(self, method, bindings=None):
+    def __init__(self, method, bindings=None, smagorinsky=None):
+        """``smagorinsky``: Smagorinsky constant Cs (turbulence_models.py:25-84) or None."""
         self.m = method
+        self.smagorinsky = smagorinsky
         self.bind = dict(bindings or {})
         self.stencil = method.stencil
         self.Q = method.stencil.Q
@@ -263,7 +265,10 @@ class OracleCollision:
             for i, p in enumerate(self.ceq_polys):
                 if p is None:
                     assert rates[i] == 0.0, "the zeroth cumulant must not be relaxed"
-            Cp = C + rates[:, None] * (ceq - C)
+            rates = np.repeat(rates[:, None], n, axis=1)
+            if self.smagorinsky is not None:
+                rates = self._smagorinsky_rates(f, rho, drho, u, rates)
+            Cp = C + rates * (ceq - C)
             cpost = self.Pinv @ Cp
             kpost = self._central_from_cumulants(cpost, rho, u3, kappa[self.midx[(0, 0, 0)]])
             if self.fm is not None:
@@ -273,6 +278,31 @@ class OracleCollision:
         if self.bg is not None:
             mpost = mpost - self.bg[:, None]
         return self.Mmono_inv @ mpost, rho, u_out
+
+    def _smagorinsky_rates(self, f, rho, drho, u, rates):
+        """Replace the shear rate by 2 / (tau0 + sqrt(18 Cs^2 Pi + tau0^2)) with
+        Pi = sqrt(2 sum_ab Pi_neq_ab^2) / rho_ref, Pi_neq = sum_i c_ia c_ib (f_i - f_eq_i) and
+        f_eq the method's equilibrium populations at (rho, u) (turbulence_models.py:59-71,
+        utils.py:4-13)."""
+        from lbmpy_amd.moments import is_shear_moment
+        feq = equilibrium_pdfs_from_eq_velocity(self, rho, u)
+        fneq = f - feq
+        D = self.D
+        pi2 = np.zeros_like(rho)
+        for a in range(D):
+            for b in range(D):
+                p_ab = (self.c[:, a] * self.c[:, b]) @ fneq
+                pi2 = pi2 + p_ab * p_ab
+        ref = rho if self.m.compressible else np.ones_like(rho)
+        Pi = np.sqrt(2.0 * pi2) / ref
+        shear = [i for i, p in enumerate(self.polys) if is_shear_moment(p, D)]
+        omega_s = self.rates[shear[0]]
+        tau0 = 1.0 / omega_s
+        w = 2.0 / (tau0 + np.sqrt(18.0 * self.smagorinsky ** 2 * Pi + tau0 ** 2))
+        out = rates.copy()
+        for i in shear:
+            out[i] = w
+        return out
 
     def _leibniz_terms(self, e):
         """Pick j with e_j > 0; return (j, list of (binomial, r)) for r <= e - e_j."""
@@ -366,8 +396,11 @@ class OracleDomain:
     Arrays are numpy float64 with shape (Q, nz+2, ny+2, nx+2) for 3-D and
     (Q, ny+2, nx+2) for 2-D (the ``fzyx`` layout)."""
 
-    def __init__(self, method, shape, faces=None, pattern="pull", bindings=None):
+    def __init__(self, method, shape, faces=None, pattern="pull", bindings=None, obstacle=None):
+        """``obstacle``: optional bool array [x, y, (z)] over the interior marking NoSlip cells
+        (what ``set_boundary(NoSlip(), mask_callback=...)`` flags)."""
         self.coll = OracleCollision(method, bindings)
+        self.obstacle = None if obstacle is None else np.asarray(obstacle, dtype=bool)
         self.method = method
         self.D = method.stencil.D
         self.Q = method.stencil.Q
@@ -405,25 +438,40 @@ class OracleDomain:
 
     def wall_links(self):
         """Index list (x, y[, z], dir) in ghost coordinates of fluid cells whose neighbour in
-        direction dir is a wall cell (what pystencils' index-list creation yields for a NoSlip
-        flag on the ghost layer)."""
+        direction dir is a boundary cell: a NoSlip ghost plane or an obstacle cell (what
+        pystencils' index-list creation yields for NoSlip flags, boundaries/boundaryhandling.py)."""
         if hasattr(self, "_links"):
             return self._links
         n = self.shape
-        walls = [a for a in range(self.D) if self.faces.get(a) == "noslip"]
-        grids = np.meshgrid(*[np.arange(1, n[a] + 1) for a in range(self.D)], indexing="ij")
+        D = self.D
+        flags = np.zeros(tuple(k + 2 for k in n), dtype=bool)          # [x, y, (z)] ghost-inclusive
+        for a in range(D):
+            if self.faces.get(a) == "noslip":
+                lo = [slice(None)] * D
+                hi = [slice(None)] * D
+                lo[a], hi[a] = 0, -1
+                flags[tuple(lo)] = True
+                flags[tuple(hi)] = True
+        if self.obstacle is not None:
+            flags[(slice(1, -1),) * D] |= self.obstacle
+        self.flags = flags
+        grids = np.meshgrid(*[np.arange(1, n[a] + 1) for a in range(D)], indexing="ij")
         cells = np.stack([g.reshape(-1) for g in grids], axis=1)            # (ncells, D)
+        cells = cells[~flags[tuple(cells[:, a] for a in range(D))]]         # fluid cells only
         rows = []
-        if walls:
-            for d in range(self.Q):
-                nb = cells + self.c[d][None, :]
-                hit = np.zeros(len(cells), dtype=bool)
-                for a in walls:
-                    hit |= (nb[:, a] == 0) | (nb[:, a] == n[a] + 1)
-                sel = cells[hit]
-                rows.append(np.concatenate([sel, np.full((len(sel), 1), d)], axis=1))
-        self._links = np.concatenate(rows) if rows else np.zeros((0, self.D + 1), dtype=int)
+        for d in range(self.Q):
+            # neighbours are addressed unwrapped: NoSlip planes flag the whole ghost row,
+            # including the ghost columns of periodic axes, as pystencils' flag field does
+            nb = cells + self.c[d][None, :]
+            hit = flags[tuple(nb[:, a] for a in range(D))]
+            sel = cells[hit]
+            rows.append(np.concatenate([sel, np.full((len(sel), 1), d)], axis=1))
+        self._links = np.concatenate(rows) if rows else np.zeros((0, D + 1), dtype=int)
         return self._links
+
+    def _link_neighbours(self, links):
+        d = links[:, self.D]
+        return [links[:, a] + self.c[d, a] for a in range(self.D)]
 
     def noslip_pull(self, arr):
         """f_in(inv(d)) at the wall neighbour := f_out(d) at the fluid cell (boundaryconditions.py:138)."""
@@ -432,7 +480,7 @@ class OracleDomain:
             return
         d = links[:, self.D]
         cell = [links[:, a] for a in range(self.D)]
-        nb = [links[:, a] + self.c[d, a] for a in range(self.D)]
+        nb = self._link_neighbours(links)
         arr[(self.inv[d],) + tuple(reversed(nb))] = arr[(d,) + tuple(reversed(cell))]
 
     def _neighbor_view(self, arr, q, off):
@@ -492,7 +540,7 @@ class OracleDomain:
             return
         d = links[:, self.D]
         cell = tuple(reversed([links[:, a] for a in range(self.D)]))
-        nb = tuple(reversed([links[:, a] + self.c[d, a] for a in range(self.D)]))
+        nb = tuple(reversed(self._link_neighbours(links)))
         if prev_even:
             arr[(d,) + nb] = arr[(self.inv[d],) + cell]
         else:
@@ -530,13 +578,14 @@ class OracleDomain:
         return self.interior().copy()
 
     def pdfs_in_natural_layout(self):
-        """Interior PDFs in pull layout (slot i holds f_i); for AA after an EVEN kernel the
-        slots hold swapped values and are read back through the accessor."""
-        f = self.pdfs_interior()
-        if self.pattern == "aa" and self.aa_parity == 0:
-            # last kernel was EVEN (or initialisation): slot inv(i) holds f_i
-            return f[self.inv]
-        return f
+        """Post-collision populations of the last step at each cell (slot i holds f*_i(x)),
+        read through the write accessor of the last kernel (macroscopic_value_kernels.py:14-26):
+        pull: dst[i][x]; AA after EVEN (or initialisation): f[inv i][x]; AA after ODD: f[i][x + c_i]."""
+        if self.pattern != "aa":
+            return self.pdfs_interior()
+        if self.aa_parity == 0:
+            return self.pdfs_interior()[self.inv]
+        return np.stack([self._neighbor_view(self.src, i, self.c[i]) for i in range(self.Q)])
 
 
 def equilibrium_pdfs(method, rho, u, bindings=None):
@@ -574,6 +623,19 @@ def equilibrium_pdfs(method, rho, u, bindings=None):
             m = m - coll.bg[:, None]
         feq = coll.Mmono_inv @ m
     return feq.reshape((coll.Q,) + shape)
+
+
+def equilibrium_pdfs_from_eq_velocity(coll, rho, u):
+    """Stored-format equilibrium populations of a central-moment/cumulant method at the
+    equilibrium velocity u (no force shift applied): Gaussian central moments times rho,
+    shifted to raw moments, minus the background, M^-1."""
+    n = rho.shape[0]
+    u3 = [u[a] if a < coll.D else np.zeros(n) for a in range(3)]
+    keq = np.stack([_gauss_central(e, rho, coll.D) for e in coll.monos])
+    m = coll._shift(keq, u3)
+    if coll.bg is not None:
+        m = m - coll.bg[:, None]
+    return coll.Mmono_inv @ m
 
 
 def _gauss_central(e, rho, dim):

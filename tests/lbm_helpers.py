@@ -64,6 +64,7 @@ def gpu_run(kernels, method, shape, faces, f0, steps, data_type="float64", ghost
     storage_view(src, D)[sl] = torch.from_numpy(init).to(src.dtype).cuda()
     bindings = bindings or {}
     if aa:
+        assert steps % 2 == 0, "compare AA runs after an even number of steps (EVEN kernel last)"
         even, odd = kernels
         for t in range(steps):
             (odd if t % 2 == 0 else even)(src=src, **bindings)

@@ -115,3 +115,24 @@ def test_row_lds_kernels_fp64(case):
     print(f"rowlds case {st} {meth.name} {pattern} {shape}", flush=True)
     got = gpu_run(gpu_kernels(kw), m, shape, fdict, f0, steps)
     assert np.abs(got - ref).max() <= FP64_TOL
+
+
+@pytest.mark.parametrize("pattern", ["aa", "pull"])
+def test_cumulant_smagorinsky_fp64(pattern):
+    """Smagorinsky model fused into the cumulant sweep (turbulence_models.py:25-84), Cs = 0.12."""
+    from lbmpy_amd import SubgridScaleModel
+    from oracle.lbm_oracle import OracleDomain
+    kw = dict(stencil=LBStencil(Stencil.D3Q27), method=Method.CUMULANT, relaxation_rate=1.99, compressible=True,
+              streaming_pattern=pattern, subgrid_scale_model=SubgridScaleModel.SMAGORINSKY, force=(1e-5, 0, 0))
+    m = create_lb_method(LBMConfig(**kw))
+    shape = (8, 6, 6)
+    f0 = seeded_state(m, shape, seed=6, du=0.05, noise=1e-3)
+    fdict = {0: "periodic", 1: "periodic", 2: "periodic"}
+    dom = OracleDomain(m, shape, faces=fdict, pattern=pattern)
+    dom.coll.smagorinsky = 0.12
+    dom.set_pdfs_interior(f0[dom.inv] if pattern == "aa" else f0)
+    for _ in range(12):
+        dom.step()
+    ref = dom.pdfs_in_natural_layout()
+    got = gpu_run(gpu_kernels(kw), m, shape, fdict, f0, 12)
+    assert np.abs(got - ref).max() <= FP64_TOL
