@@ -83,7 +83,11 @@ def test_poiseuille_channel_gpu(stencil, pattern):
     for _ in range(60):
         step.run(1000)
         ux = step.velocity[:, :] if st.D == 2 else step.velocity[:, :, :]
-        prof = ux[..., 0].mean(axis=0) if st.D == 2 else ux[..., 0].mean(axis=(0, 2))
+        rho = step.density[:, :] if st.D == 2 else step.density[:, :, :]
+        # like lbmpy's step.velocity, the getter reads the post-collision populations
+        # (lbstep.py:311-314): sum c f* / rho + F/(2 rho) = physical velocity + F / rho
+        phys = ux[..., 0] - force / rho
+        prof = phys.mean(axis=0) if st.D == 2 else phys.mean(axis=(0, 2))
         if last is not None and abs(prof.max() / last - 1) < 5e-6:
             break
         last = prof.max()
