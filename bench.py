@@ -102,6 +102,9 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--dtype", default="float32", choices=["float32", "float64", "float16"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="after the timed region, copy the PDF field once (a plain copy of known size, "
+                         "used to scale the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (pass --gpus 1)", file=sys.stderr)
@@ -170,6 +173,30 @@ def main():
     achieved = bytes_per_cell * cells / (ms_per_step / 1e3) / 1e9
 
     finite = bool(torch.isfinite(storage[:, ::16, ::16, ::16]).all().item())
+    copy_gbs = None
+    if args.calibrate:
+        # plain copies of the PDF field (known size) with regular and non-temporal accesses:
+        # the counter scale for FETCH_SIZE / WRITE_SIZE and the achievable copy bandwidth
+        import ctypes
+        from lbmpy_amd import _native as nat
+        lib = nat.load()
+        other = torch.empty_like(storage)
+        nfl = storage.numel() * storage.element_size() // 4
+        copy_gbs = {}
+        for nt in (0, 1):
+            nat.check(lib.lbm355_stream_copy(ctypes.c_void_p(storage.data_ptr()), ctypes.c_void_p(other.data_ptr()),
+                                             nfl, nt, nat.current_stream_handle()))
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(5):
+                nat.check(lib.lbm355_stream_copy(ctypes.c_void_p(storage.data_ptr()),
+                                                 ctypes.c_void_p(other.data_ptr()), nfl, nt,
+                                                 nat.current_stream_handle()))
+            b.record()
+            torch.cuda.synchronize()
+            copy_gbs["nontemporal" if nt else "regular"] = round(2 * nfl * 4 * 5 / (a.elapsed_time(b) / 1e3) / 1e9, 1)
+        print(f"calibration copy: {nfl * 4} bytes read and written per dispatch", flush=True)
+        del other
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
         "metric": "MLUPS & % HBM roofline (D3Q27 cumulant 256^3), per-form err vs FP64 oracle",
@@ -182,7 +209,8 @@ def main():
                    "cells": cells, "bytes_per_cell_step": bytes_per_cell},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel_ms": {"aa_odd": round(odd_ms, 4), "aa_even": round(even_ms, 4)}},
+                     "kernel_ms": {"aa_odd": round(odd_ms, 4), "aa_even": round(even_ms, 4)},
+                     "copy_gbs": copy_gbs},
         "cpu_baseline": cpu,
         "finite": finite,
     }

@@ -130,6 +130,11 @@ int lbm355_set_equilibrium(const lbm355_sweep_desc* desc, void* pdf, const doubl
 int lbm355_get_macroscopic(const lbm355_sweep_desc* desc, const void* pdf, double* density, double* velocity,
                            int32_t last_pattern, void* stream);
 
+/* Plain FP32 copy of n_floats elements, with regular (nontemporal = 0) or non-temporal accesses as
+ * the sweeps use them.  No lbmpy counterpart: a copy of known size that calibrates achievable
+ * bandwidth and the FETCH_SIZE / WRITE_SIZE counters against the sweep. */
+int lbm355_stream_copy(const void* src, void* dst, int64_t n_floats, int32_t nontemporal, void* stream);
+
 /* sizeof(lbm355_sweep_desc), for FFI layout checks. */
 int64_t lbm355_desc_size(void);
 

@@ -37,8 +37,8 @@ class SweepDesc(ctypes.Structure):
 
 
 EXPORTED = ("lbm355_sweep", "lbm355_table_elements", "lbm355_noslip_links", "lbm355_reduce_conserved",
-            "lbm355_set_equilibrium", "lbm355_get_macroscopic", "lbm355_desc_size", "lbm355_last_error",
-            "lbm355_version")
+            "lbm355_set_equilibrium", "lbm355_get_macroscopic", "lbm355_stream_copy", "lbm355_desc_size",
+            "lbm355_last_error", "lbm355_version")
 
 _lib = None
 
@@ -68,6 +68,8 @@ def load(path=None):
     lib.lbm355_get_macroscopic.argtypes = [ctypes.POINTER(SweepDesc), c_void_p, c_void_p, c_void_p, c_int32,
                                            c_void_p]
     lib.lbm355_get_macroscopic.restype = c_int32
+    lib.lbm355_stream_copy.argtypes = [c_void_p, c_void_p, c_int64, c_int32, c_void_p]
+    lib.lbm355_stream_copy.restype = c_int32
     lib.lbm355_desc_size.argtypes = []
     lib.lbm355_desc_size.restype = c_int64
     if lib.lbm355_desc_size() != ctypes.sizeof(SweepDesc):

@@ -1,0 +1,228 @@
+// Central-moment collision (lbmpy methods/momentbased/centralmomentbasedmethod.py:305-394 with the
+// binomial chimera transform of moment_transforms/centralmomenttransforms.py:168-483).
+//
+// lbmpy relaxes polynomial central moments K = P kappa:  K* = K + D (K_eq - K) + F_K.  The map
+// f -> kappa is linear for a given u (kappa = T(u) f, T = binomial shift o raw moments), so
+//     f* = f + T(u)^-1 [ P^-1 D P (kappa_eq - kappa) + P^-1 F_K ],
+// which is how this kernel evaluates it: the change of the populations is computed directly,
+// so no O(1) background value is ever subtracted (zero-centred storage keeps its precision).
+//   dk     = kappa_eq - kappa  (27 monomial central moments, canonical index 9 ex + 3 ey + ez);
+//            kappa = kt + kappa_w(u) with kt the central moments of the stored deviations and
+//            kappa_w those of the lattice weights; kappa_eq - gamma comes from a polynomial table
+//            in (drho, u), kappa_w - gamma = beta(u) in closed form (collide_cumulant.h).
+//   relax  = r0 dk + sum_k c_k Pinv[:,k] (P[k,:] . dk)         (rank-one corrections, as for MRT)
+//   force  Simple/Luo: T^-1 P^-1 F_K = L (the population force vector itself);
+//          Guo:        (I - D/2) in polynomial space -> (1 - r0/2) L - 1/2 sum_k c_k T^-1 Pinv[:,k] (P[k,:] . T L)
+//   df     = T(u)^-1 (relax + force part);  f* = f + df.
+// Stencils D2Q9, D3Q27 use the chimera in both directions; D3Q19 uses the chimera forward
+// (absent directions are zeros) and a 19 x 27 matrix M^-1 after the inverse shift.
+#pragma once
+#include "lbm_common.h"
+#include "collide_cumulant.h"
+
+namespace lbm355 {
+
+struct CentralTab {
+  static constexpr int EQ = 0,                 // [27][20]: (1, drho) x (1, ux, uy, uz, ux2, uy2, uz2, uxuy, uxuz, uyuz)
+      R0 = 540, COEF = 541,                    // base rate, [27] active coefficients r_k - r0
+      PROW = 568, PINV = 568 + 729,            // [27 active][27 monomials]
+      L0 = 568 + 1458, LU = L0 + 27,           // force: L_i = L0_i + LU_i . u  (population space)
+      FMODE = LU + 81,                         // 0 none, 1 Simple/Luo, 2 Guo
+      MINV = FMODE + 1,                        // [27 dirs][27 monomials] (D3Q19 backward)
+      SIZE = MINV + 729;
+};
+
+template <class S, class T>
+__device__ __forceinline__ void load_grid(const T (&f)[S::Q], T (&m)[3][3][3]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int i = dir_index<S>(a - 1, b - 1, c - 1);
+        m[a][b][c] = i >= 0 ? f[i] : T(0);
+      }
+}
+
+// populations on the 3x3x3 grid -> raw moments -> central moments about u (in place)
+// (2-D stencils sit in the cz = 0 plane: the z passes then act as the identity on it)
+template <class S, class T>
+__device__ __forceinline__ void forward_central(T (&m)[3][3][3], const T (&u)[3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) chim_fwd(m[i][j][0], m[i][j][1], m[i][j][2]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) chim_fwd(m[i][0][k], m[i][1][k], m[i][2][k]);
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) chim_fwd(m[0][j][k], m[1][j][k], m[2][j][k]);
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) shift_fwd(m[0][j][k], m[1][j][k], m[2][j][k], u[0]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) shift_fwd(m[i][0][k], m[i][1][k], m[i][2][k], u[1]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) shift_fwd(m[i][j][0], m[i][j][1], m[i][j][2], u[2]);
+}
+
+// central moments about u -> populations (the inverse of forward_central), result on the grid
+template <class S, class T>
+__device__ __forceinline__ void backward_central(T (&m)[3][3][3], const T (&u)[3], const T* __restrict__ minv,
+                                                 T (&out)[S::Q]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) shift_bwd(m[i][j][0], m[i][j][1], m[i][j][2], u[2]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) shift_bwd(m[i][0][k], m[i][1][k], m[i][2][k], u[1]);
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) shift_bwd(m[0][j][k], m[1][j][k], m[2][j][k], u[0]);
+  if (S::Q == 19) {
+    // raw moments of the 19-moment set -> populations by the tabulated inverse moment matrix
+#pragma unroll
+    for (int i = 0; i < S::Q; ++i) {
+      T acc = 0;
+#pragma unroll
+      for (int e = 0; e < 27; ++e) acc += minv[27 * i + e] * m[e / 9][(e / 3) % 3][e % 3];
+      out[i] = acc;
+    }
+    return;
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) chim_bwd(m[0][j][k], m[1][j][k], m[2][j][k]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) chim_bwd(m[i][0][k], m[i][1][k], m[i][2][k]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) chim_bwd(m[i][j][0], m[i][j][1], m[i][j][2]);
+#pragma unroll
+  for (int i = 0; i < S::Q; ++i) out[i] = m[S::c[i][0] + 1][S::c[i][1] + 1][S::c[i][2] + 1];
+}
+
+template <class T, int EX, int EY, int EZ>
+__device__ __forceinline__ T beta_e(const T (&u)[3], const T (&u2)[3]) {
+  return beta<EX, EY, EZ, T>(u, u2);
+}
+
+template <class S, class T>
+__device__ __forceinline__ void collide_central(T (&f)[S::Q], const SweepArgs<T>& a, T& rho_out_v, T (&u_out_v)[3]) {
+  constexpr int Q = S::Q, D = S::D;
+  using L = CentralTab;
+  const T* __restrict__ tab = a.tab;
+
+  T m[3][3][3];
+  load_grid<S, T>(f, m);
+  // density / velocity from the raw sums (computed again by the chimera below)
+  T s = 0, j[3] = {0, 0, 0};
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    s += f[i];
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+      if (S::c[i][d] != 0) j[d] += (S::c[i][d] > 0) ? f[i] : -f[i];
+  }
+  const bool comp = a.flags & LBM355_FLAG_COMPRESSIBLE;
+  const T drho = s;  // deviation storage (absolute input is shifted by the caller)
+  const T rho = T(1) + drho;
+  const T inv_ref = comp ? T(1) / rho : T(1);
+  T u[3] = {0, 0, 0};
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const T jr = j[d] * inv_ref, sh = T(0.5) * a.force[d] * inv_ref;
+    u[d] = (a.flags & LBM355_FLAG_EQ_SHIFT) ? jr + sh : jr;
+    u_out_v[d] = (a.flags & LBM355_FLAG_FORCE) ? jr + sh : jr;
+  }
+  rho_out_v = rho;
+  const T u2[3] = {u[0] * u[0], u[1] * u[1], u[2] * u[2]};
+  const T um[10] = {T(1), u[0], u[1], u[2], u2[0], u2[1], u2[2], u[0] * u[1], u[0] * u[2], u[1] * u[2]};
+
+  forward_central<S, T>(m, u);
+
+  // dk = (kappa_eq - gamma) - beta - kt
+  T dk[27];
+#define LBM_DK(EX, EY, EZ)                                                                        \
+  {                                                                                               \
+    constexpr int e = 9 * EX + 3 * EY + EZ;                                                       \
+    T eq = 0;                                                                                     \
+    _Pragma("unroll") for (int t = 0; t < 10; ++t) eq += (tab[L::EQ + 20 * e + t] + tab[L::EQ + 20 * e + 10 + t] * drho) * um[t]; \
+    dk[e] = eq - beta_e<T, EX, EY, EZ>(u, u2) - m[EX][EY][EZ];                                    \
+  }
+  LBM_DK(0, 0, 0) LBM_DK(0, 0, 1) LBM_DK(0, 0, 2) LBM_DK(0, 1, 0) LBM_DK(0, 1, 1) LBM_DK(0, 1, 2)
+  LBM_DK(0, 2, 0) LBM_DK(0, 2, 1) LBM_DK(0, 2, 2) LBM_DK(1, 0, 0) LBM_DK(1, 0, 1) LBM_DK(1, 0, 2)
+  LBM_DK(1, 1, 0) LBM_DK(1, 1, 1) LBM_DK(1, 1, 2) LBM_DK(1, 2, 0) LBM_DK(1, 2, 1) LBM_DK(1, 2, 2)
+  LBM_DK(2, 0, 0) LBM_DK(2, 0, 1) LBM_DK(2, 0, 2) LBM_DK(2, 1, 0) LBM_DK(2, 1, 1) LBM_DK(2, 1, 2)
+  LBM_DK(2, 2, 0) LBM_DK(2, 2, 1) LBM_DK(2, 2, 2)
+#undef LBM_DK
+
+  const T r0 = tab[L::R0];
+  T dkp[27];
+#pragma unroll
+  for (int e = 0; e < 27; ++e) dkp[e] = r0 * dk[e];
+  for (int k = 0; k < a.nactive; ++k) {
+    const T* prow = tab + L::PROW + 27 * k;
+    const T* pinv = tab + L::PINV + 27 * k;
+    T sk = 0;
+#pragma unroll
+    for (int e = 0; e < 27; ++e) sk += prow[e] * dk[e];
+    sk *= tab[L::COEF + k];
+#pragma unroll
+    for (int e = 0; e < 27; ++e) dkp[e] += sk * pinv[e];
+  }
+
+  // force vector in population space (affine in u) and its Guo correction
+  const int fmode = (int)tab[L::FMODE];
+  T fl[Q];
+  if (fmode != 0) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      T v = tab[L::L0 + i];
+#pragma unroll
+      for (int d = 0; d < D; ++d) v += tab[L::LU + 3 * i + d] * u[d];
+      fl[i] = v;
+    }
+    if (fmode == 2 && a.nactive > 0) {
+      T kl[3][3][3];
+      load_grid<S, T>(fl, kl);
+      forward_central<S, T>(kl, u);
+      for (int k = 0; k < a.nactive; ++k) {
+        const T* prow = tab + L::PROW + 27 * k;
+        const T* pinv = tab + L::PINV + 27 * k;
+        T sk = 0;
+#pragma unroll
+        for (int e = 0; e < 27; ++e) sk += prow[e] * kl[e / 9][(e / 3) % 3][e % 3];
+        sk *= T(-0.5) * tab[L::COEF + k];
+#pragma unroll
+        for (int e = 0; e < 27; ++e) dkp[e] += sk * pinv[e];
+      }
+    }
+  }
+
+#pragma unroll
+  for (int e = 0; e < 27; ++e) m[e / 9][(e / 3) % 3][e % 3] = (D == 2 && e % 3 != 0) ? T(0) : dkp[e];
+  T df[Q];
+  backward_central<S, T>(m, u, tab + L::MINV, df);
+  const T fscale = fmode == 2 ? T(1) - T(0.5) * r0 : T(1);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) f[i] += df[i] + (fmode != 0 ? fscale * fl[i] : T(0));
+}
+
+}  // namespace lbm355

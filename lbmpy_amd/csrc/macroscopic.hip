@@ -216,3 +216,30 @@ int lbm355_get_macroscopic(const lbm355_sweep_desc* d, const void* pdf, double* 
 int64_t lbm355_desc_size(void) { return (int64_t)sizeof(lbm355_sweep_desc); }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ bandwidth / counter calibration
+namespace {
+template <int NT>
+__global__ void __launch_bounds__(256) stream_copy_kernel(const float* __restrict__ src, float* __restrict__ dst,
+                                                          int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  if (NT)
+    __builtin_nontemporal_store(__builtin_nontemporal_load(src + t), dst + t);
+  else
+    dst[t] = src[t];
+}
+}  // namespace
+
+extern "C" int lbm355_stream_copy(const void* src, void* dst, int64_t n_floats, int32_t nontemporal, void* stream) {
+  if (!src || !dst || n_floats < 0) return lbm355_set_error(LBM355_ERR_ARG, "bad copy arguments");
+  if (n_floats == 0) return LBM355_OK;
+  const int64_t blocks = (n_floats + 255) / 256;
+  if (nontemporal)
+    hipLaunchKernelGGL(stream_copy_kernel<1>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (const float*)src,
+                       (float*)dst, n_floats);
+  else
+    hipLaunchKernelGGL(stream_copy_kernel<0>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (const float*)src,
+                       (float*)dst, n_floats);
+  return hipGetLastError() == hipSuccess ? LBM355_OK : lbm355_set_error(LBM355_ERR_LAUNCH, "copy launch failed");
+}

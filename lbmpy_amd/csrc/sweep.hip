@@ -17,6 +17,7 @@
 #include "lbm_common.h"
 #include "collide_moment.h"
 #include "collide_cumulant.h"
+#include "collide_central.h"
 
 namespace lbm355 {
 
@@ -64,8 +65,16 @@ __device__ __forceinline__ void store_nt(TS* p, TC v) {
 // is an aligned, contiguous, non-temporal 64-lane access:
 //   read : tile[j][x] = row(y - c_y, z - c_z)[x];  f_i(x) = tile[j][(x - c_x) mod nx]
 //   write: tile[j][x] = f_i(x);  row(y + c_y, z + c_z)[x] = tile[j][(x - c_x) mod nx]
+template <class S, class TC>
+struct Collider<S, LBM355_FAMILY_CENTRAL, TC> {
+  static constexpr bool deviation_form = true;
+  static __device__ __forceinline__ void run(TC (&f)[S::Q], const SweepArgs<TC>& a, TC& rho, TC (&u)[3]) {
+    collide_central<S, TC>(f, a, rho, u);
+  }
+};
+
 template <class S, int FAM, int PAT, bool WALLS, bool ROWLDS, class TS, class TC>
-__global__ void __launch_bounds__(512) sweep_kernel(SweepArgs<TC> a) {
+__global__ void __launch_bounds__(ROWLDS ? 512 : 256) sweep_kernel(SweepArgs<TC> a) {
   constexpr int Q = S::Q, D = S::D;
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   if (!ROWLDS && x >= a.nx) return;
@@ -362,7 +371,9 @@ template <class S, int FAM, class TC>
 void* pick_store(int store, int pat, bool walls, bool rowlds) {
   if (store == LBM355_F64) return pick_pattern<S, FAM, double, TC>(pat, walls, rowlds);
   if (store == LBM355_F32) return pick_pattern<S, FAM, float, TC>(pat, walls, rowlds);
-  if (store == LBM355_F16 && sizeof(TC) == 4) return pick_pattern<S, FAM, _Float16, TC>(pat, walls, rowlds);
+  if constexpr (sizeof(TC) == 4) {
+    if (store == LBM355_F16) return pick_pattern<S, FAM, _Float16, TC>(pat, walls, rowlds);
+  }
   return nullptr;
 }
 
@@ -378,6 +389,13 @@ void* pick_kernel(const lbm355_sweep_desc* d, bool walls, bool rowlds) {
   } else if (d->family == LBM355_FAMILY_CUMULANT) {
     if (d->stencil == LBM355_D3Q27)
       return pick_store<D3Q27, LBM355_FAMILY_CUMULANT, TC>(d->store_dtype, d->pattern, walls, rowlds);
+  } else if (d->family == LBM355_FAMILY_CENTRAL) {
+    if (d->stencil == LBM355_D2Q9)
+      return pick_store<D2Q9, LBM355_FAMILY_CENTRAL, TC>(d->store_dtype, d->pattern, walls, rowlds);
+    if (d->stencil == LBM355_D3Q19)
+      return pick_store<D3Q19, LBM355_FAMILY_CENTRAL, TC>(d->store_dtype, d->pattern, walls, rowlds);
+    if (d->stencil == LBM355_D3Q27)
+      return pick_store<D3Q27, LBM355_FAMILY_CENTRAL, TC>(d->store_dtype, d->pattern, walls, rowlds);
   }
   return nullptr;
 }
@@ -410,7 +428,8 @@ int validate(const lbm355_sweep_desc* d) {
   if (d->store_dtype < 0 || d->store_dtype > 2) return fail(LBM355_ERR_ARG, "bad store dtype");
   if (d->pattern < 0 || d->pattern > 2) return fail(LBM355_ERR_ARG, "bad pattern");
   if (!d->tables) return fail(LBM355_ERR_ARG, "method table pointer is NULL");
-  if (d->family == LBM355_FAMILY_MOMENT && (d->n_active < 0 || d->n_active > Q))
+  if ((d->family == LBM355_FAMILY_MOMENT || d->family == LBM355_FAMILY_CENTRAL) &&
+      (d->n_active < 0 || d->n_active > Q))
     return fail(LBM355_ERR_ARG, "n_active out of range");
   return LBM355_OK;
 }
@@ -513,7 +532,6 @@ int lbm355_sweep(const lbm355_sweep_desc* d, const void* pdf_src, void* pdf_dst,
   if (d->pattern == LBM355_PULL && pdf_src == pdf_dst) return fail(LBM355_ERR_ARG, "pull needs two arrays");
   if (d->family == LBM355_FAMILY_CUMULANT && d->stencil != LBM355_D3Q27)
     return fail(LBM355_ERR_UNSUPPORTED, "cumulant sweep is implemented for D3Q27");
-  if (d->family == LBM355_FAMILY_CENTRAL) return fail(LBM355_ERR_UNSUPPORTED, "central-moment sweep not available");
   if ((d->flags & LBM355_FLAG_SMAGORINSKY) && d->family != LBM355_FAMILY_CUMULANT)
     return fail(LBM355_ERR_UNSUPPORTED, "Smagorinsky is implemented for the cumulant sweep");
   hipStream_t s = (hipStream_t)stream;
@@ -523,10 +541,14 @@ int lbm355_sweep(const lbm355_sweep_desc* d, const void* pdf_src, void* pdf_dst,
 
 int64_t lbm355_table_elements(int32_t stencil, int32_t family) {
   const int Q = stencil_q(stencil);
-  if (!Q) return -1;
+  if (!Q) return fail(LBM355_ERR_ARG, "unknown stencil"), -1;
   if (family == LBM355_FAMILY_MOMENT) return 16 * Q + 2 + 2 * Q * Q;
-  if (family == LBM355_FAMILY_CUMULANT) return CumulantTab::SIZE;
-  return -1;
+  if (family == LBM355_FAMILY_CUMULANT) {
+    if (stencil != LBM355_D3Q27) return fail(LBM355_ERR_UNSUPPORTED, "the cumulant sweep is D3Q27 only"), -1;
+    return CumulantTab::SIZE;
+  }
+  if (family == LBM355_FAMILY_CENTRAL) return CentralTab::SIZE;
+  return fail(LBM355_ERR_ARG, "unknown collision family"), -1;
 }
 
 int lbm355_noslip_links(int32_t stencil, int32_t pattern, int32_t dtype, void* pdf, int64_t stride_y,

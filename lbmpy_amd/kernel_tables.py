@@ -220,10 +220,104 @@ def cumulant_table(method, bindings, dtype=np.float64):
     return rates.astype(dtype), 0
 
 
+# ------------------------------------------------------------------------------- central-moment family
+
+_CANON = [(a, b, c) for a in range(3) for b in range(3) for c in range(3)]   # index 9a + 3b + c
+_UMONO = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (2, 0, 0), (0, 2, 0), (0, 0, 2), (1, 1, 0), (1, 0, 1),
+          (0, 1, 1)]
+
+
+def _gamma(e):
+    g = Fraction(1)
+    for k in e:
+        g *= {0: Fraction(1), 1: Fraction(0), 2: Fraction(1, 3)}[k]
+    return g
+
+
+def central_table(method, bindings, dtype=np.float64):
+    """Table for the central-moment kernel (layout CentralTab in csrc/collide_central.h)."""
+    st = method.stencil
+    D, Q = st.D, st.Q
+    eq = method.equilibrium_distribution
+    if eq.c_s_sq != Fraction(1, 3):
+        raise UnsupportedMethod("c_s^2 must be 1/3")
+    if not eq.continuous:
+        raise UnsupportedMethod("discrete-equilibrium central-moment methods are not supported")
+    if eq.deviation_only:
+        # the truncated delta equilibrium differs from relaxing towards kappa_eq of the full
+        # Maxwellian by terms above second order in u; only the default (full) form is built
+        raise UnsupportedMethod("central-moment methods with delta_equilibrium=True are not supported")
+    polys = list(method.moments)
+    monos = set()
+    for p in polys:
+        for e in p.terms:
+            monos.add(tuple(e))
+    if len(monos) != Q:
+        raise UnsupportedMethod("central-moment polynomials that need de-aliasing are not supported")
+    tab = np.zeros(540 + 1 + 27 + 1458 + 27 + 81 + 1 + 729)
+    EQ, R0, COEF, PROW, PINV, L0, LU, FMODE, MINV = 0, 540, 541, 568, 568 + 729, 568 + 1458, 568 + 1458 + 27, \
+        568 + 1458 + 108, 568 + 1458 + 109
+    # kappa_eq - gamma as a polynomial in (drho, u): rho -> 1 + drho
+    full = eq
+    rho = Poly.const(EQ_VARS, 1) + Poly.var(EQ_VARS, "drho")
+    for idx, e in enumerate(_CANON):
+        if D == 2 and e[2] != 0:
+            continue
+        if e not in monos:
+            continue
+        p = full.monomial_central_moment(e).subs({"rho": rho}) - _gamma(e)
+        used = set()
+        for t, um in enumerate(_UMONO):
+            for k, dr in enumerate((0, 1)):
+                ex = (0, dr) + um
+                c = p.coeff(ex)
+                tab[EQ + 20 * idx + 10 * k + t] = float(c)
+                used.add(ex)
+        rest = [ex for ex in p.terms if ex not in used]
+        if rest:
+            raise UnsupportedMethod(f"central-moment equilibrium term outside the kernel's basis: {rest}")
+    # polynomial matrix in canonical monomial indexing, rates with a base rate + rank-one corrections
+    P = np.zeros((Q, 27))
+    for k, p in enumerate(polys):
+        for e, c in p.terms.items():
+            P[k, _CANON.index(tuple(e))] = float(c)
+    cols = [_CANON.index(e) for e in sorted(monos, key=_CANON.index)]
+    Psub = P[:, cols]
+    Pinv_sub = np.linalg.inv(Psub)
+    Pinv = np.zeros((27, Q))
+    Pinv[cols, :] = Pinv_sub
+    rates = np.array([bind_rate(r, bindings) for r in method.relaxation_rates])
+    uniq, counts = np.unique(rates, return_counts=True)
+    r0 = float(uniq[np.argmax(counts)])
+    tab[R0] = r0
+    n = 0
+    for k in range(Q):
+        if rates[k] != r0:
+            tab[COEF + n] = rates[k] - r0
+            tab[PROW + 27 * n: PROW + 27 * (n + 1)] = P[k]
+            tab[PINV + 27 * n: PINV + 27 * (n + 1)] = Pinv[:, k]
+            n += 1
+    fm = method.force_model
+    if fm is not None:
+        t0, tu = fm.population_terms(method)
+        tab[L0:L0 + Q] = [float(v) for v in t0]
+        for i in range(Q):
+            for a in range(D):
+                tab[LU + 3 * i + a] = float(tu[i][a])
+        tab[FMODE] = 2.0 if type(fm).__name__ == "Guo" else 1.0
+    if Q == 19:
+        Mm = moment_matrix([_CANON[c] for c in cols], st)
+        Mi = np.array([[float(v) for v in r] for r in mat_inv(Mm)])      # (Q dirs, Q monomials)
+        for i in range(Q):
+            for j, c in enumerate(cols):
+                tab[MINV + 27 * i + c] = Mi[i, j]
+    return tab.astype(dtype), n
+
+
 def method_table(method, bindings, dtype=np.float64):
     fam = family_of(method)
     if fam == FAMILY_MOMENT:
         return moment_table(method, bindings, dtype)
     if fam == FAMILY_CUMULANT:
         return cumulant_table(method, bindings, dtype)
-    raise UnsupportedMethod("central-moment sweep is not available on the GPU path")
+    return central_table(method, bindings, dtype)

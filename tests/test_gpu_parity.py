@@ -136,3 +136,33 @@ def test_cumulant_smagorinsky_fp64(pattern):
     ref = dom.pdfs_in_natural_layout()
     got = gpu_run(gpu_kernels(kw), m, shape, fdict, f0, 12)
     assert np.abs(got - ref).max() <= FP64_TOL
+
+
+CM_CASES = [
+    ("D2Q9", dict(relaxation_rate=1.7), ("periodic", "noslip"), "pull", (12, 8)),
+    ("D2Q9", dict(relaxation_rate=1.6, compressible=True, force=(1e-4, 2e-5)), ("periodic", "periodic"), "aa", (64, 6)),
+    ("D3Q19", dict(relaxation_rate=1.8, force=(1e-5, 0, 0)), ("periodic", "noslip", "periodic"), "pull", (10, 8, 6)),
+    ("D3Q19", dict(relaxation_rate=1.75, compressible=True), ("periodic",) * 3, "aa", (64, 4, 3)),
+    ("D3Q27", dict(relaxation_rate=1.9, compressible=True, force=(1e-5, 2e-5, -1e-5)), ("periodic",) * 3, "pull",
+     (8, 6, 10)),
+    ("D3Q27", dict(relaxation_rate=1.7, force=(2e-5, 0, 0), force_model=None), ("periodic", "noslip", "noslip"), "aa",
+     (10, 6, 5)),
+    ("D3Q27", dict(relaxation_rates=[0, 0, 0, 0, 1.7, 1.7, 1.7, 1.6, 1.6, 1.1, 1.2, 1.2, 1.2, 1.3, 1.3, 1.3, 1.4,
+                                     1.5, 1.5, 0.9, 0.8, 0.8, 0.8, 0.7, 0.7, 0.7, 0.6], compressible=True,
+                   force=(1e-5, 0, 0)), ("periodic",) * 3, "pull", (6, 6, 6)),
+]
+
+
+@pytest.mark.parametrize("case", CM_CASES, ids=lambda c: f"{c[0]}-{c[3]}-{'x'.join(map(str, c[4]))}")
+def test_central_moment_family_fp64(case):
+    """Central-moment methods (cascaded moment sets, compressible and incompressible, Guo force)."""
+    st, extra, faces, pattern, shape = case
+    extra = {k: v for k, v in extra.items() if not (k == "force_model" and v is None)}
+    kw = dict(stencil=LBStencil(Stencil[st]), method=Method.CENTRAL_MOMENT, streaming_pattern=pattern, **extra)
+    m = create_lb_method(LBMConfig(**kw))
+    f0 = seeded_state(m, shape, seed=8)
+    fdict = {a: faces[a] for a in range(m.dim)}
+    ref, _ = oracle_run(m, shape, fdict, pattern, f0, 16)
+    print(f"central-moment case {st} {pattern} {shape}", flush=True)
+    got = gpu_run(gpu_kernels(kw), m, shape, fdict, f0, 16)
+    assert np.abs(got - ref).max() <= FP64_TOL
