@@ -52,46 +52,204 @@ def d3q27_cumulant_equilibrium(rho, u, zero_centered=True):
     return out
 
 
-def cpu_baseline(n_cells_side=16, steps=4, budget_s=20.0):
-    """The FP64 oracle (numpy, 1 thread) on a bounded sample of the same workload."""
+def cpu_baseline(n=64, budget_s=12.0):
+    """The C restatement of the FP64 oracle (OpenMP over the host cores this process may use, at most 16)
+    on a bounded sample of the same workload: the TGV at n^3, AA, for about `budget_s` seconds."""
     import numpy as np
-    try:
-        from threadpoolctl import threadpool_limits
-    except ImportError:  # pragma: no cover
-        threadpool_limits = None
     from lbmpy_amd import LBMConfig, LBStencil, Stencil, Method, create_lb_method
-    from oracle.lbm_oracle import OracleDomain, equilibrium_pdfs
-    n = n_cells_side
-    omega = 1.9848
+    from oracle.c_oracle import COracle
+    from oracle.lbm_oracle import equilibrium_pdfs
+    threads = min(16, len(os.sched_getaffinity(0)))
+    u0 = 0.05
+    omega = 1.0 / (3.0 * u0 * (n / (2 * math.pi)) / 1600.0 + 0.5)
     m = create_lb_method(LBMConfig(stencil=LBStencil(Stencil.D3Q27), method=Method.CUMULANT,
-                                   relaxation_rate=omega, compressible=True))
+                                   relaxation_rate=omega, compressible=True, streaming_pattern="aa"))
     k = 2 * np.pi / n
     c = (np.arange(n) + 0.5) * k
     z, y, x = np.meshgrid(c, c, c, indexing="ij")
-    u0 = 0.05
     u = np.stack([u0 * np.sin(x) * np.cos(y) * np.cos(z), -u0 * np.cos(x) * np.sin(y) * np.cos(z), 0 * x])
     rho = 1 + 3 * (u0 ** 2 / 16) * (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2)
-    dom = OracleDomain(m, (n, n, n), pattern="aa")
-    f = equilibrium_pdfs(m, rho, u)
-    dom.set_pdfs_interior(f[dom.inv])
-    ctx = threadpool_limits(limits=1) if threadpool_limits else None
-    if ctx:
-        ctx.__enter__()
-    try:
-        dom.step()
-        t0 = time.perf_counter()
-        done = 0
-        while done < steps or time.perf_counter() - t0 < 1.0:
-            dom.step()
-            done += 1
-            if time.perf_counter() - t0 > budget_s:
-                break
-        dt = time.perf_counter() - t0
-    finally:
-        if ctx:
-            ctx.__exit__(None, None, None)
-    return {"value": n ** 3 * done / dt / 1e6, "unit": "MLUPS", "cores": 1, "kind": "port",
-            "sample": f"numpy FP64 oracle, D3Q27 cumulant AA TGV {n}^3, {done} steps"}
+    inv = np.array(m.stencil.inverse_indices())
+    f = np.ascontiguousarray(equilibrium_pdfs(m, rho, u)[inv])
+    co = COracle(m)
+    co.sweep(f, f, "aa_odd", threads)
+    co.sweep(f, f, "aa_even", threads)
+    t0 = time.perf_counter()
+    done = 0
+    while done < 2 or time.perf_counter() - t0 < budget_s:
+        co.sweep(f, f, "aa_odd", threads)
+        co.sweep(f, f, "aa_even", threads)
+        done += 2
+    dt = time.perf_counter() - t0
+    return {"value": round(n ** 3 * done / dt / 1e6, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"C/OpenMP FP64 oracle (generic moment-matrix + cumulant-recursion restatement), "
+                      f"D3Q27 cumulant AA TGV {n}^3, {done} steps in {dt:.1f} s"}
+
+
+PROFILE_COUNTERS = os.path.join(ROOT, "profiles", "sweep_d3q27_cumulant_aa_f32_256_hbm_counters.md")
+
+
+def profiled_traffic(n, dtype):
+    """HBM bytes per sweep launch from the committed FETCH_SIZE/WRITE_SIZE summary (copy-calibrated),
+    when it covers this workload; otherwise None."""
+    if n != 256 or dtype != "float32" or not os.path.exists(PROFILE_COUNTERS):
+        return None
+    import re
+    with open(PROFILE_COUNTERS) as fh:
+        m = re.search(r"traffic_bytes_per_launch\s*=\s*(\d+)", fh.read())
+    return int(m.group(1)) if m else None
+
+
+def macroscopic_from_aa_storage(storage, zero_centered=True):
+    """(rho, u) in FP64 from an AA field after an even number of steps (slot j holds f_inv(j), so the
+    velocity sums carry -c_j), evaluated plane by plane to bound the temporary memory."""
+    import torch
+    from lbmpy_amd.stencils import LBStencil
+    st = LBStencil("D3Q27")
+    c = torch.tensor([list(d) for d in st], dtype=torch.float64, device=storage.device)
+    nz = storage.shape[1]
+    rho = torch.empty(storage.shape[1:], dtype=torch.float64, device=storage.device)
+    u = torch.empty((3,) + tuple(storage.shape[1:]), dtype=torch.float64, device=storage.device)
+    for z0 in range(0, nz, 32):
+        g = storage[:, z0:z0 + 32].double()
+        r = g.sum(0) + (1.0 if zero_centered else 0.0)
+        rho[z0:z0 + 32] = r
+        u[:, z0:z0 + 32] = -torch.einsum("qa,q...->a...", c, g) / r
+    return rho, u
+
+
+def run_form(dtype, n, steps, warmup, total_steps, u0=0.05):
+    """Initialise the TGV, run `warmup` untimed steps, `steps` timed steps (one event pair per launch on
+    the launch stream), then untimed steps up to `total_steps`; returns timings and (rho, u)."""
+    import torch
+    from lbmpy_amd import (LBMConfig, LBStencil, Stencil, Method, create_lb_function, CreateKernelConfig, Target,
+                           Timestep)
+    from lbmpy_amd.sweep import create_pdf_array
+
+    nu = u0 * (n / (2 * math.pi)) / 1600.0
+    omega = 1.0 / (3.0 * nu + 0.5)
+    compute = "float64" if dtype == "float64" else "float32"
+    cfg = CreateKernelConfig(target=Target.GPU, data_type=dtype, compute_type=compute, ghost_layers=0)
+    base = dict(stencil=LBStencil(Stencil.D3Q27), method=Method.CUMULANT, relaxation_rate=omega,
+                compressible=True, zero_centered=True, streaming_pattern="aa")
+    even = create_lb_function(lbm_config=LBMConfig(**base, timestep=Timestep.EVEN), config=cfg)
+    odd = create_lb_function(lbm_config=LBMConfig(**base, timestep=Timestep.ODD), config=cfg)
+    for k in (even, odd):
+        k.set_faces(["periodic"] * 3)
+
+    pdf = create_pdf_array((n, n, n), 27, ghost_layers=0, dtype=dtype)
+    storage = pdf.permute(3, 2, 1, 0)
+    rho, u = tgv_fields(n, u0)
+    feq = d3q27_cumulant_equilibrium(rho, u)
+    inv = LBStencil("D3Q27").inverse_indices()
+    storage.copy_(feq[inv].to(storage.dtype))            # AA initial layout: slot inv(i) holds f_i
+    del feq, rho, u
+    torch.cuda.synchronize()
+
+    kernels = (odd, even)                                 # the first AA step is ODD
+    for t in range(warmup):
+        kernels[t % 2](src=pdf)
+    torch.cuda.synchronize()
+
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    torch.cuda.synchronize()
+    ev[0].record()
+    for t in range(steps):
+        kernels[t % 2](src=pdf)
+        ev[t + 1].record()
+    torch.cuda.synchronize()
+    per = [ev[i].elapsed_time(ev[i + 1]) for i in range(steps)]
+    ms = ev[0].elapsed_time(ev[-1])
+
+    for t in range(warmup + steps, total_steps):
+        kernels[t % 2](src=pdf)
+    torch.cuda.synchronize()
+    rho_f, u_f = macroscopic_from_aa_storage(storage)
+    finite = bool(torch.isfinite(u_f).all().item())
+    del pdf, storage
+    torch.cuda.empty_cache()
+    return dict(ms=ms, odd_ms=sum(per[0::2]) / len(per[0::2]), even_ms=sum(per[1::2]) / len(per[1::2]),
+                launch_ms=sum(per) / len(per), omega=omega, rho=rho_f, u=u_f, finite=finite)
+
+
+def poiseuille_startup(y, t, force, nu, h, terms=4000):
+    """Start-up of plane Poiseuille flow from rest under a body force (walls at y = 0 and y = h):
+    u = F y (h - y) / (2 nu) - sum_{n odd} 4 F h^2 / (nu pi^3 n^3) sin(n pi y / h) exp(-n^2 pi^2 nu t / h^2)."""
+    import numpy as np
+    u = force * y * (h - y) / (2.0 * nu)
+    n = np.arange(1, 2 * terms, 2, dtype=np.float64)[:, None]
+    u = u - (4.0 * force * h * h / (nu * np.pi ** 3 * n ** 3) * np.sin(n * np.pi * y[None, :] / h)
+             * np.exp(-(n ** 2) * np.pi ** 2 * nu * t / (h * h))).sum(axis=0)
+    return u
+
+
+def run_channel(args):
+    """D3Q19 MRT (weighted-orthogonal) force-driven channel: NoSlip walls in y, periodic x and z, Guo force
+    1e-6 in x, shear rate 1.8 and every other rate 1, FP32 arithmetic on zero-centred PDFs stored in FP64,
+    FP32 and FP16; pull streaming.  Reports MLUPS, the velocity profile's error against the analytic start-up
+    solution and against the FP64-storage run, and the mass drift summed in FP64."""
+    import numpy as np
+    import torch
+    from lbmpy_amd import LBMConfig, LBStencil, Stencil, Method, create_lb_function, CreateKernelConfig, Target
+    from lbmpy_amd.sweep import create_pdf_array
+    nx, ny, nz = args.channel_size
+    steps = args.channel_steps
+    force, omega = 1e-6, 1.8
+    nu = (1.0 / omega - 0.5) / 3.0
+    c = torch.tensor(LBStencil(Stencil.D3Q19).stencil_entries, dtype=torch.float64, device="cuda")
+    y = np.arange(ny) + 0.5
+    u_an = poiseuille_startup(y, steps - 0.5, force, nu, float(ny))
+    forms, ref_profile = {}, None
+    for dt in ("float64", "float32", "float16"):
+        cfg = CreateKernelConfig(target=Target.GPU, data_type=dt, compute_type="float32", ghost_layers=0)
+        k = create_lb_function(lbm_config=LBMConfig(stencil=LBStencil(Stencil.D3Q19), method=Method.MRT,
+                                                    relaxation_rate=omega, force=(force, 0.0, 0.0),
+                                                    zero_centered=True, compressible=False), config=cfg)
+        k.set_faces(["periodic", "noslip", "periodic"])
+        src = create_pdf_array((nx, ny, nz), 19, ghost_layers=0, dtype=dt)
+        dst = create_pdf_array((nx, ny, nz), 19, ghost_layers=0, dtype=dt)
+        mass0 = k.conserved(src)[0]
+        for _ in range(min(args.warmup, steps)):
+            k(src=src, dst=dst)
+            src, dst = dst, src
+        done = min(args.warmup, steps)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        timed = min(args.steps, steps - done)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(timed):
+            k(src=src, dst=dst)
+            src, dst = dst, src
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        for _ in range(steps - done - timed):
+            k(src=src, dst=dst)
+            src, dst = dst, src
+        torch.cuda.synchronize()
+        mass1 = k.conserved(src)[0]
+        col = src.permute(3, 2, 1, 0)[:, nz // 2, :, nx // 2].double()          # (19, ny)
+        rho = 1.0 + col.sum(0)
+        ux = ((c[:, 0:1] * col).sum(0) - force / 2.0) / rho
+        prof = ux.cpu().numpy()
+        umax = float(np.abs(u_an).max())
+        res = {"mlups": round(nx * ny * nz * timed / (ms / 1e3) / 1e6, 1),
+               "achieved_gbs": round(2 * 19 * src.element_size() * nx * ny * nz * timed / (ms / 1e3) / 1e9, 1),
+               "profile_err_vs_analytic_max_rel": float(np.abs(prof - u_an).max() / umax),
+               "mass_drift_rel": float((mass1 - mass0) / (nx * ny * nz)),
+               "u_centre": float(prof[ny // 2]), "u_centre_analytic": float(u_an[ny // 2])}
+        if dt == "float64":
+            ref_profile = prof
+        else:
+            res["profile_err_vs_f64_storage_max_rel"] = float(np.abs(prof - ref_profile).max() / umax)
+        forms[{"float64": "f64-storage/f32", "float32": "f32", "float16": "f16-storage/f32"}[dt]] = res
+        del src, dst
+        torch.cuda.empty_cache()
+    line = {"metric": "D3Q19 MRT channel: MLUPS, profile error vs analytic start-up and vs FP64 storage, mass drift",
+            "workload": f"D3Q19 MRT channel {nx}x{ny}x{nz}, NoSlip y, periodic x/z, Guo F=1e-6, omega_shear=1.8, "
+                        f"{steps} steps, pull, zero-centred, FP32 arithmetic",
+            "unit": "MLUPS", "forms": forms}
+    print(json.dumps(line), flush=True)
 
 
 def main():
@@ -100,119 +258,115 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--dtype", default="float32", choices=["float32", "float64", "float16"])
+    ap.add_argument("--dtype", default="float32", choices=["float32", "float64", "float16"],
+                    help="storage type of the headline form")
+    ap.add_argument("--total-steps", type=int, default=2000,
+                    help="steps each form runs before its velocity/density error vs the FP64 run is taken")
+    ap.add_argument("--forms", default="float64,float16",
+                    help="other storage forms run beside the headline one ('' for none)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="tgv", choices=["tgv", "channel"],
+                    help="tgv: the headline D3Q27 cumulant TGV; channel: D3Q19 MRT channel in three storage forms")
+    ap.add_argument("--channel-size", type=int, nargs=3, default=[512, 256, 256])
+    ap.add_argument("--channel-steps", type=int, default=20000)
     ap.add_argument("--calibrate", action="store_true",
-                    help="after the timed region, copy the PDF field once (a plain copy of known size, "
-                         "used to scale the FETCH_SIZE / WRITE_SIZE counters)")
+                    help="after the timed region, copy the PDF field (a plain copy of known size, used to scale "
+                         "the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (pass --gpus 1)", file=sys.stderr)
         sys.exit(2)
-    if args.steps % 2 or args.warmup % 2:
+    if args.steps % 2 or args.warmup % 2 or args.total_steps % 2:
         print("AA streaming needs even step counts", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "channel":
+        run_channel(args)
+        return
+
     import torch
-    from lbmpy_amd import (LBMConfig, LBStencil, Stencil, Method, create_lb_function, CreateKernelConfig, Target,
-                           Timestep)
-    from lbmpy_amd.sweep import create_pdf_array
-
     n = args.n
-    u0 = 0.05
-    nu = u0 * (n / (2 * math.pi)) / 1600.0
-    omega = 1.0 / (3.0 * nu + 0.5)
-    compute = "float64" if args.dtype == "float64" else "float32"
-    cfg = CreateKernelConfig(target=Target.GPU, data_type=args.dtype, compute_type=compute, ghost_layers=0)
-    base = dict(stencil=LBStencil(Stencil.D3Q27), method=Method.CUMULANT, relaxation_rate=omega,
-                compressible=True, zero_centered=True, streaming_pattern="aa")
-    even = create_lb_function(lbm_config=LBMConfig(**base, timestep=Timestep.EVEN), config=cfg)
-    odd = create_lb_function(lbm_config=LBMConfig(**base, timestep=Timestep.ODD), config=cfg)
-    for k in (even, odd):
-        k.set_faces(["periodic"] * 3)
-
-    pdf = create_pdf_array((n, n, n), 27, ghost_layers=0, dtype=args.dtype)
-    storage = pdf.permute(3, 2, 1, 0)
-    rho, u = tgv_fields(n, u0)
-    feq = d3q27_cumulant_equilibrium(rho, u)
-    from lbmpy_amd.stencils import LBStencil as _S
-    inv = _S("D3Q27").inverse_indices()
-    storage.copy_(feq[inv].to(storage.dtype))            # AA initial layout: slot inv(i) holds f_i
-    del feq, rho, u
-    torch.cuda.synchronize()
-
-    kernels = (odd, even)
-    for t in range(args.warmup):
-        kernels[t % 2](src=pdf)
-    torch.cuda.synchronize()
-
-    # per-launch timing of both parities (events on the launch stream)
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(8)]
-    for j, (a, b) in enumerate(ev):
-        a.record()
-        kernels[j % 2](src=pdf)
-        b.record()
-    torch.cuda.synchronize()
-    per = [a.elapsed_time(b) for a, b in ev]
-    odd_ms = sum(per[0::2]) / 4
-    even_ms = sum(per[1::2]) / 4
-
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    start.record()
-    for t in range(args.steps):
-        kernels[t % 2](src=pdf)
-    stop.record()
-    torch.cuda.synchronize()
-    ms = start.elapsed_time(stop)
-    ms_per_step = ms / args.steps
     cells = n ** 3
-    mlups = cells * args.steps / (ms / 1e3) / 1e6
-    s = {"float64": 8, "float32": 4, "float16": 2}[args.dtype]
-    bytes_per_cell = 2 * 27 * s
-    achieved = bytes_per_cell * cells / (ms_per_step / 1e3) / 1e9
+    total = max(args.total_steps, args.warmup + args.steps)
+    forms = [args.dtype] + [f for f in args.forms.split(",") if f and f != args.dtype]
+    if "float64" in forms:
+        forms = ["float64"] + [f for f in forms if f != "float64"]
+    size = {"float64": 8, "float32": 4, "float16": 2}
+    tag = {"float32": "f32", "float64": "f64", "float16": "f16-storage/f32"}
+    results = {}
+    ref = None
+    for dt in forms:
+        r = run_form(dt, n, args.steps, args.warmup, total)
+        b = 2 * 27 * size[dt]
+        res = {"mlups": round(cells * args.steps / (r["ms"] / 1e3) / 1e6, 1),
+               "ms_per_step": round(r["ms"] / args.steps, 4),
+               "kernel_ms": {"aa_odd": round(r["odd_ms"], 4), "aa_even": round(r["even_ms"], 4)},
+               "achieved_gbs": round(b * cells / (r["launch_ms"] / 1e3) / 1e9, 1), "bytes_per_cell_step": b,
+               "finite": r["finite"]}
+        res["roofline_frac"] = round(res["achieved_gbs"] / HBM_PEAK_GBS, 4)
+        if dt == "float64":
+            ref = (r["rho"], r["u"])
+        elif ref is not None:
+            du = (r["u"] - ref[1])
+            res[f"u_err_vs_f64_after_{total}_steps"] = {
+                "max_rel_u0": float(du.abs().max() / 0.05),
+                "l2_rel": float(du.norm() / ref[1].norm())}
+            res[f"rho_err_vs_f64_after_{total}_steps"] = {"max_abs": float((r["rho"] - ref[0]).abs().max())}
+        res["_launch_ms"], res["_ms"], res["_omega"] = r["launch_ms"], r["ms"], r["omega"]
+        results[dt] = res
+        del r
+        torch.cuda.empty_cache()
 
-    finite = bool(torch.isfinite(storage[:, ::16, ::16, ::16]).all().item())
+    head = results[args.dtype]
     copy_gbs = None
     if args.calibrate:
-        # plain copies of the PDF field (known size) with regular and non-temporal accesses:
-        # the counter scale for FETCH_SIZE / WRITE_SIZE and the achievable copy bandwidth
+        # plain copies of a field of the headline size with regular and non-temporal accesses: the counter
+        # scale for FETCH_SIZE / WRITE_SIZE and the achievable copy bandwidth
         import ctypes
         from lbmpy_amd import _native as nat
         lib = nat.load()
-        other = torch.empty_like(storage)
-        nfl = storage.numel() * storage.element_size() // 4
+        nbytes = 27 * cells * size[args.dtype]
+        a_buf = torch.zeros(nbytes // 4, dtype=torch.float32, device="cuda")
+        b_buf = torch.empty_like(a_buf)
+        nfl = a_buf.numel()
         copy_gbs = {}
         for nt in (0, 1):
-            nat.check(lib.lbm355_stream_copy(ctypes.c_void_p(storage.data_ptr()), ctypes.c_void_p(other.data_ptr()),
+            nat.check(lib.lbm355_stream_copy(ctypes.c_void_p(a_buf.data_ptr()), ctypes.c_void_p(b_buf.data_ptr()),
                                              nfl, nt, nat.current_stream_handle()))
-            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            a.record()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
             for _ in range(5):
-                nat.check(lib.lbm355_stream_copy(ctypes.c_void_p(storage.data_ptr()),
-                                                 ctypes.c_void_p(other.data_ptr()), nfl, nt,
+                nat.check(lib.lbm355_stream_copy(ctypes.c_void_p(a_buf.data_ptr()),
+                                                 ctypes.c_void_p(b_buf.data_ptr()), nfl, nt,
                                                  nat.current_stream_handle()))
-            b.record()
+            e1.record()
             torch.cuda.synchronize()
-            copy_gbs["nontemporal" if nt else "regular"] = round(2 * nfl * 4 * 5 / (a.elapsed_time(b) / 1e3) / 1e9, 1)
+            copy_gbs["nontemporal" if nt else "regular"] = round(2 * nfl * 4 * 5 / (e0.elapsed_time(e1) / 1e3) / 1e9, 1)
         print(f"calibration copy: {nfl * 4} bytes read and written per dispatch", flush=True)
-        del other
+        del a_buf, b_buf
+
     cpu = None if args.no_cpu else cpu_baseline()
+    b = head["bytes_per_cell_step"]
+    achieved = b * cells / (head["_launch_ms"] / 1e3) / 1e9
+    traffic = profiled_traffic(n, args.dtype)
+    forms_out = {tag[k]: {kk: vv for kk, vv in v.items() if not kk.startswith("_")} for k, v in results.items()}
     line = {
         "metric": "MLUPS & % HBM roofline (D3Q27 cumulant 256^3), per-form err vs FP64 oracle",
-        "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": {"float32": "f32", "float64": "f64", "float16": "f16-storage/f32"}[args.dtype],
+        "value": head["mlups"], "unit": "MLUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": tag[args.dtype],
         "data": "synthetic (Taylor-Green vortex initial state)",
         "config": {"workload": f"D3Q27 cumulant AA stream-collide, periodic {n}^3 TGV, u0=0.05, Re=1600 "
-                               f"(omega={omega:.5f}), zero-centred {args.dtype} PDFs",
-                   "cells": cells, "bytes_per_cell_step": bytes_per_cell},
+                               f"(omega={head['_omega']:.5f}), zero-centred {args.dtype} PDFs",
+                   "cells": cells, "bytes_per_cell_step": b},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel_ms": {"aa_odd": round(odd_ms, 4), "aa_even": round(even_ms, 4)},
-                     "copy_gbs": copy_gbs},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "lbm355::sweep_kernel<D3Q27, cumulant, AA odd/even> (both parities, same bytes)",
+                     "launch_ms": round(head["_launch_ms"], 4), "kernel_ms": head["kernel_ms"],
+                     "algorithmic_bytes_per_launch": b * cells, "copy_gbs": copy_gbs},
+        "forms": forms_out,
         "cpu_baseline": cpu,
-        "finite": finite,
+        "finite": all(v["finite"] for v in results.values()),
     }
     print(json.dumps(line), flush=True)
 

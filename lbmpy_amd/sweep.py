@@ -225,6 +225,19 @@ class SweepKernel:
                                    ctypes.c_void_p(u_o.data_ptr() if u_o is not None else 0),
                                    nat.current_stream_handle()))
 
+    def conserved(self, pdf, **bindings):
+        """FP64 sums over the interior on the GPU: (mass, momentum_x, momentum_y, momentum_z) of the stored
+        populations, read in their storage slots (for AA fields only the mass is layout independent).
+        Zero-centred storage counts the rest density 1 per cell.  Synchronises the current stream."""
+        self._check_tensor(pdf)
+        binds = {s: float(bindings.get(s, 1.0)) for s in self.free_symbols}
+        d = self.descriptor(pdf, binds)
+        out = torch.zeros(4, dtype=torch.float64, device=pdf.device)
+        lib = nat.load()
+        nat.check(lib.lbm355_reduce_conserved(ctypes.byref(d), ctypes.c_void_p(pdf.data_ptr()),
+                                              ctypes.c_void_p(out.data_ptr()), nat.current_stream_handle()))
+        return out.cpu().numpy()
+
     def _check_tensor(self, t):
         if not isinstance(t, torch.Tensor) or not t.is_cuda:
             raise TypeError("PDF fields must be torch tensors on the GPU")

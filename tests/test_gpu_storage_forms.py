@@ -1,0 +1,83 @@
+"""Reduced-precision storage on the D3Q19 MRT force-driven channel (NoSlip y, periodic x/z, Guo force 1e-6,
+shear rate 1.8, other rates 1, zero-centred PDFs, FP32 arithmetic), 64 x 48 x 4 cells, 3000 pull steps.
+
+The FP64-storage run is compared with the analytic start-up solution of plane Poiseuille flow (the FP64
+oracle itself is 1.8e-3 from it on this grid, the lattice's discretisation error) and the FP32 and FP16
+storage runs with the FP64-storage run; mass drift is summed in FP64 on the GPU.  With FP32 arithmetic the
+collision conserves mass only to FP32 round-off, so the drift bounds are those of FP32 arithmetic.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SHAPE = (64, 48, 4)
+STEPS = 3000
+FORCE, OMEGA = 1e-6, 1.8
+
+
+def _run(dtype):
+    import torch
+    from lbmpy_amd import LBMConfig, LBStencil, Stencil, Method, create_lb_function, CreateKernelConfig, Target
+    from lbmpy_amd.sweep import create_pdf_array
+    cfg = CreateKernelConfig(target=Target.GPU, data_type=dtype, compute_type="float32", ghost_layers=0)
+    k = create_lb_function(lbm_config=LBMConfig(stencil=LBStencil(Stencil.D3Q19), method=Method.MRT,
+                                                relaxation_rate=OMEGA, force=(FORCE, 0.0, 0.0), zero_centered=True,
+                                                compressible=False), config=cfg)
+    k.set_faces(["periodic", "noslip", "periodic"])
+    src = create_pdf_array(SHAPE, 19, ghost_layers=0, dtype=dtype)
+    dst = create_pdf_array(SHAPE, 19, ghost_layers=0, dtype=dtype)
+    m0 = k.conserved(src)[0]
+    for _ in range(STEPS):
+        k(src=src, dst=dst)
+        src, dst = dst, src
+    m1 = k.conserved(src)[0]
+    st = src.permute(3, 2, 1, 0).double().cpu().numpy()                 # (19, z, y, x)
+    c = np.array(LBStencil(Stencil.D3Q19).stencil_entries, dtype=np.float64)
+    rho = 1.0 + st.sum(0)
+    ux = (np.einsum("q,qzyx->zyx", c[:, 0], st) - FORCE / 2.0) / rho
+    return ux, (m1 - m0) / np.prod(SHAPE)
+
+
+@pytest.fixture(scope="module")
+def runs():
+    out = {}
+    for dt in ("float64", "float32", "float16"):
+        print(f"channel storage form {dt}", flush=True)
+        out[dt] = _run(dt)
+    return out
+
+
+def test_fp64_storage_matches_analytic_startup(runs):
+    import bench
+    ux, drift = runs["float64"]
+    nu = (1.0 / OMEGA - 0.5) / 3.0
+    y = np.arange(SHAPE[1]) + 0.5
+    ua = bench.poiseuille_startup(y, STEPS - 0.5, FORCE, nu, float(SHAPE[1]))
+    # x/z invariance of the flow
+    assert np.abs(ux - ux[:1, :, :1]).max() <= 1e-9
+    err = np.abs(ux[0, :, 0] - ua).max() / np.abs(ua).max()
+    print(f"f64 storage: profile error vs analytic {err:.3e}, mass drift per cell {drift:.3e}", flush=True)
+    assert err <= 3e-3
+    # FP32 arithmetic: each collision conserves mass to FP32 round-off only (about 1e-12 per cell and step)
+    assert abs(drift) <= 1e-7
+
+
+def test_fp32_storage_equals_fp64_storage(runs):
+    """With FP32 arithmetic every stored value is an FP32 number, so FP64 storage holds exactly what FP32
+    storage holds: the two runs agree bit for bit."""
+    np.testing.assert_array_equal(runs["float32"][0], runs["float64"][0])
+    # the FP64 mass sums use atomics, so their last bits depend on the order the blocks finish in
+    assert abs(runs["float32"][1] - runs["float64"][1]) <= 1e-12
+
+
+@pytest.mark.parametrize("dtype,tol,drift_tol", [("float16", 2e-2, 1e-6)])
+def test_reduced_storage_vs_fp64_storage(runs, dtype, tol, drift_tol):
+    ref = runs["float64"][0][0, :, 0]
+    ux, drift = runs[dtype]
+    prof = ux[0, :, 0]
+    err = np.abs(prof - ref).max() / np.abs(ref).max()
+    print(f"{dtype} storage: profile error vs FP64 storage {err:.3e}, mass drift per cell {drift:.3e}", flush=True)
+    assert np.isfinite(prof).all()
+    assert err <= tol
+    assert abs(drift) <= drift_tol
