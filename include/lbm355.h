@@ -59,6 +59,7 @@ extern "C" {
 #define LBM355_FLAG_EQ_SHIFT 4      /* equilibrium velocity shifted by F / (2 rho_ref) */
 #define LBM355_FLAG_FORCE 8         /* a force model is present (macroscopic velocity shift) */
 #define LBM355_FLAG_SMAGORINSKY 16  /* Smagorinsky subgrid model on the shear rate */
+#define LBM355_FLAG_LEARNED_CLOSURE 32 /* learned per-cell eddy viscosity on the shear rate (cumulant, FP32) */
 
 /* error codes */
 #define LBM355_OK 0
@@ -81,8 +82,9 @@ typedef struct lbm355_sweep_desc {
   int32_t n_active;      /* moment family: number of individually relaxed moments */
   double force[3];       /* constant body force */
   double smagorinsky_constant; /* Cs, used with LBM355_FLAG_SMAGORINSKY */
-  double omega_shear;    /* molecular shear relaxation rate (Smagorinsky) */
+  double omega_shear;    /* molecular shear relaxation rate (Smagorinsky, learned closure) */
   const void* tables;    /* device pointer: method table, compute_dtype elements */
+  const void* closure;   /* device pointer: closure parameter block (LBM355_FLAG_LEARNED_CLOSURE), else NULL */
 } lbm355_sweep_desc;
 
 /* One stream-collide time step over the interior.
@@ -134,6 +136,61 @@ int lbm355_get_macroscopic(const lbm355_sweep_desc* desc, const void* pdf, doubl
  * the sweeps use them.  No lbmpy counterpart: a copy of known size that calibrates achievable
  * bandwidth and the FETCH_SIZE / WRITE_SIZE counters against the sweep. */
 int lbm355_stream_copy(const void* src, void* dst, int64_t n_floats, int32_t nontemporal, void* stream);
+
+/* ---- Batched ensembles -------------------------------------------------------------------------
+ * Many independent domains ("members") advanced by one launch per time step.  lbmpy has no batched
+ * sweep: an ensemble there is a Python loop calling one compiled kernel per member and time step
+ * (LatticeBoltzmannStep.time_step, lbstep.py:271-282, once per domain).  Every member is described
+ * by its own lbm355_sweep_desc (sizes, faces, strides, force, method table), as for lbm355_sweep;
+ * all members share stencil, collision family, streaming pattern (pull, or AA) and dtypes.  Each
+ * member's cells go through the same per-cell code as lbm355_sweep, so a member's result is
+ * bit-identical to sweeping it alone. */
+typedef struct lbm355_batch lbm355_batch;
+
+/* Uploads the member table (one device allocation).  pdf_src[k] / pdf_dst[k]: member k's arrays
+ * (equal for AA).  The table keeps the pointers: the arrays must outlive the batch. */
+int lbm355_batch_create(const lbm355_sweep_desc* descs, const void* const* pdf_src, void* const* pdf_dst,
+                        int32_t n_members, lbm355_batch** batch);
+
+/* One time step of every member, one launch, no allocation (capturable in a HIP graph).
+ * step_kind: pull 0 = src -> dst, 1 = dst -> src; AA LBM355_AA_EVEN or LBM355_AA_ODD.
+ * threads per block 64/128/192/256; blocks: persistent grid size (clamped to the work); xcd_order:
+ * give each of the 8 XCDs a contiguous stretch of the tile sequence (needs blocks % 8 == 0). */
+int lbm355_batch_sweep(const lbm355_batch* batch, int32_t step_kind, int32_t threads, int32_t blocks,
+                       int32_t xcd_order, void* stream);
+
+/* Number of 64-cell tiles and of interior cells over all members. */
+int lbm355_batch_info(const lbm355_batch* batch, int64_t* n_tiles, int64_t* n_cells);
+
+/* Frees the member table (synchronous). */
+int lbm355_batch_destroy(lbm355_batch* batch);
+
+/* ---- Learned closure ----------------------------------------------------------------------------
+ * No lbmpy counterpart: a per-cell MLP (10 -> 32 -> 32 -> 1, tanh, BF16 weights, FP32 accumulation
+ * on the MFMA units) that maps a cell's non-equilibrium second moments to an eddy viscosity and
+ * replaces the Smagorinsky rate of the cumulant sweep (turbulence_models.py:25-84) when a sweep
+ * descriptor sets LBM355_FLAG_LEARNED_CLOSURE and `closure`.  The parameter block layout is
+ * lbm355_closure_params_size() bytes (lbmpy_amd/csrc/closure.h); training keeps FP32 master
+ * parameters of lbm355_closure_master_size() floats. */
+int64_t lbm355_closure_params_size(void);
+int32_t lbm355_closure_master_size(void);
+
+/* features[n][10] of cells with non-equilibrium second moments pi_neq[n][6] (xx yy zz xy xz yz)
+ * and density[n], as the fused sweep computes them. */
+int lbm355_closure_features(const float* pi_neq, const float* density, int64_t n, float* features, void* stream);
+
+/* network outputs o[n] for features[n][10] (nu_t = nu_scale * softplus(o)). */
+int lbm355_closure_forward(const void* params, const float* features, int64_t n, float* out, void* stream);
+
+/* gradient of L = (1/n) sum (softplus(o) - targets)^2 into grad[master_size] (overwritten), L into
+ * loss[0]; `blocks` persistent one-wavefront blocks. */
+int lbm355_closure_train_step(const void* params, const float* features, const float* targets, int64_t n,
+                              float* grad, float* loss, int32_t blocks, void* stream);
+
+/* Adam step (`step` >= 1) on the FP32 master parameters with moments m, v; re-packs the BF16
+ * weights and FP32 biases into the parameter block `params`. */
+int lbm355_closure_adam(float* master, const float* grad, float* m, float* v, float lr, float beta1, float beta2,
+                        float eps, int32_t step, void* params, void* stream);
 
 /* sizeof(lbm355_sweep_desc), for FFI layout checks. */
 int64_t lbm355_desc_size(void);

@@ -20,6 +20,7 @@ PULL, AA_EVEN, AA_ODD = 0, 1, 2
 F64, F32, F16 = 0, 1, 2
 FACE_GHOST, FACE_PERIODIC, FACE_NOSLIP = 0, 1, 2
 FLAG_ZERO_CENTERED, FLAG_COMPRESSIBLE, FLAG_EQ_SHIFT, FLAG_FORCE, FLAG_SMAGORINSKY = 1, 2, 4, 8, 16
+FLAG_LEARNED_CLOSURE = 32
 
 DTYPE_CODES = {"float64": F64, "float32": F32, "float16": F16}
 STENCIL_NAMES = {"D2Q9": D2Q9, "D3Q19": D3Q19, "D3Q27": D3Q27}
@@ -33,12 +34,15 @@ class SweepDesc(ctypes.Structure):
                 ("out_stride_y", c_int64), ("out_stride_z", c_int64), ("out_stride_c", c_int64),
                 ("flags", c_int32), ("n_active", c_int32),
                 ("force", c_double * 3), ("smagorinsky_constant", c_double), ("omega_shear", c_double),
-                ("tables", c_void_p)]
+                ("tables", c_void_p), ("closure", c_void_p)]
 
 
 EXPORTED = ("lbm355_sweep", "lbm355_table_elements", "lbm355_noslip_links", "lbm355_reduce_conserved",
             "lbm355_set_equilibrium", "lbm355_get_macroscopic", "lbm355_stream_copy", "lbm355_desc_size",
-            "lbm355_last_error", "lbm355_version")
+            "lbm355_last_error", "lbm355_version", "lbm355_batch_create", "lbm355_batch_sweep",
+            "lbm355_batch_info", "lbm355_batch_destroy", "lbm355_closure_params_size",
+            "lbm355_closure_master_size", "lbm355_closure_features", "lbm355_closure_forward",
+            "lbm355_closure_train_step", "lbm355_closure_adam")
 
 _lib = None
 
@@ -70,6 +74,27 @@ def load(path=None):
     lib.lbm355_get_macroscopic.restype = c_int32
     lib.lbm355_stream_copy.argtypes = [c_void_p, c_void_p, c_int64, c_int32, c_void_p]
     lib.lbm355_stream_copy.restype = c_int32
+    lib.lbm355_batch_create.argtypes = [ctypes.POINTER(SweepDesc), ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
+                                        c_int32, ctypes.POINTER(c_void_p)]
+    lib.lbm355_batch_create.restype = c_int32
+    lib.lbm355_batch_sweep.argtypes = [c_void_p, c_int32, c_int32, c_int32, c_int32, c_void_p]
+    lib.lbm355_batch_sweep.restype = c_int32
+    lib.lbm355_batch_info.argtypes = [c_void_p, ctypes.POINTER(c_int64), ctypes.POINTER(c_int64)]
+    lib.lbm355_batch_info.restype = c_int32
+    lib.lbm355_batch_destroy.argtypes = [c_void_p]
+    lib.lbm355_batch_destroy.restype = c_int32
+    lib.lbm355_closure_params_size.restype = c_int64
+    lib.lbm355_closure_master_size.restype = c_int32
+    lib.lbm355_closure_features.argtypes = [c_void_p, c_void_p, c_int64, c_void_p, c_void_p]
+    lib.lbm355_closure_features.restype = c_int32
+    lib.lbm355_closure_forward.argtypes = [c_void_p, c_void_p, c_int64, c_void_p, c_void_p]
+    lib.lbm355_closure_forward.restype = c_int32
+    lib.lbm355_closure_train_step.argtypes = [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int32,
+                                              c_void_p]
+    lib.lbm355_closure_train_step.restype = c_int32
+    lib.lbm355_closure_adam.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_float, ctypes.c_float,
+                                        ctypes.c_float, ctypes.c_float, c_int32, c_void_p, c_void_p]
+    lib.lbm355_closure_adam.restype = c_int32
     lib.lbm355_desc_size.argtypes = []
     lib.lbm355_desc_size.restype = c_int64
     if lib.lbm355_desc_size() != ctypes.sizeof(SweepDesc):

@@ -1,0 +1,291 @@
+"""Learned subgrid closure for the D3Q27 cumulant sweep.
+
+A per-cell MLP (10 -> 32 -> 32 -> 1, tanh) maps a cell's non-equilibrium second moments to an eddy
+viscosity nu_t = nu_scale * softplus(o); the shear relaxation rate becomes
+omega = 1 / (1/omega_0 + 3 nu_t).  It stands where lbmpy's Smagorinsky model stands
+(``turbulence_models.py:25-84``, ``LBMConfig.subgrid_scale_model``); lbmpy has no learned closure.
+
+Everything runs on the GPU through the C ABI (``include/lbm355.h``):
+  * inference fused into the cumulant sweep (``SweepKernel.set_closure``), BF16 MFMA per wavefront;
+  * ``lbm355_closure_features`` / ``lbm355_closure_forward`` for batches of samples;
+  * training: ``lbm355_closure_train_step`` (forward, backward and weight gradients on BF16 MFMA with
+    FP32 accumulation) and ``lbm355_closure_adam``.
+
+Training data come from the method itself: a fine D3Q27 cumulant run, box-filtered onto the coarse
+grid (:func:`subgrid_samples`).  Torch is used for device memory and for the data plumbing around the
+kernels (filtering, FFT diagnostics), never for the network's arithmetic.
+"""
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _native as nat
+
+IN, IN_PAD, H = 10, 16, 32
+OFF_W1T, OFF_W2T, OFF_W3, OFF_B1, OFF_B2, OFF_B3, MASTER_SIZE = 0, 512, 1536, 1568, 1600, 1632, 1633
+# byte offsets in the device parameter block (struct ClosureParams)
+P_W1T, P_W2T, P_W3, P_B1, P_B2, P_B3, P_NU, P_SHIFT, P_SCALE, P_SIZE = 0, 1024, 3072, 3136, 3264, 3392, 3396, 3400, \
+    3440, 3488
+
+
+def _bf16_bits(a):
+    u = np.asarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class ClosureModel:
+    """Network parameters: an FP32 master copy (training) and the packed device block the kernels read."""
+
+    def __init__(self, device="cuda", seed=0, nu_scale=1e-3):
+        self.lib = nat.load()
+        if self.lib.lbm355_closure_params_size() != P_SIZE or self.lib.lbm355_closure_master_size() != MASTER_SIZE:
+            raise ImportError("closure parameter layout mismatch between closure.h and lbmpy_amd/closure.py")
+        self.device = torch.device(device)
+        rng = np.random.default_rng(seed)
+        master = np.zeros(MASTER_SIZE, dtype=np.float32)
+        w1t = rng.uniform(-1, 1, (H, IN_PAD)) * math.sqrt(6.0 / (IN + H))
+        w1t[:, IN:] = 0.0
+        master[OFF_W1T:OFF_W2T] = w1t.ravel()
+        master[OFF_W2T:OFF_W3] = (rng.uniform(-1, 1, (H, H)) * math.sqrt(6.0 / (2 * H))).ravel()
+        master[OFF_W3:OFF_B1] = rng.uniform(-1, 1, H) * math.sqrt(6.0 / (H + 1))
+        self.master = torch.tensor(master, device=self.device)
+        self.m = torch.zeros_like(self.master)
+        self.v = torch.zeros_like(self.master)
+        self.step = 0
+        self.nu_scale = float(nu_scale)
+        self.in_shift = np.zeros(IN, dtype=np.float32)
+        self.in_scale = np.ones(IN, dtype=np.float32)
+        self.params = torch.zeros(P_SIZE, dtype=torch.uint8, device=self.device)
+        self.pack()
+
+    # ------------------------------------------------------------------ parameter block
+    def pack(self):
+        """Write the device parameter block from the master copy (BF16 weights, FP32 biases)."""
+        m = self.master.cpu().numpy()
+        blk = np.zeros(P_SIZE, dtype=np.uint8)
+        blk[P_W1T:P_W2T] = _bf16_bits(m[OFF_W1T:OFF_W2T]).view(np.uint8)
+        blk[P_W2T:P_W3] = _bf16_bits(m[OFF_W2T:OFF_W3]).view(np.uint8)
+        blk[P_W3:P_B1] = _bf16_bits(m[OFF_W3:OFF_B1]).view(np.uint8)
+        blk[P_B1:P_B2] = m[OFF_B1:OFF_B2].astype(np.float32).view(np.uint8)
+        blk[P_B2:P_B3] = m[OFF_B2:OFF_B3].astype(np.float32).view(np.uint8)
+        blk[P_B3:P_NU] = np.float32(m[OFF_B3]).reshape(1).view(np.uint8)
+        blk[P_NU:P_SHIFT] = np.float32(self.nu_scale).reshape(1).view(np.uint8)
+        blk[P_SHIFT:P_SCALE] = self.in_shift.astype(np.float32).view(np.uint8)
+        blk[P_SCALE:P_SCALE + 40] = self.in_scale.astype(np.float32).view(np.uint8)
+        self.params.copy_(torch.from_numpy(blk))
+
+    def set_normalisation(self, features):
+        """Per-feature shift/scale from a sample of raw features (mean and 1/std)."""
+        f = features.double()
+        self.in_shift = f.mean(0).float().cpu().numpy()
+        std = f.std(0).float().cpu().numpy()
+        self.in_scale = (1.0 / np.maximum(std, 1e-30)).astype(np.float32)
+        self.pack()
+
+    def set_output_bias(self, targets):
+        """Start the output at the mean target: softplus(b3) = mean(y)."""
+        y = float(targets.double().mean().clamp_min(1e-6))
+        b3 = math.log(math.expm1(y)) if y < 20 else y
+        self.master[OFF_B3] = b3
+        self.pack()
+
+    def state(self):
+        return {"master": self.master.cpu().numpy(), "in_shift": self.in_shift, "in_scale": self.in_scale,
+                "nu_scale": self.nu_scale}
+
+    def load_state(self, st):
+        self.master.copy_(torch.as_tensor(np.asarray(st["master"], dtype=np.float32), device=self.device))
+        self.in_shift = np.asarray(st["in_shift"], dtype=np.float32)
+        self.in_scale = np.asarray(st["in_scale"], dtype=np.float32)
+        self.nu_scale = float(st["nu_scale"])
+        self.pack()
+
+    # ------------------------------------------------------------------ kernels
+    def features(self, pi_neq, rho):
+        """Raw features (n, 10) from non-equilibrium second moments (n, 6) and density (n,)."""
+        pi_neq = pi_neq.float().contiguous()
+        rho = rho.float().contiguous()
+        out = torch.empty((pi_neq.shape[0], IN), dtype=torch.float32, device=self.device)
+        nat.check(self.lib.lbm355_closure_features(_ptr(pi_neq), _ptr(rho), pi_neq.shape[0], _ptr(out),
+                                                   nat.current_stream_handle()))
+        return out
+
+    def forward(self, features):
+        """Network outputs o (n,)."""
+        x = features.float().contiguous()
+        out = torch.empty(x.shape[0], dtype=torch.float32, device=self.device)
+        nat.check(self.lib.lbm355_closure_forward(_ptr(self.params), _ptr(x), x.shape[0], _ptr(out),
+                                                  nat.current_stream_handle()))
+        return out
+
+    def eddy_viscosity(self, features):
+        return self.nu_scale * torch.nn.functional.softplus(self.forward(features))
+
+    def gradient(self, features, targets, blocks=1024):
+        """(loss, grad) of L = mean((softplus(o) - y)^2) at the current parameters (GPU kernels)."""
+        x = features.float().contiguous()
+        y = targets.float().contiguous()
+        grad = torch.empty(MASTER_SIZE, dtype=torch.float32, device=self.device)
+        loss = torch.empty(1, dtype=torch.float32, device=self.device)
+        nat.check(self.lib.lbm355_closure_train_step(_ptr(self.params), _ptr(x), _ptr(y), x.shape[0], _ptr(grad),
+                                                     _ptr(loss), blocks, nat.current_stream_handle()))
+        return loss, grad
+
+    def adam_step(self, grad, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        self.step += 1
+        nat.check(self.lib.lbm355_closure_adam(_ptr(self.master), _ptr(grad), _ptr(self.m), _ptr(self.v), lr, beta1,
+                                               beta2, eps, self.step, _ptr(self.params), nat.current_stream_handle()))
+
+    def train(self, features, targets, epochs=4, batch=65536, lr=2e-3, seed=0, log=None):
+        """Adam on minibatches of (features, targets = nu_t / nu_scale); returns the per-epoch mean loss."""
+        n = features.shape[0]
+        g = torch.Generator(device=self.device).manual_seed(seed)
+        history = []
+        for ep in range(epochs):
+            perm = torch.randperm(n, generator=g, device=self.device)
+            tot = torch.zeros((), device=self.device, dtype=torch.float64)
+            nb = 0
+            for s in range(0, n, batch):
+                idx = perm[s:s + batch]
+                loss, grad = self.gradient(features[idx], targets[idx])
+                self.adam_step(grad, lr=lr)
+                tot += loss[0].double()
+                nb += 1
+            history.append(float(tot / nb))
+            if log:
+                log(f"closure training epoch {ep}: loss {history[-1]:.4e}")
+        return history
+
+
+# ---------------------------------------------------------------------- flow fields and diagnostics
+
+def solenoidal_field(n, k0=4.0, u_rms=0.04, seed=0, device="cuda"):
+    """Random divergence-free velocity (3, n, n, n) [component, z, y, x] with energy spectrum
+    E(k) ~ k^4 exp(-2 (k/k0)^2) (k in box wavenumbers), scaled to the given rms per component."""
+    gen = torch.Generator(device="cpu").manual_seed(seed)
+    k1 = torch.fft.fftfreq(n, d=1.0 / n, dtype=torch.float64)          # integer wavenumbers
+    kr = torch.fft.rfftfreq(n, d=1.0 / n, dtype=torch.float64)
+    kz, ky, kx = torch.meshgrid(k1, k1, kr, indexing="ij")
+    kk = torch.sqrt(kx ** 2 + ky ** 2 + kz ** 2)
+    amp = torch.where(kk > 0, torch.sqrt(kk ** 4 * torch.exp(-2 * (kk / k0) ** 2) / (4 * math.pi * kk ** 2 + 1e-300)),
+                      torch.zeros_like(kk))
+    re = torch.randn((3,) + kk.shape, generator=gen, dtype=torch.float64)
+    im = torch.randn((3,) + kk.shape, generator=gen, dtype=torch.float64)
+    c = torch.complex(re, im) * amp
+    kv = torch.stack([kx, ky, kz])
+    kdotc = (kv * c).sum(0)
+    c = c - kv * kdotc / torch.where(kk > 0, kk ** 2, torch.ones_like(kk))            # project out k || c
+    u = torch.stack([torch.fft.irfftn(c[a], s=(n, n, n)) for a in range(3)]).to(device)
+    u = u * (u_rms / torch.sqrt((u ** 2).mean(dim=(1, 2, 3)).mean()))
+    return u
+
+
+def box_filter(a, f):
+    """Average over f^3 boxes of the last three axes."""
+    s = a.shape
+    n = s[-1] // f
+    return a.reshape(*s[:-3], n, f, n, f, n, f).mean(dim=(-5, -3, -1))
+
+
+def cumulant_equilibrium_aa(storage, rho, u, chunk=16):
+    """Writes the zero-centred D3Q27 cumulant-method equilibrium of (rho, u) into an AA field's storage
+    (27, z, y, x) in the layout before the first (ODD) step: slot inv(i) holds f_i.
+    f_i = rho prod_a phi_{c_ia}(u_a), phi_0 = 2/3 - u^2, phi_(+-1) = (1/3 + u^2 +- u)/2."""
+    from .stencils import LBStencil
+    st = LBStencil("D3Q27")
+    inv = st.inverse_indices()
+    w1 = {0: 2.0 / 3.0, 1: 1.0 / 6.0, -1: 1.0 / 6.0}
+    nz = storage.shape[1]
+    for z0 in range(0, nz, chunk):
+        r = rho[z0:z0 + chunk].double()
+        uu = [u[a, z0:z0 + chunk].double() for a in range(3)]
+
+        def phi(c, ua):
+            return (2.0 / 3.0 - ua * ua) if c == 0 else 0.5 * (1.0 / 3.0 + ua * ua + c * ua)
+        for i, cv in enumerate(st):
+            f = r * phi(cv[0], uu[0]) * phi(cv[1], uu[1]) * phi(cv[2], uu[2]) - w1[cv[0]] * w1[cv[1]] * w1[cv[2]]
+            storage[inv[i], z0:z0 + chunk] = f.to(storage.dtype)
+
+
+def macroscopic_aa(storage, chunk=16):
+    """(rho, u) in FP64 of a zero-centred D3Q27 AA field after an even number of steps
+    (slot j holds f_inv(j), so the momentum sums carry -c_j)."""
+    from .stencils import LBStencil
+    c = torch.tensor([list(d) for d in LBStencil("D3Q27")], dtype=torch.float64, device=storage.device)
+    nz = storage.shape[1]
+    rho = torch.empty(storage.shape[1:], dtype=torch.float64, device=storage.device)
+    u = torch.empty((3,) + tuple(storage.shape[1:]), dtype=torch.float64, device=storage.device)
+    for z0 in range(0, nz, chunk):
+        g = storage[:, z0:z0 + chunk].double()
+        r = g.sum(0) + 1.0
+        rho[z0:z0 + chunk] = r
+        u[:, z0:z0 + chunk] = -torch.einsum("qa,q...->a...", c, g) / r
+    return rho, u
+
+
+def kinetic_energy(u):
+    return 0.5 * float((u ** 2).sum(0).mean())
+
+
+def energy_spectrum(u):
+    """Shell-summed spectrum E(k), k = 0..n/2, normalised so that sum_k E(k) = (1/2) <|u|^2>."""
+    n = u.shape[-1]
+    uh = torch.fft.fftn(u, dim=(1, 2, 3)) / n ** 3
+    e = 0.5 * (uh.real ** 2 + uh.imag ** 2).sum(0)
+    k1 = torch.fft.fftfreq(n, d=1.0 / n, device=u.device)
+    kz, ky, kx = torch.meshgrid(k1, k1, k1, indexing="ij")
+    shell = torch.round(torch.sqrt(kx ** 2 + ky ** 2 + kz ** 2)).long().clamp(max=n // 2)
+    spec = torch.zeros(n // 2 + 1, dtype=torch.float64, device=u.device)
+    spec.scatter_add_(0, shell.ravel(), e.ravel().double())
+    return spec.cpu().numpy()
+
+
+def _ddx(a, axis):
+    return 0.5 * (torch.roll(a, -1, dims=axis) - torch.roll(a, 1, dims=axis))
+
+
+def _box3(a):
+    """Periodic 3x3x3 average over the last three axes."""
+    out = a
+    for ax in (-1, -2, -3):
+        out = (torch.roll(out, 1, ax) + out + torch.roll(out, -1, ax)) / 3.0
+    return out
+
+
+def subgrid_samples(rho_f, u_f, factor, omega0, model):
+    """Training samples from a fine-grid state (rho_f, u_f on [z, y, x]) box-filtered by `factor` onto the
+    coarse grid.  The subgrid stress tau_ij = <rho u_i u_j>/<rho> - ~u_i ~u_j (Favre filter) and the filtered
+    strain rate S_ij give the least-squares eddy viscosity nu_t = max(0, -<tau^d : S> / (2 <S : S>)) (local 3^3
+    averages); the coarse cell's pre-collision non-equilibrium moments are Pi_neq = -2 rho c_s^2 tau_eff S with
+    tau_eff = 1/omega0 + 3 nu_t (Chapman-Enskog).  Returns (features (n, 10), targets nu_t / nu_scale (n,))."""
+    rb = box_filter(rho_f, factor)
+    ub = box_filter(rho_f[None] * u_f, factor) / rb[None]
+    axes = {0: -1, 1: -2, 2: -3}            # velocity component a -> tensor axis of coordinate a
+    S = {}
+    for a in range(3):
+        for b in range(a, 3):
+            S[(a, b)] = 0.5 * (_ddx(ub[a], axes[b]) + _ddx(ub[b], axes[a]))
+    tau = {}
+    for a in range(3):
+        for b in range(a, 3):
+            tau[(a, b)] = box_filter(rho_f * u_f[a] * u_f[b], factor) / rb - ub[a] * ub[b]
+    trt = (tau[(0, 0)] + tau[(1, 1)] + tau[(2, 2)]) / 3.0
+    num = torch.zeros_like(rb)
+    den = torch.zeros_like(rb)
+    for (a, b), s in S.items():
+        w = 1.0 if a == b else 2.0
+        td = tau[(a, b)] - (trt if a == b else 0.0)
+        num = num + w * td * s
+        den = den + w * s * s
+    nu_t = torch.clamp(-_box3(num) / (2.0 * _box3(den) + 1e-30), min=0.0)
+    tau_eff = 1.0 / omega0 + 3.0 * nu_t
+    order = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
+    pi = torch.stack([-(2.0 / 3.0) * rb * tau_eff * S[k] for k in order], dim=-1).reshape(-1, 6)
+    feats = model.features(pi.float(), rb.reshape(-1).float())
+    return feats, (nu_t.reshape(-1) / model.nu_scale).float()

@@ -1,0 +1,242 @@
+// Batched ensemble sweep: many independent domains (members) of one stencil / collision family /
+// streaming pattern / storage type, each with its own sizes, faces, force and method table,
+// advanced by ONE launch per time step.
+//
+// Work split: every member's interior cells, in memory order (x fastest), are cut into tiles of
+// 64 cells, one wavefront each; tiles are numbered consecutively over the members.  The grid is
+// persistent: wave w of G handles tiles w, w + G, w + 2G, ... until all are done, so any grid size
+// covers any ensemble and every wave reaches the end of its loop.  A wave finds its member by a
+// binary search over the members' first-tile prefix (wave-uniform, scalar loads), decodes (x, y, z)
+// and runs sweep_cell, the same per-cell function as the single-domain sweep (sweep_cell.h):
+// a member's cells get bit-identical results to a single-domain run of that member.
+//
+// Optional XCD-aware order: MI355X dispatches consecutive workgroups round-robin over its 8 XCDs
+// (each with its own L2).  With `xcd_order`, the workgroups of one XCD take one contiguous
+// stretch of the tile sequence, so the rows one tile's neighbour reads touch are fetched into the
+// same L2 that the neighbouring tiles are processed on.
+#include <string>
+#include <vector>
+
+#include "sweep_cell.h"
+
+namespace lbm355 {
+
+template <class TC>
+struct BatchEntry {
+  SweepArgs<TC> a;
+  int64_t first_tile;   // index of this member's first tile
+  int64_t ncells;       // interior cells
+};
+
+template <class S, int FAM, int PAT, class TS, class TC>
+__global__ void __launch_bounds__(256) batch_kernel(const BatchEntry<TC>* __restrict__ members, int nmembers,
+                                                    int64_t ntiles, int swap, int xcd_order) {
+  const int waves_per_block = blockDim.x >> 6;
+  int b = blockIdx.x;
+  if (xcd_order && (gridDim.x & 7) == 0) b = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  const int64_t wave = (int64_t)b * waves_per_block + (threadIdx.x >> 6);
+  const int64_t nwaves = (int64_t)gridDim.x * waves_per_block;
+  const int lane = threadIdx.x & 63;
+  for (int64_t tile = wave; tile < ntiles; tile += nwaves) {
+    // last member whose first tile is <= tile (tile is uniform over the wave)
+    int lo = 0, hi = nmembers - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (members[mid].first_tile <= tile) lo = mid;
+      else hi = mid - 1;
+    }
+    lo = __builtin_amdgcn_readfirstlane(lo);
+    const BatchEntry<TC>* e = members + lo;
+    const int64_t local = (tile - e->first_tile) * 64 + lane;
+    if (local < e->ncells) {
+      SweepArgs<TC> a = e->a;
+      if (swap) {
+        void* t = const_cast<void*>(a.src);
+        a.src = a.dst;
+        a.dst = t;
+      }
+      const int nx = a.nx, ny = a.ny;
+      const int64_t row = local / nx;
+      const int x = (int)(local - row * nx);
+      const int z = (int)(row / ny);
+      const int y = (int)(row - (int64_t)z * ny);
+      sweep_cell<S, FAM, PAT, true, TS, TC>(a, x, y, z);
+    }
+  }
+}
+
+template <class TC>
+using BatchFn = void (*)(const BatchEntry<TC>*, int, int64_t, int, int);
+
+template <class S, int FAM, class TS, class TC>
+void* batch_pick_pattern(int pat) {
+  switch (pat) {
+    case LBM355_PULL: return (void*)batch_kernel<S, FAM, LBM355_PULL, TS, TC>;
+    case LBM355_AA_EVEN: return (void*)batch_kernel<S, FAM, LBM355_AA_EVEN, TS, TC>;
+    case LBM355_AA_ODD: return (void*)batch_kernel<S, FAM, LBM355_AA_ODD, TS, TC>;
+  }
+  return nullptr;
+}
+
+template <class S, int FAM, class TC>
+void* batch_pick_store(int store, int pat) {
+  if constexpr (sizeof(TC) == 8) {
+    if (store == LBM355_F64) return batch_pick_pattern<S, FAM, double, TC>(pat);
+  } else {
+    if (store == LBM355_F32) return batch_pick_pattern<S, FAM, float, TC>(pat);
+    if (store == LBM355_F16) return batch_pick_pattern<S, FAM, _Float16, TC>(pat);
+  }
+  return nullptr;
+}
+
+template <class TC>
+void* batch_pick(const lbm355_sweep_desc* d, int pattern) {
+  if (d->family == LBM355_FAMILY_MOMENT) {
+    if (d->stencil == LBM355_D2Q9) return batch_pick_store<D2Q9, LBM355_FAMILY_MOMENT, TC>(d->store_dtype, pattern);
+    if (d->stencil == LBM355_D3Q19) return batch_pick_store<D3Q19, LBM355_FAMILY_MOMENT, TC>(d->store_dtype, pattern);
+    if (d->stencil == LBM355_D3Q27) return batch_pick_store<D3Q27, LBM355_FAMILY_MOMENT, TC>(d->store_dtype, pattern);
+  } else if (d->family == LBM355_FAMILY_CUMULANT && d->stencil == LBM355_D3Q27) {
+    return batch_pick_store<D3Q27, LBM355_FAMILY_CUMULANT, TC>(d->store_dtype, pattern);
+  }
+  return nullptr;
+}
+
+}  // namespace lbm355
+
+using namespace lbm355;
+
+struct lbm355_batch {
+  int32_t stencil, family, store_dtype, compute_dtype, pattern_kind;  // pattern_kind: PULL or AA
+  int32_t nmembers;
+  int64_t ntiles;
+  int64_t ncells;
+  void* entries;          // device array of BatchEntry<TC>
+  int device;
+};
+
+namespace {
+int berr(int code, const std::string& msg) { return lbm355_set_error(code, msg.c_str()); }
+
+template <class TC>
+int upload(const lbm355_sweep_desc* d, const void* const* src, void* const* dst, int32_t n, lbm355_batch* b) {
+  std::vector<BatchEntry<TC>> host(n);
+  int64_t tiles = 0, cells = 0;
+  for (int k = 0; k < n; ++k) {
+    host[k].a = make_args<TC>(&d[k], src[k], dst[k], nullptr, nullptr);
+    host[k].first_tile = tiles;
+    host[k].ncells = (int64_t)d[k].nx * d[k].ny * d[k].nz;
+    tiles += (host[k].ncells + 63) / 64;
+    cells += host[k].ncells;
+  }
+  void* dev = nullptr;
+  hipError_t e = hipMalloc(&dev, sizeof(BatchEntry<TC>) * (size_t)n);
+  if (e != hipSuccess) return berr(LBM355_ERR_LAUNCH, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  e = hipMemcpy(dev, host.data(), sizeof(BatchEntry<TC>) * (size_t)n, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(dev);
+    return berr(LBM355_ERR_LAUNCH, std::string("hipMemcpy failed: ") + hipGetErrorString(e));
+  }
+  b->entries = dev;
+  b->ntiles = tiles;
+  b->ncells = cells;
+  return LBM355_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int lbm355_batch_create(const lbm355_sweep_desc* descs, const void* const* pdf_src, void* const* pdf_dst,
+                        int32_t n_members, lbm355_batch** out) {
+  if (!descs || !pdf_src || !pdf_dst || !out || n_members <= 0) return berr(LBM355_ERR_ARG, "bad batch arguments");
+  *out = nullptr;
+  const lbm355_sweep_desc& d0 = descs[0];
+  const int kind0 = d0.pattern == LBM355_PULL ? LBM355_PULL : LBM355_AA_EVEN;
+  for (int k = 0; k < n_members; ++k) {
+    const lbm355_sweep_desc& d = descs[k];
+    int rc = lbm355_validate_desc(&d);
+    if (rc) return rc;
+    if (d.stencil != d0.stencil || d.family != d0.family || d.store_dtype != d0.store_dtype ||
+        d.compute_dtype != d0.compute_dtype || (d.pattern == LBM355_PULL ? LBM355_PULL : LBM355_AA_EVEN) != kind0)
+      return berr(LBM355_ERR_ARG, "batch members must share stencil, family, streaming pattern and dtypes");
+    if (d.flags & (LBM355_FLAG_SMAGORINSKY | LBM355_FLAG_LEARNED_CLOSURE))
+      return berr(LBM355_ERR_UNSUPPORTED, "subgrid models are not batched");
+    if (!pdf_src[k] || !pdf_dst[k]) return berr(LBM355_ERR_ARG, "NULL member PDF pointer");
+    if (kind0 == LBM355_PULL && pdf_src[k] == pdf_dst[k]) return berr(LBM355_ERR_ARG, "pull needs two arrays");
+    if (kind0 != LBM355_PULL && pdf_src[k] != pdf_dst[k]) return berr(LBM355_ERR_ARG, "AA needs src == dst");
+  }
+  void* probe = d0.compute_dtype == LBM355_F64 ? batch_pick<double>(&d0, kind0) : batch_pick<float>(&d0, kind0);
+  if (!probe) return berr(LBM355_ERR_UNSUPPORTED, "no batched kernel for this stencil/family/dtype combination");
+  lbm355_batch* b = new lbm355_batch();
+  b->stencil = d0.stencil;
+  b->family = d0.family;
+  b->store_dtype = d0.store_dtype;
+  b->compute_dtype = d0.compute_dtype;
+  b->pattern_kind = kind0;
+  b->nmembers = n_members;
+  (void)hipGetDevice(&b->device);
+  int rc = d0.compute_dtype == LBM355_F64 ? upload<double>(descs, pdf_src, pdf_dst, n_members, b)
+                                          : upload<float>(descs, pdf_src, pdf_dst, n_members, b);
+  if (rc) {
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return LBM355_OK;
+}
+
+int lbm355_batch_sweep(const lbm355_batch* b, int32_t step_kind, int32_t threads, int32_t blocks,
+                       int32_t xcd_order, void* stream) {
+  if (!b) return berr(LBM355_ERR_ARG, "batch is NULL");
+  if (threads != 64 && threads != 128 && threads != 192 && threads != 256)
+    return berr(LBM355_ERR_ARG, "threads per block must be 64, 128, 192 or 256");
+  if (blocks <= 0) return berr(LBM355_ERR_ARG, "blocks must be positive");
+  int pattern, swap = 0;
+  if (b->pattern_kind == LBM355_PULL) {
+    if (step_kind != 0 && step_kind != 1) return berr(LBM355_ERR_ARG, "pull step_kind is 0 (src->dst) or 1 (dst->src)");
+    pattern = LBM355_PULL;
+    swap = step_kind;
+  } else {
+    if (step_kind != LBM355_AA_EVEN && step_kind != LBM355_AA_ODD)
+      return berr(LBM355_ERR_ARG, "AA step_kind is LBM355_AA_EVEN or LBM355_AA_ODD");
+    pattern = step_kind;
+  }
+  lbm355_sweep_desc d{};
+  d.stencil = b->stencil;
+  d.family = b->family;
+  d.store_dtype = b->store_dtype;
+  void* k = b->compute_dtype == LBM355_F64 ? batch_pick<double>(&d, pattern) : batch_pick<float>(&d, pattern);
+  if (!k) return berr(LBM355_ERR_UNSUPPORTED, "no batched kernel");
+  const int64_t max_blocks = (b->ntiles * 64 + threads - 1) / threads;
+  const int grid = (int)(blocks < max_blocks ? blocks : max_blocks);
+  hipStream_t s = (hipStream_t)stream;
+  if (b->compute_dtype == LBM355_F64)
+    hipLaunchKernelGGL((BatchFn<double>)k, dim3(grid), dim3(threads), 0, s, (const BatchEntry<double>*)b->entries,
+                       b->nmembers, b->ntiles, swap, xcd_order);
+  else
+    hipLaunchKernelGGL((BatchFn<float>)k, dim3(grid), dim3(threads), 0, s, (const BatchEntry<float>*)b->entries,
+                       b->nmembers, b->ntiles, swap, xcd_order);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return berr(LBM355_ERR_LAUNCH, std::string("launch failed: ") + hipGetErrorString(e));
+  return LBM355_OK;
+}
+
+int lbm355_batch_info(const lbm355_batch* b, int64_t* n_tiles, int64_t* n_cells) {
+  if (!b) return berr(LBM355_ERR_ARG, "batch is NULL");
+  if (n_tiles) *n_tiles = b->ntiles;
+  if (n_cells) *n_cells = b->ncells;
+  return LBM355_OK;
+}
+
+int lbm355_batch_destroy(lbm355_batch* b) {
+  if (!b) return LBM355_OK;
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  (void)hipSetDevice(b->device);
+  hipError_t e = hipFree(b->entries);
+  (void)hipSetDevice(cur);
+  delete b;
+  if (e != hipSuccess) return berr(LBM355_ERR_LAUNCH, std::string("hipFree failed: ") + hipGetErrorString(e));
+  return LBM355_OK;
+}
+
+}  // extern "C"

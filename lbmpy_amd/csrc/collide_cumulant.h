@@ -18,6 +18,7 @@
 // = 0), which is used to drop them from the cumulant relations.
 #pragma once
 #include "lbm_common.h"
+#include "closure.h"
 
 namespace lbm355 {
 
@@ -85,7 +86,9 @@ struct CumulantTab {
   static constexpr int RATES = 0, SIZE = 27;
 };
 
-template <class T>
+// LEARNED: the shear rate comes from the learned closure (a separate kernel variant, so that the plain
+// sweeps carry none of its MFMA register state)
+template <class T, bool LEARNED = false>
 __device__ __forceinline__ void collide_cumulant_d3q27(T (&f)[27], const SweepArgs<T>& a, T& rho_out_v,
                                                        T (&u_out_v)[3]) {
   using S = D3Q27;
@@ -158,7 +161,7 @@ __device__ __forceinline__ void collide_cumulant_d3q27(T (&f)[27], const SweepAr
 
   // shear relaxation rate, possibly replaced by the Smagorinsky rate
   T rs[5] = {r[4], r[5], r[6], r[7], r[8]};
-  if (a.flags & LBM355_FLAG_SMAGORINSKY) {
+  if (LEARNED || (a.flags & LBM355_FLAG_SMAGORINSKY)) {
     // Pi_neq_ab = rho mu'_ab - (u_a F_b + u_b F_a) / 2  (second moment of f - f_eq)
     const T pxx = rho * mu200 - u[0] * a.force[0];
     const T pyy = rho * mu020 - u[1] * a.force[1];
@@ -166,9 +169,20 @@ __device__ __forceinline__ void collide_cumulant_d3q27(T (&f)[27], const SweepAr
     const T pxy = rho * mu110 - T(0.5) * (u[0] * a.force[1] + u[1] * a.force[0]);
     const T pxz = rho * mu101 - T(0.5) * (u[0] * a.force[2] + u[2] * a.force[0]);
     const T pyz = rho * mu011 - T(0.5) * (u[1] * a.force[2] + u[2] * a.force[1]);
-    const T pi = sqrt(T(2) * (pxx * pxx + pyy * pyy + pzz * pzz + T(2) * (pxy * pxy + pxz * pxz + pyz * pyz))) * irho;
-    const T tau0 = T(1) / a.omega_shear;
-    const T w = T(2) / (tau0 + sqrt(a.smag_cs2x18 * pi + tau0 * tau0));
+    T w = a.omega_shear;
+    if (a.flags & LBM355_FLAG_SMAGORINSKY) {
+      const T pi = sqrt(T(2) * (pxx * pxx + pyy * pyy + pzz * pzz + T(2) * (pxy * pxy + pxz * pxz + pyz * pyz))) * irho;
+      const T tau0 = T(1) / a.omega_shear;
+      w = T(2) / (tau0 + sqrt(a.smag_cs2x18 * pi + tau0 * tau0));
+    }
+    if constexpr (LEARNED && sizeof(T) == 4) {
+      // wave-collective: every lane of the wavefront reaches this point (the host requires nx % 64 == 0)
+      const ClosureParams* P = reinterpret_cast<const ClosureParams*>(a.closure);
+      const float p6[6] = {pxx, pyy, pzz, pxy, pxz, pyz};
+      float feat[CLOSURE_IN];
+      closure_features(p6, rho, feat);
+      w = closure_omega(P, mlp_forward_wave(P, feat, nullptr, nullptr), a.omega_shear);
+    }
     rs[0] = rs[1] = rs[2] = rs[3] = rs[4] = w;
   }
 

@@ -84,6 +84,7 @@ struct SweepArgs {
   TC force[3];
   TC smag_cs2x18;             // 18 * Cs^2 (Smagorinsky)
   TC omega_shear;             // shear rate replaced by the subgrid model
+  const void* closure;        // learned-closure parameter block (LBM355_FLAG_LEARNED_CLOSURE)
 };
 
 // ---------------------------------------------------------------- addressing

@@ -14,127 +14,68 @@
 #include <cstring>
 #include <cstdio>
 
-#include "lbm_common.h"
-#include "collide_moment.h"
-#include "collide_cumulant.h"
-#include "collide_central.h"
+#include "sweep_cell.h"
 
 namespace lbm355 {
 
-template <class S, int FAM, class TC>
-struct Collider;
-
-template <class S, class TC>
-struct Collider<S, LBM355_FAMILY_MOMENT, TC> {
-  static constexpr bool deviation_form = false;
-  static __device__ __forceinline__ void run(TC (&f)[S::Q], const SweepArgs<TC>& a, TC& rho, TC (&u)[3]) {
-    collide_moment<S, TC>(f, a, rho, u);
-  }
-};
-
-template <class TC>
-struct Collider<D3Q27, LBM355_FAMILY_CUMULANT, TC> {
-  static constexpr bool deviation_form = true;
-  static __device__ __forceinline__ void run(TC (&f)[27], const SweepArgs<TC>& a, TC& rho, TC (&u)[3]) {
-    collide_cumulant_d3q27<TC>(f, a, rho, u);
-  }
-};
-
-// number of directions with a non-zero x component (those are shifted through LDS)
-template <class S>
-__host__ __device__ constexpr int n_xshift() {
-  int n = 0;
-  for (int i = 0; i < S::Q; ++i) n += S::c[i][0] != 0;
-  return n;
-}
-
-// Non-temporal accesses for aligned, streamed-once data (each PDF is read and written exactly
-// once per sweep); misaligned (x +- 1) accesses keep the default policy so that the partially
-// used cache lines are served from cache to the neighbouring wavefront.
-template <class TS, class TC>
-__device__ __forceinline__ TC load_nt(const TS* p) {
-  return (TC)__builtin_nontemporal_load(p);
-}
-template <class TS, class TC>
-__device__ __forceinline__ void store_nt(TS* p, TC v) {
-  __builtin_nontemporal_store((TS)v, p);
-}
-
-// ROWLDS: the block spans one whole row and x is periodic (blockDim.x == nx).  The +-1
-// x-shifts of pull / AA-odd streaming are then done in LDS, so every global load and store
-// is an aligned, contiguous, non-temporal 64-lane access:
-//   read : tile[j][x] = row(y - c_y, z - c_z)[x];  f_i(x) = tile[j][(x - c_x) mod nx]
-//   write: tile[j][x] = f_i(x);  row(y + c_y, z + c_z)[x] = tile[j][(x - c_x) mod nx]
-template <class S, class TC>
-struct Collider<S, LBM355_FAMILY_CENTRAL, TC> {
-  static constexpr bool deviation_form = true;
-  static __device__ __forceinline__ void run(TC (&f)[S::Q], const SweepArgs<TC>& a, TC& rho, TC (&u)[3]) {
-    collide_central<S, TC>(f, a, rho, u);
-  }
-};
-
 template <class S, int FAM, int PAT, bool WALLS, bool ROWLDS, class TS, class TC>
 __global__ void __launch_bounds__(ROWLDS ? 512 : 256) sweep_kernel(SweepArgs<TC> a) {
-  constexpr int Q = S::Q, D = S::D;
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (!ROWLDS && x >= a.nx) return;
-  const int y = blockIdx.y;
-  const int z = blockIdx.z;
-
-  const AxisNb ax = axis_neighbours(x, a.nx, a.g, a.face[0]);
-  const AxisNb ay = axis_neighbours(y, a.ny, a.g, a.face[1]);
-  AxisNb az = {0, 0, false, false};
-  if (D == 3) az = axis_neighbours(z, a.nz, a.g, a.face[2]);
-  // in-plane element offsets of the 3 x positions, 3 rows and 3 planes around the cell
-  const int sy = (int)a.sy, sz = (int)a.sz;
-  const int X[3] = {ax.lo, x + a.g, ax.hi};
-  const int Y[3] = {ay.lo * sy, (y + a.g) * sy, ay.hi * sy};
-  const int Z[3] = {D == 3 ? az.lo * sz : 0, D == 3 ? (z + a.g) * sz : 0, D == 3 ? az.hi * sz : 0};
-  const int center = X[1] + Y[1] + Z[1];
-  // LDS row positions of x - 1 and x + 1 (periodic)
-  const int XL[3] = {ax.lo - a.g, x, ax.hi - a.g};
-
-  const TS* __restrict__ src = reinterpret_cast<const TS*>(a.src);
-  TS* __restrict__ dst = reinterpret_cast<TS*>(a.dst);
-  const int64_t sq = a.sq;
-  extern __shared__ unsigned char lds_raw[];
-  TS* tile = reinterpret_cast<TS*>(lds_raw);
-  const int nx = a.nx;
-
-  TC f[Q];
-  if (PAT == LBM355_AA_EVEN) {
-#pragma unroll
-    for (int i = 0; i < Q; ++i) f[i] = load_nt<TS, TC>(src + i * sq + center);
+  if constexpr (!ROWLDS) {
+    if (x >= a.nx) return;
+    sweep_cell<S, FAM, PAT, WALLS, TS, TC>(a, x, blockIdx.y, blockIdx.z);
   } else {
-    // pull reads src[i] at x - c_i; AA odd reads slot inv(i) there.  At a wall the
-    // bounce-back value sits in the cell itself: pull slot inv(i), AA odd slot i.
-    int j = 0;
+    // The block spans one whole periodic row (blockDim.x == nx).  The +-1 x-shifts of pull / AA-odd
+    // streaming go through LDS, so every global access is an aligned, contiguous, non-temporal one.
+    constexpr int Q = S::Q, D = S::D;
+    const int y = blockIdx.y;
+    const int z = blockIdx.z;
+    const AxisNb ax = axis_neighbours(x, a.nx, a.g, a.face[0]);
+    const AxisNb ay = axis_neighbours(y, a.ny, a.g, a.face[1]);
+    AxisNb az = {0, 0, false, false};
+    if (D == 3) az = axis_neighbours(z, a.nz, a.g, a.face[2]);
+    const int sy = (int)a.sy, sz = (int)a.sz;
+    const int X1 = x + a.g;
+    const int Y[3] = {ay.lo * sy, (y + a.g) * sy, ay.hi * sy};
+    const int Z[3] = {D == 3 ? az.lo * sz : 0, D == 3 ? (z + a.g) * sz : 0, D == 3 ? az.hi * sz : 0};
+    const int center = X1 + Y[1] + Z[1];
+    // LDS row positions of x - 1 and x + 1 (periodic)
+    const int XL[3] = {ax.lo - a.g, x, ax.hi - a.g};
+    const TS* __restrict__ src = reinterpret_cast<const TS*>(a.src);
+    TS* __restrict__ dst = reinterpret_cast<TS*>(a.dst);
+    const int64_t sq = a.sq;
+    extern __shared__ unsigned char lds_raw[];
+    TS* tile = reinterpret_cast<TS*>(lds_raw);
+    const int nx = a.nx;
+
+    TC f[Q];
+    if (PAT == LBM355_AA_EVEN) {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) {
-      const int cx = S::c[i][0], cy = S::c[i][1], cz = S::c[i][2];
-      const int ii = S::inv[i];
-      const int plane = (PAT == LBM355_PULL) ? i : ii;
-      const int wplane = (PAT == LBM355_PULL) ? ii : i;
-      bool wall = false;
-      if (WALLS)
-        wall = (cx == 1 && ax.wall_lo) || (cx == -1 && ax.wall_hi) || (cy == 1 && ay.wall_lo) ||
-               (cy == -1 && ay.wall_hi) || (cz == 1 && az.wall_lo) || (cz == -1 && az.wall_hi);
-      if (ROWLDS && cx != 0) {
-        // x is periodic here, so `wall` is uniform over the row (block)
-        if (wall)
-          f[i] = load_nt<TS, TC>(src + wplane * sq + center);
-        else
-          tile[j * nx + x] = __builtin_nontemporal_load(src + plane * sq + X[1] + Y[1 - cy] + Z[1 - cz]);
-        ++j;
-      } else if (ROWLDS) {
-        const TS* p = wall ? (src + wplane * sq + center) : (src + plane * sq + X[1] + Y[1 - cy] + Z[1 - cz]);
-        f[i] = load_nt<TS, TC>(p);
-      } else {
-        const TS* p = wall ? (src + wplane * sq + center) : (src + plane * sq + X[1 - cx] + Y[1 - cy] + Z[1 - cz]);
-        f[i] = load_as<TS, TC>(p);
+      for (int i = 0; i < Q; ++i) f[i] = load_nt<TS, TC>(src + i * sq + center);
+    } else {
+      int j = 0;
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        const int cx = S::c[i][0], cy = S::c[i][1], cz = S::c[i][2];
+        const int ii = S::inv[i];
+        const int plane = (PAT == LBM355_PULL) ? i : ii;
+        const int wplane = (PAT == LBM355_PULL) ? ii : i;
+        bool wall = false;
+        if (WALLS)
+          wall = (cy == 1 && ay.wall_lo) || (cy == -1 && ay.wall_hi) || (cz == 1 && az.wall_lo) ||
+                 (cz == -1 && az.wall_hi);
+        if (cx != 0) {
+          // x is periodic here, so `wall` is uniform over the row (block)
+          if (wall)
+            f[i] = load_nt<TS, TC>(src + wplane * sq + center);
+          else
+            tile[j * nx + x] = __builtin_nontemporal_load(src + plane * sq + X1 + Y[1 - cy] + Z[1 - cz]);
+          ++j;
+        } else {
+          const TS* p = wall ? (src + wplane * sq + center) : (src + plane * sq + X1 + Y[1 - cy] + Z[1 - cz]);
+          f[i] = load_nt<TS, TC>(p);
+        }
       }
-    }
-    if (ROWLDS) {
       __syncthreads();
       j = 0;
 #pragma unroll
@@ -150,63 +91,37 @@ __global__ void __launch_bounds__(ROWLDS ? 512 : 256) sweep_kernel(SweepArgs<TC>
       }
       __syncthreads();
     }
-  }
 
-  const bool absolute = !(a.flags & LBM355_FLAG_ZERO_CENTERED);
-  if (Collider<S, FAM, TC>::deviation_form && absolute) {
-#pragma unroll
-    for (int i = 0; i < Q; ++i) f[i] -= TC(weight<S>(i));
-  }
+    collide_cell<S, FAM, TC>(f, a, x, y, z);
 
-  TC rho, u[3];
-  Collider<S, FAM, TC>::run(f, a, rho, u);
-
-  if (Collider<S, FAM, TC>::deviation_form && absolute) {
+    if (PAT == LBM355_PULL) {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) f[i] += TC(weight<S>(i));
-  }
-
-  if (a.rho_out || a.u_out) {
-    const int64_t o = (int64_t)(x + a.g) + (int64_t)(y + a.g) * a.osy + (D == 3 ? (int64_t)(z + a.g) * a.osz : 0);
-    if (a.rho_out) a.rho_out[o] = rho;
-    if (a.u_out) {
+      for (int i = 0; i < Q; ++i) store_nt<TS, TC>(dst + i * sq + center, f[i]);
+    } else if (PAT == LBM355_AA_EVEN) {
 #pragma unroll
-      for (int d = 0; d < D; ++d) a.u_out[o + d * a.osc] = u[d];
-    }
-  }
-
-  if (PAT == LBM355_PULL) {
+      for (int i = 0; i < Q; ++i) store_nt<TS, TC>(dst + S::inv[i] * sq + center, f[i]);
+    } else {
+      // AA odd writes f_i to x + c_i slot i; at a wall it is bounced into the cell's slot inv(i)
+      int j = 0;
 #pragma unroll
-    for (int i = 0; i < Q; ++i) store_nt<TS, TC>(dst + i * sq + center, f[i]);
-  } else if (PAT == LBM355_AA_EVEN) {
-#pragma unroll
-    for (int i = 0; i < Q; ++i) store_nt<TS, TC>(dst + S::inv[i] * sq + center, f[i]);
-  } else {
-    // AA odd writes f_i to x + c_i slot i; at a wall it is bounced into the cell's slot inv(i)
-    int j = 0;
-#pragma unroll
-    for (int i = 0; i < Q; ++i) {
-      const int cx = S::c[i][0], cy = S::c[i][1], cz = S::c[i][2];
-      const int ii = S::inv[i];
-      bool wall = false;
-      if (WALLS)
-        wall = (cx == -1 && ax.wall_lo) || (cx == 1 && ax.wall_hi) || (cy == -1 && ay.wall_lo) ||
-               (cy == 1 && ay.wall_hi) || (cz == -1 && az.wall_lo) || (cz == 1 && az.wall_hi);
-      if (ROWLDS && cx != 0) {
-        if (wall)
-          store_as<TS, TC>(dst + ii * sq + center, f[i]);
-        else
-          tile[j * nx + x] = (TS)f[i];
-        ++j;
-      } else if (ROWLDS) {
-        TS* p = wall ? (dst + ii * sq + center) : (dst + i * sq + X[1] + Y[1 + cy] + Z[1 + cz]);
-        store_nt<TS, TC>(p, f[i]);
-      } else {
-        TS* p = wall ? (dst + ii * sq + center) : (dst + i * sq + X[1 + cx] + Y[1 + cy] + Z[1 + cz]);
-        store_as<TS, TC>(p, f[i]);
+      for (int i = 0; i < Q; ++i) {
+        const int cx = S::c[i][0], cy = S::c[i][1], cz = S::c[i][2];
+        const int ii = S::inv[i];
+        bool wall = false;
+        if (WALLS)
+          wall = (cy == -1 && ay.wall_lo) || (cy == 1 && ay.wall_hi) || (cz == -1 && az.wall_lo) ||
+                 (cz == 1 && az.wall_hi);
+        if (cx != 0) {
+          if (wall)
+            store_as<TS, TC>(dst + ii * sq + center, f[i]);
+          else
+            tile[j * nx + x] = (TS)f[i];
+          ++j;
+        } else {
+          TS* p = wall ? (dst + ii * sq + center) : (dst + i * sq + X1 + Y[1 + cy] + Z[1 + cz]);
+          store_nt<TS, TC>(p, f[i]);
+        }
       }
-    }
-    if (ROWLDS) {
       __syncthreads();
       j = 0;
 #pragma unroll
@@ -217,7 +132,7 @@ __global__ void __launch_bounds__(ROWLDS ? 512 : 256) sweep_kernel(SweepArgs<TC>
         if (WALLS)
           wall = (cy == -1 && ay.wall_lo) || (cy == 1 && ay.wall_hi) || (cz == -1 && az.wall_lo) ||
                  (cz == 1 && az.wall_hi);
-        if (!wall) __builtin_nontemporal_store(tile[j * nx + XL[1 - cx]], dst + i * sq + X[1] + Y[1 + cy] + Z[1 + cz]);
+        if (!wall) __builtin_nontemporal_store(tile[j * nx + XL[1 - cx]], dst + i * sq + X1 + Y[1 + cy] + Z[1 + cz]);
         ++j;
       }
     }
@@ -387,8 +302,15 @@ void* pick_kernel(const lbm355_sweep_desc* d, bool walls, bool rowlds) {
     if (d->stencil == LBM355_D3Q27)
       return pick_store<D3Q27, LBM355_FAMILY_MOMENT, TC>(d->store_dtype, d->pattern, walls, rowlds);
   } else if (d->family == LBM355_FAMILY_CUMULANT) {
-    if (d->stencil == LBM355_D3Q27)
-      return pick_store<D3Q27, LBM355_FAMILY_CUMULANT, TC>(d->store_dtype, d->pattern, walls, rowlds);
+    if (d->stencil != LBM355_D3Q27) return nullptr;
+    if (d->flags & LBM355_FLAG_LEARNED_CLOSURE) {
+      if constexpr (sizeof(TC) == 4) {
+        if (d->store_dtype == LBM355_F32)
+          return pick_pattern<D3Q27, FAMILY_CUMULANT_LEARNED, float, TC>(d->pattern, walls, rowlds);
+      }
+      return nullptr;
+    }
+    return pick_store<D3Q27, LBM355_FAMILY_CUMULANT, TC>(d->store_dtype, d->pattern, walls, rowlds);
   } else if (d->family == LBM355_FAMILY_CENTRAL) {
     if (d->stencil == LBM355_D2Q9)
       return pick_store<D2Q9, LBM355_FAMILY_CENTRAL, TC>(d->store_dtype, d->pattern, walls, rowlds);
@@ -442,30 +364,7 @@ int launch_sweep(const lbm355_sweep_desc* d, const void* src, void* dst, void* r
   const bool rowlds = use_rowlds(d, &lds);
   void* k = pick_kernel<TC>(d, walls, rowlds);
   if (!k) return fail(LBM355_ERR_UNSUPPORTED, "no kernel for this stencil/family/dtype combination");
-  SweepArgs<TC> a;
-  a.src = src;
-  a.dst = dst;
-  a.rho_out = (TC*)rho;
-  a.u_out = (TC*)u;
-  a.tab = (const TC*)d->tables;
-  a.nx = d->nx;
-  a.ny = d->ny;
-  a.nz = d->nz;
-  a.g = d->ghost_layers;
-  a.sy = d->stride_y;
-  a.sz = d->stride_z;
-  a.sq = d->stride_q;
-  a.osy = d->out_stride_y;
-  a.osz = d->out_stride_z;
-  a.osc = d->out_stride_c;
-  for (int i = 0; i < 3; ++i) {
-    a.face[i] = d->face[i];
-    a.force[i] = (TC)d->force[i];
-  }
-  a.flags = d->flags;
-  a.nactive = d->n_active;
-  a.smag_cs2x18 = (TC)(18.0 * d->smagorinsky_constant * d->smagorinsky_constant);
-  a.omega_shear = (TC)d->omega_shear;
+  SweepArgs<TC> a = make_args<TC>(d, src, dst, rho, u);
   Launch l = geometry(d->nx, d->ny, d->nz, rowlds);
   hipLaunchKernelGGL((KernelFn<TC>)k, l.grid, l.block, rowlds ? lds : 0, s, a);
   hipError_t e = hipGetLastError();
@@ -521,6 +420,8 @@ int reduce_dtype(const lbm355_sweep_desc* d, const void* pdf, double* out, hipSt
 }
 }  // namespace
 
+int lbm355_validate_desc(const lbm355_sweep_desc* d) { return validate(d); }
+
 extern "C" {
 
 int lbm355_sweep(const lbm355_sweep_desc* d, const void* pdf_src, void* pdf_dst, void* density_out,
@@ -534,6 +435,12 @@ int lbm355_sweep(const lbm355_sweep_desc* d, const void* pdf_src, void* pdf_dst,
     return fail(LBM355_ERR_UNSUPPORTED, "cumulant sweep is implemented for D3Q27");
   if ((d->flags & LBM355_FLAG_SMAGORINSKY) && d->family != LBM355_FAMILY_CUMULANT)
     return fail(LBM355_ERR_UNSUPPORTED, "Smagorinsky is implemented for the cumulant sweep");
+  if (d->flags & LBM355_FLAG_LEARNED_CLOSURE) {
+    if (d->family != LBM355_FAMILY_CUMULANT || d->compute_dtype != LBM355_F32 || d->store_dtype != LBM355_F32)
+      return fail(LBM355_ERR_UNSUPPORTED, "the learned closure runs in the FP32 cumulant sweep (FP32 storage)");
+    if (!d->closure) return fail(LBM355_ERR_ARG, "closure parameter pointer is NULL");
+    if (d->nx % 64) return fail(LBM355_ERR_UNSUPPORTED, "the learned closure needs nx % 64 == 0 (whole wavefronts)");
+  }
   hipStream_t s = (hipStream_t)stream;
   if (d->compute_dtype == LBM355_F64) return launch_sweep<double>(d, pdf_src, pdf_dst, density_out, velocity_out, s);
   return launch_sweep<float>(d, pdf_src, pdf_dst, density_out, velocity_out, s);
@@ -576,6 +483,6 @@ int lbm355_reduce_conserved(const lbm355_sweep_desc* d, const void* pdf, double*
 
 const char* lbm355_last_error(void) { return g_err.c_str(); }
 
-int32_t lbm355_version(void) { return 1; }
+int32_t lbm355_version(void) { return 2; }
 
 }  // extern "C"

@@ -1,0 +1,200 @@
+// Per-cell pieces of the stream-collide sweep shared by every kernel that updates cells: the
+// single-domain sweeps (sweep.hip) and the batched ensemble sweep (batch.hip).  Both call
+// collide_cell / sweep_cell, so a cell updated in a batch runs exactly the instructions of the
+// same cell updated on its own.
+#pragma once
+#include "lbm_common.h"
+#include "collide_moment.h"
+#include "collide_cumulant.h"
+#include "collide_central.h"
+
+namespace lbm355 {
+
+template <class S, int FAM, class TC>
+struct Collider;
+
+template <class S, class TC>
+struct Collider<S, LBM355_FAMILY_MOMENT, TC> {
+  static constexpr bool deviation_form = false;
+  static __device__ __forceinline__ void run(TC (&f)[S::Q], const SweepArgs<TC>& a, TC& rho, TC (&u)[3]) {
+    collide_moment<S, TC>(f, a, rho, u);
+  }
+};
+
+// kernel-variant family code of the cumulant sweep with the learned closure (internal, not in the ABI)
+constexpr int FAMILY_CUMULANT_LEARNED = 0x102;
+
+template <class TC>
+struct Collider<D3Q27, FAMILY_CUMULANT_LEARNED, TC> {
+  static constexpr bool deviation_form = true;
+  static __device__ __forceinline__ void run(TC (&f)[27], const SweepArgs<TC>& a, TC& rho, TC (&u)[3]) {
+    collide_cumulant_d3q27<TC, true>(f, a, rho, u);
+  }
+};
+
+template <class TC>
+struct Collider<D3Q27, LBM355_FAMILY_CUMULANT, TC> {
+  static constexpr bool deviation_form = true;
+  static __device__ __forceinline__ void run(TC (&f)[27], const SweepArgs<TC>& a, TC& rho, TC (&u)[3]) {
+    collide_cumulant_d3q27<TC>(f, a, rho, u);
+  }
+};
+
+// number of directions with a non-zero x component (those are shifted through LDS)
+template <class S>
+__host__ __device__ constexpr int n_xshift() {
+  int n = 0;
+  for (int i = 0; i < S::Q; ++i) n += S::c[i][0] != 0;
+  return n;
+}
+
+// Non-temporal accesses for aligned, streamed-once data (each PDF is read and written exactly
+// once per sweep); misaligned (x +- 1) accesses keep the default policy so that the partially
+// used cache lines are served from cache to the neighbouring wavefront.
+template <class TS, class TC>
+__device__ __forceinline__ TC load_nt(const TS* p) {
+  return (TC)__builtin_nontemporal_load(p);
+}
+template <class TS, class TC>
+__device__ __forceinline__ void store_nt(TS* p, TC v) {
+  __builtin_nontemporal_store((TS)v, p);
+}
+
+// ROWLDS: the block spans one whole row and x is periodic (blockDim.x == nx).  The +-1
+// x-shifts of pull / AA-odd streaming are then done in LDS, so every global load and store
+// is an aligned, contiguous, non-temporal 64-lane access:
+//   read : tile[j][x] = row(y - c_y, z - c_z)[x];  f_i(x) = tile[j][(x - c_x) mod nx]
+//   write: tile[j][x] = f_i(x);  row(y + c_y, z + c_z)[x] = tile[j][(x - c_x) mod nx]
+template <class S, class TC>
+struct Collider<S, LBM355_FAMILY_CENTRAL, TC> {
+  static constexpr bool deviation_form = true;
+  static __device__ __forceinline__ void run(TC (&f)[S::Q], const SweepArgs<TC>& a, TC& rho, TC (&u)[3]) {
+    collide_central<S, TC>(f, a, rho, u);
+  }
+};
+
+// Collision of one cell's populations plus the optional density / velocity outputs.  Shared by every
+// sweep kernel (single domain, row-LDS and batched), so all of them run the same arithmetic.
+template <class S, int FAM, class TC>
+__device__ __forceinline__ void collide_cell(TC (&f)[S::Q], const SweepArgs<TC>& a, int x, int y, int z) {
+  constexpr int Q = S::Q, D = S::D;
+  const bool absolute = !(a.flags & LBM355_FLAG_ZERO_CENTERED);
+  if (Collider<S, FAM, TC>::deviation_form && absolute) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] -= TC(weight<S>(i));
+  }
+  TC rho, u[3];
+  Collider<S, FAM, TC>::run(f, a, rho, u);
+  if (Collider<S, FAM, TC>::deviation_form && absolute) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] += TC(weight<S>(i));
+  }
+  if (a.rho_out || a.u_out) {
+    const int64_t o = (int64_t)(x + a.g) + (int64_t)(y + a.g) * a.osy + (D == 3 ? (int64_t)(z + a.g) * a.osz : 0);
+    if (a.rho_out) a.rho_out[o] = rho;
+    if (a.u_out) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) a.u_out[o + d * a.osc] = u[d];
+    }
+  }
+}
+
+// One cell of the sweep with direct (non-LDS) addressing: load with streaming, collide, store.
+template <class S, int FAM, int PAT, bool WALLS, class TS, class TC>
+__device__ __forceinline__ void sweep_cell(const SweepArgs<TC>& a, int x, int y, int z) {
+  constexpr int Q = S::Q, D = S::D;
+  const AxisNb ax = axis_neighbours(x, a.nx, a.g, a.face[0]);
+  const AxisNb ay = axis_neighbours(y, a.ny, a.g, a.face[1]);
+  AxisNb az = {0, 0, false, false};
+  if (D == 3) az = axis_neighbours(z, a.nz, a.g, a.face[2]);
+  // in-plane element offsets of the 3 x positions, 3 rows and 3 planes around the cell
+  const int sy = (int)a.sy, sz = (int)a.sz;
+  const int X[3] = {ax.lo, x + a.g, ax.hi};
+  const int Y[3] = {ay.lo * sy, (y + a.g) * sy, ay.hi * sy};
+  const int Z[3] = {D == 3 ? az.lo * sz : 0, D == 3 ? (z + a.g) * sz : 0, D == 3 ? az.hi * sz : 0};
+  const int center = X[1] + Y[1] + Z[1];
+  const TS* __restrict__ src = reinterpret_cast<const TS*>(a.src);
+  TS* __restrict__ dst = reinterpret_cast<TS*>(a.dst);
+  const int64_t sq = a.sq;
+
+  TC f[Q];
+  if (PAT == LBM355_AA_EVEN) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] = load_nt<TS, TC>(src + i * sq + center);
+  } else {
+    // pull reads src[i] at x - c_i; AA odd reads slot inv(i) there.  At a wall the
+    // bounce-back value sits in the cell itself: pull slot inv(i), AA odd slot i.
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const int cx = S::c[i][0], cy = S::c[i][1], cz = S::c[i][2];
+      const int ii = S::inv[i];
+      const int plane = (PAT == LBM355_PULL) ? i : ii;
+      const int wplane = (PAT == LBM355_PULL) ? ii : i;
+      bool wall = false;
+      if (WALLS)
+        wall = (cx == 1 && ax.wall_lo) || (cx == -1 && ax.wall_hi) || (cy == 1 && ay.wall_lo) ||
+               (cy == -1 && ay.wall_hi) || (cz == 1 && az.wall_lo) || (cz == -1 && az.wall_hi);
+      const TS* p = wall ? (src + wplane * sq + center) : (src + plane * sq + X[1 - cx] + Y[1 - cy] + Z[1 - cz]);
+      f[i] = load_as<TS, TC>(p);
+    }
+  }
+
+  collide_cell<S, FAM, TC>(f, a, x, y, z);
+
+  if (PAT == LBM355_PULL) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) store_nt<TS, TC>(dst + i * sq + center, f[i]);
+  } else if (PAT == LBM355_AA_EVEN) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) store_nt<TS, TC>(dst + S::inv[i] * sq + center, f[i]);
+  } else {
+    // AA odd writes f_i to x + c_i slot i; at a wall it is bounced into the cell's slot inv(i)
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const int cx = S::c[i][0], cy = S::c[i][1], cz = S::c[i][2];
+      const int ii = S::inv[i];
+      bool wall = false;
+      if (WALLS)
+        wall = (cx == -1 && ax.wall_lo) || (cx == 1 && ax.wall_hi) || (cy == -1 && ay.wall_lo) ||
+               (cy == 1 && ay.wall_hi) || (cz == -1 && az.wall_lo) || (cz == 1 && az.wall_hi);
+      TS* p = wall ? (dst + ii * sq + center) : (dst + i * sq + X[1 + cx] + Y[1 + cy] + Z[1 + cz]);
+      store_as<TS, TC>(p, f[i]);
+    }
+  }
+}
+
+// Kernel arguments from a public descriptor (include/lbm355.h).
+template <class TC>
+inline SweepArgs<TC> make_args(const lbm355_sweep_desc* d, const void* src, void* dst, void* rho, void* u) {
+  SweepArgs<TC> a;
+  a.src = src;
+  a.dst = dst;
+  a.rho_out = (TC*)rho;
+  a.u_out = (TC*)u;
+  a.tab = (const TC*)d->tables;
+  a.nx = d->nx;
+  a.ny = d->ny;
+  a.nz = d->nz;
+  a.g = d->ghost_layers;
+  a.sy = d->stride_y;
+  a.sz = d->stride_z;
+  a.sq = d->stride_q;
+  a.osy = d->out_stride_y;
+  a.osz = d->out_stride_z;
+  a.osc = d->out_stride_c;
+  for (int i = 0; i < 3; ++i) {
+    a.face[i] = d->face[i];
+    a.force[i] = (TC)d->force[i];
+  }
+  a.flags = d->flags;
+  a.nactive = d->n_active;
+  a.smag_cs2x18 = (TC)(18.0 * d->smagorinsky_constant * d->smagorinsky_constant);
+  a.omega_shear = (TC)d->omega_shear;
+  a.closure = d->closure;
+  return a;
+}
+
+}  // namespace lbm355
+
+// checks a sweep descriptor (sizes, strides, dtypes, table pointer); defined in sweep.hip
+int lbm355_validate_desc(const lbm355_sweep_desc* d);

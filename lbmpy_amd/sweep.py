@@ -107,6 +107,7 @@ class SweepKernel:
             if model != SubgridScaleModel.SMAGORINSKY:
                 raise NotImplementedError(f"subgrid model {model} is not implemented on the GPU path")
             self.smagorinsky = float(const) if const is not None else 0.12
+        self.closure = None
         self._tables = {}
         self.free_symbols = m.relaxation_rate_symbols()
         # validate the method once with dummy bindings so unsupported configurations fail here
@@ -114,6 +115,17 @@ class SweepKernel:
         nat.load()
 
     # ------------------------------------------------------------------ configuration
+    def set_closure(self, model):
+        """Use a learned closure (:class:`lbmpy_amd.closure.ClosureModel`) for the shear rate of this FP32
+        cumulant sweep in place of the method's fixed rate (or its Smagorinsky rate); ``None`` removes it."""
+        if model is not None:
+            if self.family != nat.FAMILY_CUMULANT or self.compute != nat.F32 or self.store != nat.F32:
+                raise NotImplementedError("the learned closure runs in the FP32 cumulant sweep with FP32 storage")
+            if self.smagorinsky is not None:
+                raise ValueError("a sweep uses either the Smagorinsky model or a learned closure")
+        self.closure = model
+        return self
+
     def set_faces(self, faces):
         """Per-axis face treatment: 'periodic', 'noslip' or 'ghost'."""
         codes = {"periodic": nat.FACE_PERIODIC, "noslip": nat.FACE_NOSLIP, "ghost": nat.FACE_GHOST}
@@ -172,6 +184,10 @@ class SweepKernel:
         if self.smagorinsky is not None:
             flags |= nat.FLAG_SMAGORINSKY
             d.smagorinsky_constant = self.smagorinsky
+            d.omega_shear = bind_rate(get_shear_relaxation_rate(m), bindings)
+        if self.closure is not None:
+            flags |= nat.FLAG_LEARNED_CLOSURE
+            d.closure = self.closure.params.data_ptr()
             d.omega_shear = bind_rate(get_shear_relaxation_rate(m), bindings)
         d.flags = flags
         d.n_active = nact

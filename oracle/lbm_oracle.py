@@ -72,10 +72,12 @@ def _mono_exponents(polys, dim):
 class OracleCollision:
     """Collision operator of one method on arrays shaped (Q, ncells)."""
 
-    def __init__(self, method, bindings=None, smagorinsky=None):
-        """``smagorinsky``: Smagorinsky constant Cs (turbulence_models.py:25-84) or None."""
+    def __init__(self, method, bindings=None, smagorinsky=None, closure=None):
+        """``smagorinsky``: Smagorinsky constant Cs (turbulence_models.py:25-84) or None;
+        ``closure``: callable (pi_neq (n, 6), rho (n,), omega_shear) -> shear rate (n,), or None."""
         self.m = method
         self.smagorinsky = smagorinsky
+        self.closure = closure
         self.bind = dict(bindings or {})
         self.stencil = method.stencil
         self.Q = method.stencil.Q
@@ -268,6 +270,8 @@ class OracleCollision:
             rates = np.repeat(rates[:, None], n, axis=1)
             if self.smagorinsky is not None:
                 rates = self._smagorinsky_rates(f, rho, drho, u, rates)
+            if self.closure is not None:
+                rates = self._closure_rates(f, rho, drho, u, rates)
             Cp = C + rates * (ceq - C)
             cpost = self.Pinv @ Cp
             kpost = self._central_from_cumulants(cpost, rho, u3, kappa[self.midx[(0, 0, 0)]])
@@ -279,26 +283,42 @@ class OracleCollision:
             mpost = mpost - self.bg[:, None]
         return self.Mmono_inv @ mpost, rho, u_out
 
+    def _pi_neq(self, f, rho, drho, u):
+        """Pi_neq_ab = sum_i c_ia c_ib (f_i - f_eq_i), f_eq the method's equilibrium populations at (rho, u)
+        (utils.py:4-13 as used by turbulence_models.py:59-64); rows xx, yy, zz, xy, xz, yz (3-D)."""
+        feq = equilibrium_pdfs_from_eq_velocity(self, rho, u)
+        fneq = f - feq
+        pairs = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)] if self.D == 3 else [(0, 0), (1, 1), (0, 1)]
+        return np.stack([(self.c[:, a] * self.c[:, b]) @ fneq for a, b in pairs])
+
     def _smagorinsky_rates(self, f, rho, drho, u, rates):
         """Replace the shear rate by 2 / (tau0 + sqrt(18 Cs^2 Pi + tau0^2)) with
         Pi = sqrt(2 sum_ab Pi_neq_ab^2) / rho_ref, Pi_neq = sum_i c_ia c_ib (f_i - f_eq_i) and
         f_eq the method's equilibrium populations at (rho, u) (turbulence_models.py:59-71,
         utils.py:4-13)."""
         from lbmpy_amd.moments import is_shear_moment
-        feq = equilibrium_pdfs_from_eq_velocity(self, rho, u)
-        fneq = f - feq
         D = self.D
-        pi2 = np.zeros_like(rho)
-        for a in range(D):
-            for b in range(D):
-                p_ab = (self.c[:, a] * self.c[:, b]) @ fneq
-                pi2 = pi2 + p_ab * p_ab
+        p = self._pi_neq(f, rho, drho, u)
+        diag = p[:D]
+        off = p[D:]
+        pi2 = (diag ** 2).sum(0) + 2.0 * (off ** 2).sum(0)
         ref = rho if self.m.compressible else np.ones_like(rho)
         Pi = np.sqrt(2.0 * pi2) / ref
-        shear = [i for i, p in enumerate(self.polys) if is_shear_moment(p, D)]
+        shear = [i for i, p_ in enumerate(self.polys) if is_shear_moment(p_, D)]
         omega_s = self.rates[shear[0]]
         tau0 = 1.0 / omega_s
         w = 2.0 / (tau0 + np.sqrt(18.0 * self.smagorinsky ** 2 * Pi + tau0 ** 2))
+        out = rates.copy()
+        for i in shear:
+            out[i] = w
+        return out
+
+    def _closure_rates(self, f, rho, drho, u, rates):
+        """Shear rate per cell from ``self.closure(pi_neq (n, 6), rho (n,), omega_shear)`` (a learned closure)."""
+        from lbmpy_amd.moments import is_shear_moment
+        p = self._pi_neq(f, rho, drho, u)
+        shear = [i for i, p_ in enumerate(self.polys) if is_shear_moment(p_, self.D)]
+        w = self.closure(p.T, rho, self.rates[shear[0]])
         out = rates.copy()
         for i in shear:
             out[i] = w

@@ -49,7 +49,7 @@ def test_every_declared_symbol_is_exported(lib):
 
 def test_descriptor_layout_matches(lib):
     from lbmpy_amd import _native
-    assert lib.lbm355_desc_size() == ctypes.sizeof(_native.SweepDesc) == 152
+    assert lib.lbm355_desc_size() == ctypes.sizeof(_native.SweepDesc) == 160
     assert lib.lbm355_version() >= 1
 
 
@@ -66,7 +66,7 @@ def test_header_constants_match_binding():
         "LBM355_FACE_NOSLIP": n.FACE_NOSLIP,
         "LBM355_FLAG_ZERO_CENTERED": n.FLAG_ZERO_CENTERED, "LBM355_FLAG_COMPRESSIBLE": n.FLAG_COMPRESSIBLE,
         "LBM355_FLAG_EQ_SHIFT": n.FLAG_EQ_SHIFT, "LBM355_FLAG_FORCE": n.FLAG_FORCE,
-        "LBM355_FLAG_SMAGORINSKY": n.FLAG_SMAGORINSKY,
+        "LBM355_FLAG_SMAGORINSKY": n.FLAG_SMAGORINSKY, "LBM355_FLAG_LEARNED_CLOSURE": n.FLAG_LEARNED_CLOSURE,
     }
     for k, v in expect.items():
         assert macros.get(k) == v, k
