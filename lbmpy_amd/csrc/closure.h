@@ -14,7 +14,8 @@
 //
 // Features of a cell (raw, before the per-feature normalisation x' = (x - shift) * scale):
 //   q_ab = Pi_neq_ab / rho (xx, yy, zz, xy, xz, yz), Pi = sqrt(2 q:q), tr q, q:q, tr(q^3)
-// Output: nu_t = nu_scale * softplus(o);  omega = 1 / (1/omega_0 + 3 nu_t).
+// Output: nu_t = nu_scale * softplus(o) - nu_shift;  omega = 1 / (1/omega_0 + 3 nu_t), with the relaxation
+// time kept >= 0.5 + 3e-6 (omega < 2).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -32,7 +33,8 @@ struct ClosureParams {
   float b3;
   float nu_scale;
   float in_shift[CLOSURE_IN], in_scale[CLOSURE_IN];
-  float pad[2];
+  float nu_shift;                           // constant viscosity offset (calibrated), see closure_omega
+  float pad;
 };
 
 typedef short v4s __attribute__((ext_vector_type(4)));
@@ -147,9 +149,11 @@ __device__ __forceinline__ float mlp_forward_wave(const ClosureParams* __restric
   return out;
 }
 
-// eddy-viscosity relaxation rate from the network output
+// shear relaxation rate from the network output: nu_t = nu_scale softplus(o) - nu_shift added to the
+// molecular viscosity, relaxation time at least 0.5 + 3e-6
 __device__ __forceinline__ float closure_omega(const ClosureParams* __restrict__ P, float o, float omega0) {
-  return 1.f / (1.f / omega0 + 3.f * P->nu_scale * softplus(o));
+  const float tau = 1.f / omega0 + 3.f * (P->nu_scale * softplus(o) - P->nu_shift);
+  return 1.f / fmaxf(tau, 0.500003f);
 }
 
 }  // namespace lbm355

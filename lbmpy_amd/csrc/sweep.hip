@@ -48,11 +48,12 @@ __global__ void __launch_bounds__(ROWLDS ? 512 : 256) sweep_kernel(SweepArgs<TC>
     TS* tile = reinterpret_cast<TS*>(lds_raw);
     const int nx = a.nx;
 
+    // walls are only in y / z here (x is periodic), so whether a block's row touches one is uniform
+    const bool row_walls = WALLS && (ay.wall_lo || ay.wall_hi || az.wall_lo || az.wall_hi);
+
     TC f[Q];
-    if (PAT == LBM355_AA_EVEN) {
-#pragma unroll
-      for (int i = 0; i < Q; ++i) f[i] = load_nt<TS, TC>(src + i * sq + center);
-    } else {
+    auto load = [&](auto walls) {
+      constexpr bool W = decltype(walls)::value;
       int j = 0;
 #pragma unroll
       for (int i = 0; i < Q; ++i) {
@@ -61,11 +62,10 @@ __global__ void __launch_bounds__(ROWLDS ? 512 : 256) sweep_kernel(SweepArgs<TC>
         const int plane = (PAT == LBM355_PULL) ? i : ii;
         const int wplane = (PAT == LBM355_PULL) ? ii : i;
         bool wall = false;
-        if (WALLS)
+        if (W)
           wall = (cy == 1 && ay.wall_lo) || (cy == -1 && ay.wall_hi) || (cz == 1 && az.wall_lo) ||
                  (cz == -1 && az.wall_hi);
         if (cx != 0) {
-          // x is periodic here, so `wall` is uniform over the row (block)
           if (wall)
             f[i] = load_nt<TS, TC>(src + wplane * sq + center);
           else
@@ -83,32 +83,24 @@ __global__ void __launch_bounds__(ROWLDS ? 512 : 256) sweep_kernel(SweepArgs<TC>
         const int cx = S::c[i][0], cy = S::c[i][1], cz = S::c[i][2];
         if (cx == 0) continue;
         bool wall = false;
-        if (WALLS)
+        if (W)
           wall = (cy == 1 && ay.wall_lo) || (cy == -1 && ay.wall_hi) || (cz == 1 && az.wall_lo) ||
                  (cz == -1 && az.wall_hi);
         if (!wall) f[i] = (TC)tile[j * nx + XL[1 - cx]];
         ++j;
       }
       __syncthreads();
-    }
-
-    collide_cell<S, FAM, TC>(f, a, x, y, z);
-
-    if (PAT == LBM355_PULL) {
-#pragma unroll
-      for (int i = 0; i < Q; ++i) store_nt<TS, TC>(dst + i * sq + center, f[i]);
-    } else if (PAT == LBM355_AA_EVEN) {
-#pragma unroll
-      for (int i = 0; i < Q; ++i) store_nt<TS, TC>(dst + S::inv[i] * sq + center, f[i]);
-    } else {
-      // AA odd writes f_i to x + c_i slot i; at a wall it is bounced into the cell's slot inv(i)
+    };
+    // AA odd writes f_i to x + c_i slot i; at a wall it is bounced into the cell's slot inv(i)
+    auto store = [&](auto walls) {
+      constexpr bool W = decltype(walls)::value;
       int j = 0;
 #pragma unroll
       for (int i = 0; i < Q; ++i) {
         const int cx = S::c[i][0], cy = S::c[i][1], cz = S::c[i][2];
         const int ii = S::inv[i];
         bool wall = false;
-        if (WALLS)
+        if (W)
           wall = (cy == -1 && ay.wall_lo) || (cy == 1 && ay.wall_hi) || (cz == -1 && az.wall_lo) ||
                  (cz == 1 && az.wall_hi);
         if (cx != 0) {
@@ -129,12 +121,35 @@ __global__ void __launch_bounds__(ROWLDS ? 512 : 256) sweep_kernel(SweepArgs<TC>
         const int cx = S::c[i][0], cy = S::c[i][1], cz = S::c[i][2];
         if (cx == 0) continue;
         bool wall = false;
-        if (WALLS)
+        if (W)
           wall = (cy == -1 && ay.wall_lo) || (cy == 1 && ay.wall_hi) || (cz == -1 && az.wall_lo) ||
                  (cz == 1 && az.wall_hi);
         if (!wall) __builtin_nontemporal_store(tile[j * nx + XL[1 - cx]], dst + i * sq + X1 + Y[1 + cy] + Z[1 + cz]);
         ++j;
       }
+    };
+
+    if (PAT == LBM355_AA_EVEN) {
+#pragma unroll
+      for (int i = 0; i < Q; ++i) f[i] = load_nt<TS, TC>(src + i * sq + center);
+    } else if (row_walls) {
+      load(std::true_type{});
+    } else {
+      load(std::false_type{});
+    }
+
+    collide_cell<S, FAM, TC>(f, a, x, y, z);
+
+    if (PAT == LBM355_PULL) {
+#pragma unroll
+      for (int i = 0; i < Q; ++i) store_nt<TS, TC>(dst + i * sq + center, f[i]);
+    } else if (PAT == LBM355_AA_EVEN) {
+#pragma unroll
+      for (int i = 0; i < Q; ++i) store_nt<TS, TC>(dst + S::inv[i] * sq + center, f[i]);
+    } else if (row_walls) {
+      store(std::true_type{});
+    } else {
+      store(std::false_type{});
     }
   }
 }

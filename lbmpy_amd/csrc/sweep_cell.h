@@ -3,6 +3,8 @@
 // collide_cell / sweep_cell, so a cell updated in a batch runs exactly the instructions of the
 // same cell updated on its own.
 #pragma once
+#include <type_traits>
+
 #include "lbm_common.h"
 #include "collide_moment.h"
 #include "collide_cumulant.h"
@@ -100,6 +102,8 @@ __device__ __forceinline__ void collide_cell(TC (&f)[S::Q], const SweepArgs<TC>&
 }
 
 // One cell of the sweep with direct (non-LDS) addressing: load with streaming, collide, store.
+// With WALLS, only cells next to a NoSlip face take the bounce-back addressing; the test is per cell and
+// the wall-free addressing is the same code as without WALLS, so interior cells pay nothing for walls.
 template <class S, int FAM, int PAT, bool WALLS, class TS, class TC>
 __device__ __forceinline__ void sweep_cell(const SweepArgs<TC>& a, int x, int y, int z) {
   constexpr int Q = S::Q, D = S::D;
@@ -116,14 +120,14 @@ __device__ __forceinline__ void sweep_cell(const SweepArgs<TC>& a, int x, int y,
   const TS* __restrict__ src = reinterpret_cast<const TS*>(a.src);
   TS* __restrict__ dst = reinterpret_cast<TS*>(a.dst);
   const int64_t sq = a.sq;
+  const bool near_wall =
+      WALLS && (ax.wall_lo || ax.wall_hi || ay.wall_lo || ay.wall_hi || az.wall_lo || az.wall_hi);
 
   TC f[Q];
-  if (PAT == LBM355_AA_EVEN) {
-#pragma unroll
-    for (int i = 0; i < Q; ++i) f[i] = load_nt<TS, TC>(src + i * sq + center);
-  } else {
-    // pull reads src[i] at x - c_i; AA odd reads slot inv(i) there.  At a wall the
-    // bounce-back value sits in the cell itself: pull slot inv(i), AA odd slot i.
+  // pull reads src[i] at x - c_i; AA odd reads slot inv(i) there.  At a wall the
+  // bounce-back value sits in the cell itself: pull slot inv(i), AA odd slot i.
+  auto load = [&](auto walls) {
+    constexpr bool W = decltype(walls)::value;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       const int cx = S::c[i][0], cy = S::c[i][1], cz = S::c[i][2];
@@ -131,12 +135,36 @@ __device__ __forceinline__ void sweep_cell(const SweepArgs<TC>& a, int x, int y,
       const int plane = (PAT == LBM355_PULL) ? i : ii;
       const int wplane = (PAT == LBM355_PULL) ? ii : i;
       bool wall = false;
-      if (WALLS)
+      if (W)
         wall = (cx == 1 && ax.wall_lo) || (cx == -1 && ax.wall_hi) || (cy == 1 && ay.wall_lo) ||
                (cy == -1 && ay.wall_hi) || (cz == 1 && az.wall_lo) || (cz == -1 && az.wall_hi);
       const TS* p = wall ? (src + wplane * sq + center) : (src + plane * sq + X[1 - cx] + Y[1 - cy] + Z[1 - cz]);
       f[i] = load_as<TS, TC>(p);
     }
+  };
+  // AA odd writes f_i to x + c_i slot i; at a wall it is bounced into the cell's slot inv(i)
+  auto store = [&](auto walls) {
+    constexpr bool W = decltype(walls)::value;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const int cx = S::c[i][0], cy = S::c[i][1], cz = S::c[i][2];
+      const int ii = S::inv[i];
+      bool wall = false;
+      if (W)
+        wall = (cx == -1 && ax.wall_lo) || (cx == 1 && ax.wall_hi) || (cy == -1 && ay.wall_lo) ||
+               (cy == 1 && ay.wall_hi) || (cz == -1 && az.wall_lo) || (cz == 1 && az.wall_hi);
+      TS* p = wall ? (dst + ii * sq + center) : (dst + i * sq + X[1 + cx] + Y[1 + cy] + Z[1 + cz]);
+      store_as<TS, TC>(p, f[i]);
+    }
+  };
+
+  if (PAT == LBM355_AA_EVEN) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] = load_nt<TS, TC>(src + i * sq + center);
+  } else if (near_wall) {
+    load(std::true_type{});
+  } else {
+    load(std::false_type{});
   }
 
   collide_cell<S, FAM, TC>(f, a, x, y, z);
@@ -147,19 +175,10 @@ __device__ __forceinline__ void sweep_cell(const SweepArgs<TC>& a, int x, int y,
   } else if (PAT == LBM355_AA_EVEN) {
 #pragma unroll
     for (int i = 0; i < Q; ++i) store_nt<TS, TC>(dst + S::inv[i] * sq + center, f[i]);
+  } else if (near_wall) {
+    store(std::true_type{});
   } else {
-    // AA odd writes f_i to x + c_i slot i; at a wall it is bounced into the cell's slot inv(i)
-#pragma unroll
-    for (int i = 0; i < Q; ++i) {
-      const int cx = S::c[i][0], cy = S::c[i][1], cz = S::c[i][2];
-      const int ii = S::inv[i];
-      bool wall = false;
-      if (WALLS)
-        wall = (cx == -1 && ax.wall_lo) || (cx == 1 && ax.wall_hi) || (cy == -1 && ay.wall_lo) ||
-               (cy == 1 && ay.wall_hi) || (cz == -1 && az.wall_lo) || (cz == 1 && az.wall_hi);
-      TS* p = wall ? (dst + ii * sq + center) : (dst + i * sq + X[1 + cx] + Y[1 + cy] + Z[1 + cz]);
-      store_as<TS, TC>(p, f[i]);
-    }
+    store(std::false_type{});
   }
 }
 

@@ -108,6 +108,7 @@ class SweepKernel:
                 raise NotImplementedError(f"subgrid model {model} is not implemented on the GPU path")
             self.smagorinsky = float(const) if const is not None else 0.12
         self.closure = None
+        self._shear_rate = None          # (bindings key, bound shear rate): the lookup is symbolic and slow
         self._tables = {}
         self.free_symbols = m.relaxation_rate_symbols()
         # validate the method once with dummy bindings so unsupported configurations fail here
@@ -141,6 +142,13 @@ class SweepKernel:
             t = (torch.tensor(arr, dtype=dt, device=device), nact)
             self._tables[key] = t
         return t
+
+    def shear_rate(self, bindings):
+        """The method's shear relaxation rate with the free symbols bound (cached per binding set)."""
+        key = tuple(sorted(bindings.items()))
+        if self._shear_rate is None or self._shear_rate[0] != key:
+            self._shear_rate = (key, bind_rate(get_shear_relaxation_rate(self.method), bindings))
+        return self._shear_rate[1]
 
     def descriptor(self, arr, bindings, out_strides=(0, 0, 0)):
         m = self.method
@@ -184,11 +192,11 @@ class SweepKernel:
         if self.smagorinsky is not None:
             flags |= nat.FLAG_SMAGORINSKY
             d.smagorinsky_constant = self.smagorinsky
-            d.omega_shear = bind_rate(get_shear_relaxation_rate(m), bindings)
+            d.omega_shear = self.shear_rate(bindings)
         if self.closure is not None:
             flags |= nat.FLAG_LEARNED_CLOSURE
             d.closure = self.closure.params.data_ptr()
-            d.omega_shear = bind_rate(get_shear_relaxation_rate(m), bindings)
+            d.omega_shear = self.shear_rate(bindings)
         d.flags = flags
         d.n_active = nact
         d.tables = tab.data_ptr()

@@ -7,7 +7,7 @@ module.  It restates, in numpy, the network that ``lbmpy_amd/csrc/closure.h`` ru
     h1 = tanh(W1t x' + b1),  a1 = bf16(h1)               (32 units)
     h2 = tanh(W2t a1 + b2)                               (32 units)
     o  = w3 . h2 + b3                                    (w3 in BF16)
-    nu_t = nu_scale * softplus(o),  omega = 1 / (1/omega_0 + 3 nu_t)
+    nu_t = nu_scale * softplus(o) - nu_shift,  omega = 1 / max(1/omega_0 + 3 nu_t, 0.5 + 3e-6)
 
 and its gradient for the training loss L = (1/n) sum (softplus(o) - y)^2 with respect to the master
 parameters (straight-through the BF16 roundings).  The features follow closure_features: q = Pi_neq / rho,
@@ -88,5 +88,6 @@ def loss_and_grad(master, shift, scale, x, y, round_bf16=True):
     return loss, grad
 
 
-def omega(o, nu_scale, omega0):
-    return 1.0 / (1.0 / omega0 + 3.0 * nu_scale * softplus(o))
+def omega(o, nu_scale, omega0, nu_shift=0.0):
+    """omega = 1 / max(1/omega0 + 3 (nu_scale softplus(o) - nu_shift), 0.5 + 3e-6)."""
+    return 1.0 / np.maximum(1.0 / omega0 + 3.0 * (nu_scale * softplus(o) - nu_shift), 0.500003)
