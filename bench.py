@@ -280,8 +280,10 @@ def run_ensemble(args):
     src = [create_pdf_array(sh, 19, ghost_layers=0, dtype="float32") for sh in shapes]
     dst = [create_pdf_array(sh, 19, ghost_layers=0, dtype="float32") for sh in shapes]
     cells = sum(int(np.prod(sh)) for sh in shapes)
+    print(f"ensemble: {n} members, {cells} cells", flush=True)
     b = BatchedSweep(kernels, src, dst)
     tuned = b.autotune()
+    print(f"ensemble: autotuned launch {tuned['launch']}", flush=True)
     for a in src + dst:
         a.zero_()
     b.steps_done = 0
@@ -295,6 +297,7 @@ def run_ensemble(args):
     ms = e0.elapsed_time(e1)
     batched = [a.clone() for a in b.current_arrays()]
     del graph
+    print(f"ensemble: batched run {ms:.1f} ms", flush=True)
     # lbmpy's way: one launch per member and step
     for a in src + dst:
         a.zero_()
@@ -324,6 +327,29 @@ def run_ensemble(args):
     print(json.dumps(line), flush=True)
 
 
+def run_closure(args):
+    """Learned closure on decaying homogeneous turbulence: D3Q27 cumulant, FP32 PDFs, AA, periodic; coarse
+    `closure_coarse`^3 grid, fine grid 4x finer (box filter by 4), seeded solenoidal field with
+    E(k) ~ k^4 exp(-2 (k/k0)^2), k0 = 4, u_rms = 0.04, omega_shear = 1.99.  Trained on the fine run of seed 0,
+    judged on seed 1 against the filtered fine run, beside Smagorinsky (Cs = 0.12) and no subgrid model."""
+    from lbmpy_amd.closure import turbulence_study
+    r = turbulence_study(n_coarse=args.closure_coarse, factor=4, steps=args.closure_steps,
+                         snap_every=args.closure_steps // 12, train_from=args.closure_steps // 6, epochs=20,
+                         batch=65536, seed=0, log=lambda s: print(s, flush=True))
+    r.pop("closure")
+    models = {k: {"ke_rel_l2_err": round(v["ke_rel_l2_err"], 5), "spectrum_log_err": round(v["spectrum_log_err"], 5),
+                  "coarse_mlups": round(v["mlups"], 1)} for k, v in r["models"].items()}
+    line = {"metric": "learned closure: kinetic-energy decay and spectrum error vs the box-filtered fine run",
+            "workload": f"D3Q27 cumulant decaying turbulence, coarse {r['n_coarse']}^3 (omega 1.99), fine "
+                        f"{r['n_fine']}^3 (omega {r['omega_fine']:.5f}), k0=4, u_rms=0.04, {r['steps']} coarse steps, "
+                        f"FP32 AA; trained on seed 0, judged on seed 1",
+            "models": models, "alpha": r["alpha"], "beta": r["beta"], "offset_only_beta": r["offset_only_beta"],
+            "train_loss_first_last": [r["train_loss"][0], r["train_loss"][-1]],
+            "train_seconds": round(r["train_seconds"], 2), "fine_mlups": round(r["fine_mlups"], 1),
+            "ref_ke": r["ref_ke"], "ke": {k: v["ke"] for k, v in r["models"].items()}}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -337,9 +363,11 @@ def main():
     ap.add_argument("--forms", default="float64,float16",
                     help="other storage forms run beside the headline one ('' for none)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="tgv", choices=["tgv", "channel", "ensemble"],
+    ap.add_argument("--workload", default="tgv", choices=["tgv", "channel", "ensemble", "closure"],
                     help="tgv: the headline D3Q27 cumulant TGV; channel: D3Q19 MRT channel in three storage forms; "
-                         "ensemble: 256 batched D3Q19 TRT channels")
+                         "ensemble: 256 batched D3Q19 TRT channels; closure: learned closure on decaying turbulence")
+    ap.add_argument("--closure-coarse", type=int, default=128)
+    ap.add_argument("--closure-steps", type=int, default=1200)
     ap.add_argument("--members", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=1000)
     ap.add_argument("--seed", type=int, default=2024)
@@ -361,6 +389,9 @@ def main():
         return
     if args.workload == "ensemble":
         run_ensemble(args)
+        return
+    if args.workload == "closure":
+        run_closure(args)
         return
 
     import torch

@@ -147,7 +147,7 @@ int lbm355_stream_copy(const void* src, void* dst, int64_t n_floats, int32_t non
  * bit-identical to sweeping it alone. */
 typedef struct lbm355_batch lbm355_batch;
 
-/* Uploads the member table (one device allocation).  pdf_src[k] / pdf_dst[k]: member k's arrays
+/* Uploads the member table and a tile -> member map (two device allocations).  pdf_src[k] / pdf_dst[k]: member k's arrays
  * (equal for AA).  The table keeps the pointers: the arrays must outlive the batch. */
 int lbm355_batch_create(const lbm355_sweep_desc* descs, const void* const* pdf_src, void* const* pdf_dst,
                         int32_t n_members, lbm355_batch** batch);

@@ -181,13 +181,15 @@ def solenoidal_field(n, k0=4.0, u_rms=0.04, seed=0, device="cuda"):
     kk = torch.sqrt(kx ** 2 + ky ** 2 + kz ** 2)
     amp = torch.where(kk > 0, torch.sqrt(kk ** 4 * torch.exp(-2 * (kk / k0) ** 2) / (4 * math.pi * kk ** 2 + 1e-300)),
                       torch.zeros_like(kk))
-    re = torch.randn((3,) + kk.shape, generator=gen, dtype=torch.float64)
-    im = torch.randn((3,) + kk.shape, generator=gen, dtype=torch.float64)
+    re = torch.randn((3,) + kk.shape, generator=gen, dtype=torch.float64).to(device)
+    im = torch.randn((3,) + kk.shape, generator=gen, dtype=torch.float64).to(device)
+    kx, ky, kz, kk, amp = (t.to(device) for t in (kx, ky, kz, kk, amp))
     c = torch.complex(re, im) * amp
+    del re, im
     kv = torch.stack([kx, ky, kz])
     kdotc = (kv * c).sum(0)
     c = c - kv * kdotc / torch.where(kk > 0, kk ** 2, torch.ones_like(kk))            # project out k || c
-    u = torch.stack([torch.fft.irfftn(c[a], s=(n, n, n)) for a in range(3)]).to(device)
+    u = torch.stack([torch.fft.irfftn(c[a], s=(n, n, n)) for a in range(3)])
     u = u * (u_rms / torch.sqrt((u ** 2).mean(dim=(1, 2, 3)).mean()))
     return u
 

@@ -5,10 +5,10 @@
 // Work split: every member's interior cells, in memory order (x fastest), are cut into tiles of
 // 64 cells, one wavefront each; tiles are numbered consecutively over the members.  The grid is
 // persistent: wave w of G handles tiles w, w + G, w + 2G, ... until all are done, so any grid size
-// covers any ensemble and every wave reaches the end of its loop.  A wave finds its member by a
-// binary search over the members' first-tile prefix (wave-uniform, scalar loads), decodes (x, y, z)
-// and runs sweep_cell, the same per-cell function as the single-domain sweep (sweep_cell.h):
-// a member's cells get bit-identical results to a single-domain run of that member.
+// covers any ensemble and every wave reaches the end of its loop.  A wave reads its tile's member from
+// a tile -> member map (one wave-uniform load), decodes (x, y, z) with 32-bit arithmetic (a member
+// holds < 2^31 cells) and runs sweep_cell, the same per-cell function as the single-domain sweep
+// (sweep_cell.h): a member's cells get bit-identical results to a single-domain run of that member.
 //
 // Optional XCD-aware order: MI355X dispatches consecutive workgroups round-robin over its 8 XCDs
 // (each with its own L2).  With `xcd_order`, the workgroups of one XCD take one contiguous
@@ -28,9 +28,12 @@ struct BatchEntry {
   int64_t ncells;       // interior cells
 };
 
+// occupancy target 6 waves per SIMD: the compiler otherwise keeps 113 VGPRs (4 waves) for the argument
+// loads and index decoding; 80 VGPRs hold everything without spills
 template <class S, int FAM, int PAT, class TS, class TC>
-__global__ void __launch_bounds__(256) batch_kernel(const BatchEntry<TC>* __restrict__ members, int nmembers,
-                                                    int64_t ntiles, int swap, int xcd_order) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8)))
+batch_kernel(const BatchEntry<TC>* __restrict__ members, const int32_t* __restrict__ tile_member, int64_t ntiles,
+             int swap, int xcd_order) {
   const int waves_per_block = blockDim.x >> 6;
   int b = blockIdx.x;
   if (xcd_order && (gridDim.x & 7) == 0) b = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
@@ -38,35 +41,28 @@ __global__ void __launch_bounds__(256) batch_kernel(const BatchEntry<TC>* __rest
   const int64_t nwaves = (int64_t)gridDim.x * waves_per_block;
   const int lane = threadIdx.x & 63;
   for (int64_t tile = wave; tile < ntiles; tile += nwaves) {
-    // last member whose first tile is <= tile (tile is uniform over the wave)
-    int lo = 0, hi = nmembers - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (members[mid].first_tile <= tile) lo = mid;
-      else hi = mid - 1;
-    }
-    lo = __builtin_amdgcn_readfirstlane(lo);
-    const BatchEntry<TC>* e = members + lo;
-    const int64_t local = (tile - e->first_tile) * 64 + lane;
-    if (local < e->ncells) {
+    const int m = __builtin_amdgcn_readfirstlane(tile_member[tile]);     // tile is uniform over the wave
+    const BatchEntry<TC>* e = members + m;
+    const uint32_t local = (uint32_t)(tile - e->first_tile) * 64u + (uint32_t)lane;
+    if (local < (uint32_t)e->ncells) {
       SweepArgs<TC> a = e->a;
       if (swap) {
         void* t = const_cast<void*>(a.src);
         a.src = a.dst;
         a.dst = t;
       }
-      const int nx = a.nx, ny = a.ny;
-      const int64_t row = local / nx;
+      const uint32_t nx = (uint32_t)a.nx, ny = (uint32_t)a.ny;
+      const uint32_t row = local / nx;
       const int x = (int)(local - row * nx);
-      const int z = (int)(row / ny);
-      const int y = (int)(row - (int64_t)z * ny);
-      sweep_cell<S, FAM, PAT, true, TS, TC>(a, x, y, z);
+      const uint32_t zz = row / ny;
+      const int y = (int)(row - zz * ny);
+      sweep_cell<S, FAM, PAT, true, TS, TC>(a, x, y, (int)zz);
     }
   }
 }
 
 template <class TC>
-using BatchFn = void (*)(const BatchEntry<TC>*, int, int64_t, int, int);
+using BatchFn = void (*)(const BatchEntry<TC>*, const int32_t*, int64_t, int, int);
 
 template <class S, int FAM, class TS, class TC>
 void* batch_pick_pattern(int pat) {
@@ -111,6 +107,7 @@ struct lbm355_batch {
   int64_t ntiles;
   int64_t ncells;
   void* entries;          // device array of BatchEntry<TC>
+  int32_t* tile_member;   // device array: member of each 64-cell tile
   int device;
 };
 
@@ -120,23 +117,31 @@ int berr(int code, const std::string& msg) { return lbm355_set_error(code, msg.c
 template <class TC>
 int upload(const lbm355_sweep_desc* d, const void* const* src, void* const* dst, int32_t n, lbm355_batch* b) {
   std::vector<BatchEntry<TC>> host(n);
+  std::vector<int32_t> map;
   int64_t tiles = 0, cells = 0;
   for (int k = 0; k < n; ++k) {
     host[k].a = make_args<TC>(&d[k], src[k], dst[k], nullptr, nullptr);
     host[k].first_tile = tiles;
     host[k].ncells = (int64_t)d[k].nx * d[k].ny * d[k].nz;
-    tiles += (host[k].ncells + 63) / 64;
+    if (host[k].ncells >= (int64_t(1) << 31)) return berr(LBM355_ERR_ARG, "a batch member must hold < 2^31 cells");
+    const int64_t t = (host[k].ncells + 63) / 64;
+    map.insert(map.end(), (size_t)t, k);
+    tiles += t;
     cells += host[k].ncells;
   }
   void* dev = nullptr;
+  void* dmap = nullptr;
   hipError_t e = hipMalloc(&dev, sizeof(BatchEntry<TC>) * (size_t)n);
-  if (e != hipSuccess) return berr(LBM355_ERR_LAUNCH, std::string("hipMalloc failed: ") + hipGetErrorString(e));
-  e = hipMemcpy(dev, host.data(), sizeof(BatchEntry<TC>) * (size_t)n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(&dmap, sizeof(int32_t) * map.size());
+  if (e == hipSuccess) e = hipMemcpy(dev, host.data(), sizeof(BatchEntry<TC>) * (size_t)n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dmap, map.data(), sizeof(int32_t) * map.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
-    (void)hipFree(dev);
-    return berr(LBM355_ERR_LAUNCH, std::string("hipMemcpy failed: ") + hipGetErrorString(e));
+    if (dev) (void)hipFree(dev);
+    if (dmap) (void)hipFree(dmap);
+    return berr(LBM355_ERR_LAUNCH, std::string("batch table upload failed: ") + hipGetErrorString(e));
   }
   b->entries = dev;
+  b->tile_member = (int32_t*)dmap;
   b->ntiles = tiles;
   b->ncells = cells;
   return LBM355_OK;
@@ -211,10 +216,10 @@ int lbm355_batch_sweep(const lbm355_batch* b, int32_t step_kind, int32_t threads
   hipStream_t s = (hipStream_t)stream;
   if (b->compute_dtype == LBM355_F64)
     hipLaunchKernelGGL((BatchFn<double>)k, dim3(grid), dim3(threads), 0, s, (const BatchEntry<double>*)b->entries,
-                       b->nmembers, b->ntiles, swap, xcd_order);
+                       b->tile_member, b->ntiles, swap, xcd_order);
   else
     hipLaunchKernelGGL((BatchFn<float>)k, dim3(grid), dim3(threads), 0, s, (const BatchEntry<float>*)b->entries,
-                       b->nmembers, b->ntiles, swap, xcd_order);
+                       b->tile_member, b->ntiles, swap, xcd_order);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return berr(LBM355_ERR_LAUNCH, std::string("launch failed: ") + hipGetErrorString(e));
   return LBM355_OK;
@@ -233,6 +238,8 @@ int lbm355_batch_destroy(lbm355_batch* b) {
   (void)hipGetDevice(&cur);
   (void)hipSetDevice(b->device);
   hipError_t e = hipFree(b->entries);
+  hipError_t e2 = hipFree(b->tile_member);
+  if (e == hipSuccess) e = e2;
   (void)hipSetDevice(cur);
   delete b;
   if (e != hipSuccess) return berr(LBM355_ERR_LAUNCH, std::string("hipFree failed: ") + hipGetErrorString(e));
