@@ -137,6 +137,34 @@ int lbm355_get_macroscopic(const lbm355_sweep_desc* desc, const void* pdf, doubl
  * bandwidth and the FETCH_SIZE / WRITE_SIZE counters against the sweep. */
 int lbm355_stream_copy(const void* src, void* dst, int64_t n_floats, int32_t nontemporal, void* stream);
 
+/* ---- Boundary links (general) --------------------------------------------------------------------
+ * One boundary condition applied on an index list of links between two sweeps, addressed the way
+ * lbmpy's BetweenTimestepsIndexing does (advanced_streaming/indexing.py:12-181): f_out(i) through
+ * the write accessor of the kernel that ran last, f_in(i) through the read accessor of the next one.
+ * Link entries are (x, y, z, d) in ghost-layer coordinates of the PDF array (z = 0 in 2-D); d points
+ * from the fluid cell into the boundary cell.  Coordinates on periodic axes are wrapped into the
+ * interior (the sweeps wrap periodic axes in their index math, so there are no ghost copies). */
+#define LBM355_BC_NOSLIP 0   /* f_in(inv d) = f_out(d)                   boundaryconditions.py:104-139 */
+#define LBM355_BC_UBB 1      /* f_in(inv d) = f_out(d) - 6 w_d rho c_d.u  boundaryconditions.py:724-842 */
+#define LBM355_BC_FREESLIP 2 /* f_in(inv d) = f_out(inv mirror(d)) at x + c_d + n, boundaryconditions.py:360-487 */
+
+typedef struct lbm355_boundary_desc {
+  int32_t stencil, pattern, store_dtype, kind; /* pattern: the kernel that ran last (PULL, AA_EVEN, AA_ODD) */
+  int32_t nx, ny, nz, ghost_layers;            /* interior sizes (nz = 1 in 2-D) */
+  int32_t periodic[3];                         /* 1: wrap this axis */
+  int32_t flags;                               /* LBM355_FLAG_ZERO_CENTERED | LBM355_FLAG_COMPRESSIBLE (UBB) */
+  int32_t normal[3];                           /* FreeSlip: wall normal into the fluid, one non-zero entry */
+  int32_t pad;
+  int64_t stride_y, stride_z, stride_q;
+  double velocity[3];                          /* UBB wall velocity (unless per-link velocities are given) */
+  double density;                              /* UBB wall density if > 0; else from the fluid cell
+                                                  (compressible) or 1 (incompressible) */
+} lbm355_boundary_desc;
+
+/* Applies one boundary condition on n_links links; link_velocity: NULL or n_links x 3 doubles (UBB). */
+int lbm355_boundary_links(const lbm355_boundary_desc* desc, void* pdf, const int32_t* links, int64_t n_links,
+                          const double* link_velocity, void* stream);
+
 /* ---- Batched ensembles -------------------------------------------------------------------------
  * Many independent domains ("members") advanced by one launch per time step.  lbmpy has no batched
  * sweep: an ensemble there is a Python loop calling one compiled kernel per member and time step

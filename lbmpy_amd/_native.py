@@ -26,6 +26,17 @@ DTYPE_CODES = {"float64": F64, "float32": F32, "float16": F16}
 STENCIL_NAMES = {"D2Q9": D2Q9, "D3Q19": D3Q19, "D3Q27": D3Q27}
 
 
+BC_NOSLIP, BC_UBB, BC_FREESLIP = 0, 1, 2
+
+
+class BoundaryDesc(ctypes.Structure):
+    _fields_ = [("stencil", c_int32), ("pattern", c_int32), ("store_dtype", c_int32), ("kind", c_int32),
+                ("nx", c_int32), ("ny", c_int32), ("nz", c_int32), ("ghost_layers", c_int32),
+                ("periodic", c_int32 * 3), ("flags", c_int32), ("normal", c_int32 * 3), ("pad", c_int32),
+                ("stride_y", c_int64), ("stride_z", c_int64), ("stride_q", c_int64),
+                ("velocity", c_double * 3), ("density", c_double)]
+
+
 class SweepDesc(ctypes.Structure):
     _fields_ = [("stencil", c_int32), ("family", c_int32), ("pattern", c_int32), ("store_dtype", c_int32),
                 ("compute_dtype", c_int32), ("nx", c_int32), ("ny", c_int32), ("nz", c_int32),
@@ -42,7 +53,7 @@ EXPORTED = ("lbm355_sweep", "lbm355_table_elements", "lbm355_noslip_links", "lbm
             "lbm355_last_error", "lbm355_version", "lbm355_batch_create", "lbm355_batch_sweep",
             "lbm355_batch_info", "lbm355_batch_destroy", "lbm355_closure_params_size",
             "lbm355_closure_master_size", "lbm355_closure_features", "lbm355_closure_forward",
-            "lbm355_closure_train_step", "lbm355_closure_adam")
+            "lbm355_closure_train_step", "lbm355_closure_adam", "lbm355_boundary_links")
 
 _lib = None
 
@@ -95,6 +106,9 @@ def load(path=None):
     lib.lbm355_closure_adam.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_float, ctypes.c_float,
                                         ctypes.c_float, ctypes.c_float, c_int32, c_void_p, c_void_p]
     lib.lbm355_closure_adam.restype = c_int32
+    lib.lbm355_boundary_links.argtypes = [ctypes.POINTER(BoundaryDesc), c_void_p, c_void_p, c_int64, c_void_p,
+                                          c_void_p]
+    lib.lbm355_boundary_links.restype = c_int32
     lib.lbm355_desc_size.argtypes = []
     lib.lbm355_desc_size.restype = c_int64
     if lib.lbm355_desc_size() != ctypes.sizeof(SweepDesc):

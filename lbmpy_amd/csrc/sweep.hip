@@ -11,6 +11,7 @@
 // (boundaries/boundaryconditions.py:104-139): a pull read from a wall cell returns
 // src[inv i][x]; AA odd reads src[i][x] and redirects the write to f[inv i][x].
 #include <string>
+#include <type_traits>
 #include <cstring>
 #include <cstdio>
 
@@ -179,6 +180,104 @@ __global__ void noslip_links_kernel(T* pdf, int64_t sy, int64_t sz, int64_t sq, 
     pdf[d * sq + nb] = pdf[id * sq + cell];
   } else {
     pdf[id * sq + cell] = pdf[d * sq + nb];
+  }
+}
+
+// ------------------------------------------------------------------ general boundary links
+struct BoundaryArgs {
+  int64_t sy, sz, sq;
+  int n[3], g, periodic[3];
+  int zc, comp;
+  int normal[3], mirror_axis;
+  double velocity[3];
+  double density;
+};
+
+__device__ __forceinline__ int wrap_axis(int c, int n, int g, int periodic) {
+  if (!periodic) return c;
+  int k = c - g;
+  k = k < 0 ? k + n : (k >= n ? k - n : k);
+  return k + g;
+}
+
+// f_out(i) of the cell at (cx, cy, cz) after kernel PAT, and f_in(i) of the kernel that follows
+template <class S, int PAT>
+__device__ __forceinline__ int64_t bc_out(const BoundaryArgs& b, int i, int cx, int cy, int cz) {
+  int slot = i;
+  if (PAT == LBM355_AA_ODD) {
+    cx += S::c[i][0];
+    cy += S::c[i][1];
+    cz += S::c[i][2];
+  } else if (PAT == LBM355_AA_EVEN) {
+    slot = S::inv[i];
+  }
+  return slot * b.sq + wrap_axis(cx, b.n[0], b.g, b.periodic[0]) + wrap_axis(cy, b.n[1], b.g, b.periodic[1]) * b.sy +
+         (S::D == 3 ? wrap_axis(cz, b.n[2], b.g, b.periodic[2]) * b.sz : 0);
+}
+template <class S, int PAT>
+__device__ __forceinline__ int64_t bc_in(const BoundaryArgs& b, int i, int cx, int cy, int cz) {
+  int slot = i;
+  if (PAT != LBM355_AA_ODD) {
+    cx -= S::c[i][0];
+    cy -= S::c[i][1];
+    cz -= S::c[i][2];
+    if (PAT == LBM355_AA_EVEN) slot = S::inv[i];
+  }
+  return slot * b.sq + wrap_axis(cx, b.n[0], b.g, b.periodic[0]) + wrap_axis(cy, b.n[1], b.g, b.periodic[1]) * b.sy +
+         (S::D == 3 ? wrap_axis(cz, b.n[2], b.g, b.periodic[2]) * b.sz : 0);
+}
+
+template <class S, int PAT, class T, int KIND>
+__global__ void boundary_links_kernel(T* pdf, const int32_t* links, int64_t n, const double* link_vel, BoundaryArgs b) {
+  using TC = typename std::conditional<sizeof(T) == 8, double, float>::type;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int32_t* l = links + 4 * t;
+  const int x = l[0], y = l[1], z = l[2], d = l[3];
+  if (d < 0 || d >= S::Q) return;
+  int id = 0, cdx = 0, cdy = 0, cdz = 0;
+#pragma unroll
+  for (int i = 0; i < S::Q; ++i)
+    if (i == d) {
+      id = S::inv[i];
+      cdx = S::c[i][0];
+      cdy = S::c[i][1];
+      cdz = S::c[i][2];
+    }
+  const int64_t target = bc_in<S, PAT>(b, id, x, y, z);
+  if (KIND == LBM355_BC_NOSLIP) {
+    pdf[target] = pdf[bc_out<S, PAT>(b, d, x, y, z)];
+  } else if (KIND == LBM355_BC_UBB) {
+    TC u[3] = {(TC)b.velocity[0], (TC)b.velocity[1], (TC)b.velocity[2]};
+    if (link_vel) {
+      u[0] = (TC)link_vel[3 * t];
+      u[1] = (TC)link_vel[3 * t + 1];
+      u[2] = (TC)link_vel[3 * t + 2];
+    }
+    TC rho = TC(1);
+    if (b.density > 0) {
+      rho = (TC)b.density;
+    } else if (b.comp) {
+      TC s = 0;
+#pragma unroll
+      for (int i = 0; i < S::Q; ++i) s += (TC)pdf[bc_out<S, PAT>(b, i, x, y, z)];
+      rho = b.zc ? TC(1) + s : s;
+    }
+    const TC cu = (TC)cdx * u[0] + (TC)cdy * u[1] + (TC)cdz * u[2];
+    TC w = 0;
+#pragma unroll
+    for (int i = 0; i < S::Q; ++i)
+      if (i == d) w = (TC)weight<S>(i);
+    pdf[target] = (T)((TC)pdf[bc_out<S, PAT>(b, d, x, y, z)] - TC(6) * w * cu * rho);
+  } else {
+    // specular reflection at the half-way plane: mirror d along the wall normal, take the inverse
+    int mc[3] = {cdx, cdy, cdz};
+    mc[b.mirror_axis] = -mc[b.mirror_axis];
+    int m = 0;
+#pragma unroll
+    for (int i = 0; i < S::Q; ++i)
+      if (S::c[i][0] == -mc[0] && S::c[i][1] == -mc[1] && S::c[i][2] == -mc[2]) m = i;
+    pdf[target] = pdf[bc_out<S, PAT>(b, m, x + cdx + b.normal[0], y + cdy + b.normal[1], z + cdz + b.normal[2])];
   }
 }
 
@@ -373,6 +472,10 @@ int validate(const lbm355_sweep_desc* d) {
 
 template <class TC>
 int launch_sweep(const lbm355_sweep_desc* d, const void* src, void* dst, void* rho, void* u, hipStream_t s) {
+  if (d->store_dtype == LBM355_F16) {
+    const int rc = lbm355_launch_sweep_h2(d, src, dst, rho, u, s);
+    if (rc != -1) return rc;
+  }
   bool walls = false;
   for (int a = 0; a < stencil_d(d->stencil); ++a) walls |= d->face[a] == LBM355_FACE_NOSLIP;
   size_t lds = 0;
@@ -412,6 +515,40 @@ int links_dtype(int dtype, int pattern, void* pdf, int64_t sy, int64_t sz, int64
   if (dtype == LBM355_F32) return launch_links<S, float>(pattern, pdf, sy, sz, sq, links, n, s);
   if (dtype == LBM355_F16) return launch_links<S, _Float16>(pattern, pdf, sy, sz, sq, links, n, s);
   return fail(LBM355_ERR_ARG, "bad dtype");
+}
+
+template <class S, int PAT, class T>
+int launch_bc_kind(const lbm355_boundary_desc* d, T* pdf, const int32_t* links, int64_t n, const double* vel,
+                   const BoundaryArgs& b, hipStream_t s) {
+  const int grid = (int)((n + 255) / 256);
+  if (d->kind == LBM355_BC_NOSLIP)
+    hipLaunchKernelGGL((boundary_links_kernel<S, PAT, T, LBM355_BC_NOSLIP>), dim3(grid), dim3(256), 0, s, pdf, links, n,
+                       vel, b);
+  else if (d->kind == LBM355_BC_UBB)
+    hipLaunchKernelGGL((boundary_links_kernel<S, PAT, T, LBM355_BC_UBB>), dim3(grid), dim3(256), 0, s, pdf, links, n, vel,
+                       b);
+  else
+    hipLaunchKernelGGL((boundary_links_kernel<S, PAT, T, LBM355_BC_FREESLIP>), dim3(grid), dim3(256), 0, s, pdf, links,
+                       n, vel, b);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(LBM355_ERR_LAUNCH, std::string("launch failed: ") + hipGetErrorString(e));
+  return LBM355_OK;
+}
+
+template <class S, class T>
+int launch_bc(const lbm355_boundary_desc* d, void* pdf, const int32_t* links, int64_t n, const double* vel,
+              const BoundaryArgs& b, hipStream_t s) {
+  if (d->pattern == LBM355_PULL) return launch_bc_kind<S, LBM355_PULL, T>(d, (T*)pdf, links, n, vel, b, s);
+  if (d->pattern == LBM355_AA_EVEN) return launch_bc_kind<S, LBM355_AA_EVEN, T>(d, (T*)pdf, links, n, vel, b, s);
+  return launch_bc_kind<S, LBM355_AA_ODD, T>(d, (T*)pdf, links, n, vel, b, s);
+}
+
+template <class S>
+int launch_bc_dtype(const lbm355_boundary_desc* d, void* pdf, const int32_t* links, int64_t n, const double* vel,
+                    const BoundaryArgs& b, hipStream_t s) {
+  if (d->store_dtype == LBM355_F64) return launch_bc<S, double>(d, pdf, links, n, vel, b, s);
+  if (d->store_dtype == LBM355_F32) return launch_bc<S, float>(d, pdf, links, n, vel, b, s);
+  return launch_bc<S, _Float16>(d, pdf, links, n, vel, b, s);
 }
 
 template <class S, class TS>
@@ -485,6 +622,50 @@ int lbm355_noslip_links(int32_t stencil, int32_t pattern, int32_t dtype, void* p
   if (stencil == LBM355_D3Q27)
     return links_dtype<D3Q27>(dtype, pattern, pdf, stride_y, stride_z, stride_q, links, n_links, s);
   return fail(LBM355_ERR_ARG, "unknown stencil");
+}
+
+int lbm355_boundary_links(const lbm355_boundary_desc* d, void* pdf, const int32_t* links, int64_t n_links,
+                          const double* link_velocity, void* stream) {
+  if (!d) return fail(LBM355_ERR_ARG, "desc is NULL");
+  if (n_links == 0) return LBM355_OK;
+  if (!pdf || !links || n_links < 0) return fail(LBM355_ERR_ARG, "bad pointer or link count");
+  if (d->pattern < 0 || d->pattern > 2) return fail(LBM355_ERR_ARG, "bad pattern");
+  if (d->store_dtype < 0 || d->store_dtype > 2) return fail(LBM355_ERR_ARG, "bad dtype");
+  if (d->kind < LBM355_BC_NOSLIP || d->kind > LBM355_BC_FREESLIP) return fail(LBM355_ERR_ARG, "bad boundary kind");
+  const int D = stencil_d(d->stencil);
+  if (!stencil_q(d->stencil)) return fail(LBM355_ERR_ARG, "unknown stencil");
+  if (d->nx <= 0 || d->ny <= 0 || d->nz <= 0 || d->ghost_layers < 0) return fail(LBM355_ERR_ARG, "bad sizes");
+  BoundaryArgs b;
+  b.sy = d->stride_y;
+  b.sz = d->stride_z;
+  b.sq = d->stride_q;
+  b.n[0] = d->nx;
+  b.n[1] = d->ny;
+  b.n[2] = d->nz;
+  b.g = d->ghost_layers;
+  for (int a = 0; a < 3; ++a) {
+    b.periodic[a] = (a < D) ? (d->periodic[a] != 0) : 0;
+    b.normal[a] = (a < D) ? d->normal[a] : 0;
+    b.velocity[a] = d->velocity[a];
+  }
+  b.zc = (d->flags & LBM355_FLAG_ZERO_CENTERED) ? 1 : 0;
+  b.comp = (d->flags & LBM355_FLAG_COMPRESSIBLE) ? 1 : 0;
+  b.density = d->density;
+  b.mirror_axis = 0;
+  if (d->kind == LBM355_BC_FREESLIP) {
+    int nz = 0;
+    for (int a = 0; a < D; ++a)
+      if (b.normal[a] != 0) {
+        if (b.normal[a] != 1 && b.normal[a] != -1) return fail(LBM355_ERR_ARG, "FreeSlip normal entries must be -1, 0, 1");
+        b.mirror_axis = a;
+        ++nz;
+      }
+    if (nz != 1) return fail(LBM355_ERR_ARG, "FreeSlip needs a normal along one axis");
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (d->stencil == LBM355_D2Q9) return launch_bc_dtype<D2Q9>(d, pdf, links, n_links, link_velocity, b, s);
+  if (d->stencil == LBM355_D3Q19) return launch_bc_dtype<D3Q19>(d, pdf, links, n_links, link_velocity, b, s);
+  return launch_bc_dtype<D3Q27>(d, pdf, links, n_links, link_velocity, b, s);
 }
 
 int lbm355_reduce_conserved(const lbm355_sweep_desc* d, const void* pdf, double* out, void* stream) {

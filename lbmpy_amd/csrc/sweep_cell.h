@@ -217,3 +217,7 @@ inline SweepArgs<TC> make_args(const lbm355_sweep_desc* d, const void* src, void
 
 // checks a sweep descriptor (sizes, strides, dtypes, table pointer); defined in sweep.hip
 int lbm355_validate_desc(const lbm355_sweep_desc* d);
+
+// row-staged FP16 sweep (sweep_h2.hip): launches and returns a status when it applies, else returns -1
+int lbm355_launch_sweep_h2(const lbm355_sweep_desc* d, const void* src, void* dst, void* rho, void* u,
+                           hipStream_t s);

@@ -23,7 +23,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
-from .boundaries import LatticeBoltzmannBoundaryHandling
+from .boundaries import LatticeBoltzmannBoundaryHandling, UBB, FreeSlip
 from .creationfunctions import (update_with_default_parameters, create_lb_function, Timestep, CreateKernelConfig)
 from .enums import Stencil, Target
 from .kernel_tables import method_table
@@ -215,24 +215,53 @@ class LatticeBoltzmannStep:
         self._aa_next_odd = True
         torch.cuda.synchronize()
 
-    def _links(self):
-        links = self._boundary_handling.links
-        if self._links_version is not links:
-            self._links_version = links
-            self._links_dev = torch.from_numpy(np.ascontiguousarray(links)).cuda() if len(links) else None
+    def _link_groups(self):
+        """Device copies of each boundary object's links (rebuilt when the flags change)."""
+        groups = self._boundary_handling.link_groups
+        key = self._boundary_handling.version
+        if self._links_version != key:
+            self._links_version = key
+            self._links_dev = [(obj, torch.from_numpy(np.ascontiguousarray(l)).cuda(),
+                                None if v is None else torch.from_numpy(np.ascontiguousarray(v)).cuda())
+                               for obj, l, v in groups]
         return self._links_dev
 
     def _apply_links(self, pattern):
-        links = self._links()
-        if links is None:
+        groups = self._link_groups()
+        if not groups:
             return
         a = self._src
+        D = self.dim
+        m = self.method
         lib = nat.load()
-        nat.check(lib.lbm355_noslip_links(nat.STENCIL_NAMES[self.method.stencil.name], pattern,
-                                          nat.DTYPE_CODES[self._config.data_type], ctypes.c_void_p(a.data_ptr()),
-                                          a.stride(1), a.stride(2) if self.dim == 3 else 0, a.stride(self.dim),
-                                          ctypes.c_void_p(links.data_ptr()), links.shape[0],
-                                          nat.current_stream_handle()))
+        for obj, links, vel in groups:
+            d = nat.BoundaryDesc()
+            d.stencil = nat.STENCIL_NAMES[m.stencil.name]
+            d.pattern = pattern
+            d.store_dtype = nat.DTYPE_CODES[self._config.data_type]
+            d.kind = obj.kind
+            sizes = list(self._domain_size) + [1] * (3 - D)
+            d.nx, d.ny, d.nz = sizes
+            d.ghost_layers = 1
+            for ax in range(3):
+                d.periodic[ax] = int(ax < D and bool(self.periodicity[ax]))
+            d.flags = (nat.FLAG_ZERO_CENTERED if m.zero_centered_pdfs else 0) | \
+                (nat.FLAG_COMPRESSIBLE if m.compressible else 0)
+            d.stride_y = a.stride(1)
+            d.stride_z = a.stride(2) if D == 3 else 0
+            d.stride_q = a.stride(D)
+            if isinstance(obj, UBB):
+                if not obj.velocity_is_callable:
+                    for ax, v in enumerate(obj._velocity):
+                        d.velocity[ax] = float(v)
+                d.density = float(obj._density) if obj._density is not None else 0.0
+            elif isinstance(obj, FreeSlip):
+                for ax, v in enumerate(obj.normal_direction):
+                    d.normal[ax] = int(v)
+            nat.check(lib.lbm355_boundary_links(ctypes.byref(d), ctypes.c_void_p(a.data_ptr()),
+                                                ctypes.c_void_p(links.data_ptr()), links.shape[0],
+                                                ctypes.c_void_p(vel.data_ptr() if vel is not None else 0),
+                                                nat.current_stream_handle()))
 
     # ------------------------------------------------------------------ stepping
     def pre_run(self):

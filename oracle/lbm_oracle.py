@@ -416,9 +416,13 @@ class OracleDomain:
     Arrays are numpy float64 with shape (Q, nz+2, ny+2, nx+2) for 3-D and
     (Q, ny+2, nx+2) for 2-D (the ``fzyx`` layout)."""
 
-    def __init__(self, method, shape, faces=None, pattern="pull", bindings=None, obstacle=None):
+    def __init__(self, method, shape, faces=None, pattern="pull", bindings=None, obstacle=None, boundaries=None):
         """``obstacle``: optional bool array [x, y, (z)] over the interior marking NoSlip cells
-        (what ``set_boundary(NoSlip(), mask_callback=...)`` flags)."""
+        (what ``set_boundary(NoSlip(), mask_callback=...)`` flags).  ``boundaries``: optional list of
+        (flags, spec) with flags a bool array [x, y, (z)] over the ghost-inclusive domain and spec
+        {"kind": "noslip"}, {"kind": "ubb", "velocity": u_w, "density": rho_w or None} or
+        {"kind": "freeslip", "normal": n} (n points from the wall into the fluid)."""
+        self.boundaries = list(boundaries or [])
         self.coll = OracleCollision(method, bindings)
         self.obstacle = None if obstacle is None else np.asarray(obstacle, dtype=bool)
         self.method = method
@@ -457,14 +461,17 @@ class OracleDomain:
                 arr[self._sl(a, -1)] = arr[self._sl(a, 1)]
 
     def wall_links(self):
-        """Index list (x, y[, z], dir) in ghost coordinates of fluid cells whose neighbour in
-        direction dir is a boundary cell: a NoSlip ghost plane or an obstacle cell (what
-        pystencils' index-list creation yields for NoSlip flags, boundaries/boundaryhandling.py)."""
+        """Index list (x, y[, z], dir, kind) in ghost coordinates of fluid cells whose neighbour in
+        direction dir is a boundary cell: a NoSlip ghost plane or obstacle cell (kind -1) or a cell of
+        ``boundaries[kind]`` (what pystencils' index-list creation yields, boundaries/boundaryhandling.py).
+        Neighbours are addressed unwrapped: a boundary plane flags the whole ghost row, including the
+        ghost columns of periodic axes, as pystencils' flag field does."""
         if hasattr(self, "_links"):
             return self._links
         n = self.shape
         D = self.D
-        flags = np.zeros(tuple(k + 2 for k in n), dtype=bool)          # [x, y, (z)] ghost-inclusive
+        gshape = tuple(k + 2 for k in n)
+        flags = np.zeros(gshape, dtype=bool)          # [x, y, (z)] ghost-inclusive, NoSlip
         for a in range(D):
             if self.faces.get(a) == "noslip":
                 lo = [slice(None)] * D
@@ -474,34 +481,107 @@ class OracleDomain:
                 flags[tuple(hi)] = True
         if self.obstacle is not None:
             flags[(slice(1, -1),) * D] |= self.obstacle
-        self.flags = flags
+        kind = np.full(gshape, -2, dtype=int)         # -2 fluid, -1 NoSlip, k >= 0 boundaries[k]
+        for k in range(len(self.boundaries) - 1, -1, -1):
+            kind[self.boundaries[k][0]] = k
+        kind[flags] = -1
+        # the ghost layers of periodic axes hold the periodic images of the interior cells' flags
+        for a in range(D):
+            if self.faces.get(a) == "periodic":
+                lo = [slice(None)] * D
+                hi = [slice(None)] * D
+                lo[a], hi[a] = 0, -1
+                src_lo = [slice(None)] * D
+                src_hi = [slice(None)] * D
+                src_lo[a], src_hi[a] = -2, 1
+                kind[tuple(lo)] = kind[tuple(src_lo)]
+                kind[tuple(hi)] = kind[tuple(src_hi)]
+        self.flags = kind > -2
         grids = np.meshgrid(*[np.arange(1, n[a] + 1) for a in range(D)], indexing="ij")
         cells = np.stack([g.reshape(-1) for g in grids], axis=1)            # (ncells, D)
-        cells = cells[~flags[tuple(cells[:, a] for a in range(D))]]         # fluid cells only
+        cells = cells[~self.flags[tuple(cells[:, a] for a in range(D))]]    # fluid cells only
         rows = []
         for d in range(self.Q):
-            # neighbours are addressed unwrapped: NoSlip planes flag the whole ghost row,
-            # including the ghost columns of periodic axes, as pystencils' flag field does
             nb = cells + self.c[d][None, :]
-            hit = flags[tuple(nb[:, a] for a in range(D))]
+            kk = kind[tuple(nb[:, a] for a in range(D))]
+            hit = kk > -2
             sel = cells[hit]
-            rows.append(np.concatenate([sel, np.full((len(sel), 1), d)], axis=1))
-        self._links = np.concatenate(rows) if rows else np.zeros((0, D + 1), dtype=int)
+            rows.append(np.concatenate([sel, np.full((len(sel), 1), d), kk[hit][:, None]], axis=1))
+        self._links = np.concatenate(rows) if rows else np.zeros((0, D + 2), dtype=int)
         return self._links
 
     def _link_neighbours(self, links):
         d = links[:, self.D]
         return [links[:, a] + self.c[d, a] for a in range(self.D)]
 
-    def noslip_pull(self, arr):
-        """f_in(inv(d)) at the wall neighbour := f_out(d) at the fluid cell (boundaryconditions.py:138)."""
+    def _out_addr(self, prev, i, cell):
+        """f_out(i) at `cell` after a `prev` kernel (its write accessor, indexing.py:41-46)."""
+        if prev == "odd":
+            coords = [cell[a] + self.c[i, a] for a in range(self.D)]
+            slot = i
+        else:
+            coords = list(cell)
+            slot = self.inv[i] if prev == "even" else i
+        return (slot,) + tuple(reversed(coords))
+
+    def _in_addr(self, prev, i, cell):
+        """f_in(i) at `cell`: the read accessor of the kernel that follows `prev`."""
+        if prev == "odd":
+            return (i,) + tuple(reversed(list(cell)))
+        coords = [cell[a] - self.c[i, a] for a in range(self.D)]
+        slot = self.inv[i] if prev == "even" else i
+        return (slot,) + tuple(reversed(coords))
+
+    def apply_links(self, arr, prev):
+        """Boundary links between time steps; `prev` is the kernel that ran last ("pull", "even", "odd"):
+        NoSlip  f_in(inv d) = f_out(d)                                  (boundaryconditions.py:104-139)
+        UBB     f_in(inv d) = f_out(d) - 6 w_d rho (c_d . u_w)          (boundaryconditions.py:724-842)
+                rho = sum_i f_out(i) (compressible; + 1 zero-centred), the given wall density, or 1
+        FreeSlip f_in(inv d) = f_out(inv(mirror(d))) at x + c_d + n   (boundaryconditions.py:360-487)"""
         links = self.wall_links()
         if len(links) == 0:
             return
-        d = links[:, self.D]
-        cell = [links[:, a] for a in range(self.D)]
-        nb = self._link_neighbours(links)
-        arr[(self.inv[d],) + tuple(reversed(nb))] = arr[(d,) + tuple(reversed(cell))]
+        D = self.D
+        for k in sorted(set(links[:, D + 1].tolist())):
+            sel = links[links[:, D + 1] == k]
+            d = sel[:, D]
+            cell = [sel[:, a] for a in range(D)]
+            spec = {"kind": "noslip"} if k < 0 else self.boundaries[k][1]
+            target = self._in_addr(prev, self.inv[d], cell)
+            if spec["kind"] == "noslip":
+                arr[target] = arr[self._out_addr(prev, d, cell)]
+            elif spec["kind"] == "ubb":
+                uw = np.asarray(spec["velocity"], dtype=np.float64)
+                cu = self.c[d, :D] @ uw[:D]
+                w = self.coll.w[d]
+                if spec.get("density") is not None:
+                    rho = float(spec["density"])
+                elif self.method.compressible:
+                    rho = sum(arr[self._out_addr(prev, np.full_like(d, i), cell)] for i in range(self.Q))
+                    rho = rho + (1.0 if self.method.zero_centered_pdfs else 0.0)
+                else:
+                    rho = 1.0
+                arr[target] = arr[self._out_addr(prev, d, cell)] - 6.0 * w * cu * rho
+            elif spec["kind"] == "freeslip":
+                nrm = np.asarray(spec["normal"], dtype=int)
+                ax = int(np.nonzero(nrm)[0][0])
+                mirrored = self.c[d].copy()
+                mirrored[:, ax] *= -1
+                mi = np.array([self._dir_index(tuple(v)) for v in mirrored])
+                src_cell = [cell[a] + self.c[d, a] + nrm[a] for a in range(D)]
+                arr[target] = arr[self._out_addr(prev, self.inv[mi], src_cell)]
+            else:
+                raise NotImplementedError(spec["kind"])
+
+    def _dir_index(self, v):
+        for i in range(self.Q):
+            if tuple(self.c[i, :self.D]) == tuple(v[:self.D]):
+                return i
+        raise KeyError(v)
+
+    def noslip_pull(self, arr):
+        """Boundary links before a pull step (the previous kernel was a pull sweep)."""
+        self.apply_links(arr, "pull")
 
     def _neighbor_view(self, arr, q, off):
         """arr[q] at (x + off) over the interior."""
@@ -528,8 +608,9 @@ class OracleDomain:
             self.interior(self.dst)[...] = fpost
             self.src, self.dst = self.dst, self.src
         else:
-            if any(self.faces.get(a) == "ghost" for a in range(self.D)):
-                raise NotImplementedError("the AA oracle covers periodic and NoSlip faces")
+            if any(self.faces.get(a) == "ghost" for a in range(self.D)) and not self.boundaries:
+                # populations the ODD step writes into a ghost plane come back only through boundary links
+                raise NotImplementedError("the AA oracle needs periodic, NoSlip or boundary-covered faces")
             if self.aa_parity == 0:
                 # periodicity(EVEN), boundary(prev=EVEN), then the ODD kernel:
                 # read f[x - c_i](inv i), write f[x + c_i](i)
@@ -551,20 +632,11 @@ class OracleDomain:
             self.aa_parity ^= 1
 
     def noslip_aa(self, arr, prev_even):
-        """NoSlip between AA time steps (``advanced_streaming/indexing.py:12-181``): after an EVEN
-        step the population leaving x in direction d sits in slot inv(d) of x and is placed where
-        the ODD step reads it (slot d of the wall cell); after an ODD step the value written into
-        the wall cell (slot d) is returned to slot inv(d) of x, where the EVEN step reads it."""
-        links = self.wall_links()
-        if len(links) == 0:
-            return
-        d = links[:, self.D]
-        cell = tuple(reversed([links[:, a] for a in range(self.D)]))
-        nb = tuple(reversed(self._link_neighbours(links)))
-        if prev_even:
-            arr[(d,) + nb] = arr[(self.inv[d],) + cell]
-        else:
-            arr[(self.inv[d],) + cell] = arr[(d,) + nb]
+        """Boundary links between AA time steps (``advanced_streaming/indexing.py:12-181``): after an
+        EVEN step the population leaving x in direction d sits in slot inv(d) of x and is placed where
+        the ODD step reads it (slot d of the wall cell); after an ODD step the value written into the
+        wall cell (slot d) is returned to slot inv(d) of x, where the EVEN step reads it."""
+        self.apply_links(arr, "even" if prev_even else "odd")
 
     def _fold_ghost_writes(self):
         """Values the ODD kernel wrote into ghost cells belong to the periodic images
