@@ -147,6 +147,7 @@ int lbm355_stream_copy(const void* src, void* dst, int64_t n_floats, int32_t non
 #define LBM355_BC_NOSLIP 0   /* f_in(inv d) = f_out(d)                   boundaryconditions.py:104-139 */
 #define LBM355_BC_UBB 1      /* f_in(inv d) = f_out(d) - 6 w_d rho c_d.u  boundaryconditions.py:724-842 */
 #define LBM355_BC_FREESLIP 2 /* f_in(inv d) = f_out(inv mirror(d)) at x + c_d + n, boundaryconditions.py:360-487 */
+#define LBM355_BC_FIXED_DENSITY 3 /* f_in(inv d) = 2 f_eq,sym(d; rho_w, u) - f_out(d), boundaryconditions.py:1044-1102 */
 
 typedef struct lbm355_boundary_desc {
   int32_t stencil, pattern, store_dtype, kind; /* pattern: the kernel that ran last (PULL, AA_EVEN, AA_ODD) */
@@ -155,10 +156,18 @@ typedef struct lbm355_boundary_desc {
   int32_t flags;                               /* LBM355_FLAG_ZERO_CENTERED | LBM355_FLAG_COMPRESSIBLE (UBB) */
   int32_t normal[3];                           /* FreeSlip: wall normal into the fluid, one non-zero entry */
   int32_t pad;
+  int32_t fused_noslip[3];                     /* 1: this axis's faces are NoSlip walls fused into the sweep
+                                                  (LBM355_FACE_NOSLIP); after an AA odd step a population
+                                                  bound into such a wall is read back from the cell itself */
+  int32_t pad2;
   int64_t stride_y, stride_z, stride_q;
   double velocity[3];                          /* UBB wall velocity (unless per-link velocities are given) */
   double density;                              /* UBB wall density if > 0; else from the fluid cell
-                                                  (compressible) or 1 (incompressible) */
+                                                  (compressible) or 1 (incompressible).  FixedDensity: rho_w */
+  double force[3];                             /* FixedDensity: body force (with LBM355_FLAG_EQ_SHIFT) */
+  const double* equilibrium;                   /* FixedDensity: device table [Q][8] of the equilibrium in
+                                                  storage form, f_eq_i = A + B drho + lam (L.u + Qd.u^2 + Qo.uu):
+                                                  A, B, Qd[3], Qo[3] (xy, xz, yz) per direction */
 } lbm355_boundary_desc;
 
 /* Applies one boundary condition on n_links links; link_velocity: NULL or n_links x 3 doubles (UBB). */

@@ -26,15 +26,17 @@ DTYPE_CODES = {"float64": F64, "float32": F32, "float16": F16}
 STENCIL_NAMES = {"D2Q9": D2Q9, "D3Q19": D3Q19, "D3Q27": D3Q27}
 
 
-BC_NOSLIP, BC_UBB, BC_FREESLIP = 0, 1, 2
+BC_NOSLIP, BC_UBB, BC_FREESLIP, BC_FIXED_DENSITY = 0, 1, 2, 3
 
 
 class BoundaryDesc(ctypes.Structure):
     _fields_ = [("stencil", c_int32), ("pattern", c_int32), ("store_dtype", c_int32), ("kind", c_int32),
                 ("nx", c_int32), ("ny", c_int32), ("nz", c_int32), ("ghost_layers", c_int32),
                 ("periodic", c_int32 * 3), ("flags", c_int32), ("normal", c_int32 * 3), ("pad", c_int32),
+                ("fused_noslip", c_int32 * 3), ("pad2", c_int32),
                 ("stride_y", c_int64), ("stride_z", c_int64), ("stride_q", c_int64),
-                ("velocity", c_double * 3), ("density", c_double)]
+                ("velocity", c_double * 3), ("density", c_double), ("force", c_double * 3),
+                ("equilibrium", c_void_p)]
 
 
 class SweepDesc(ctypes.Structure):

@@ -1,7 +1,7 @@
-"""Boundaries for the GPU step driver: NoSlip, UBB and FreeSlip.
+"""Boundaries for the GPU step driver: NoSlip, UBB, FreeSlip and FixedDensity.
 
 Mirrors lbmpy ``boundaries/boundaryconditions.py`` (``NoSlip`` :104-139, ``UBB`` :724-842,
-``FreeSlip`` :360-487) and the index-list machinery of ``boundaries/boundaryhandling.py:15-199``:
+``FreeSlip`` :360-487, ``FixedDensity`` :1044-1102) and the index-list machinery of ``boundaries/boundaryhandling.py:15-199``:
 boundary cells are flagged on a (ghost-layer inclusive) flag field with the id of their boundary
 object; every link from a fluid cell x into a boundary cell in direction d becomes an index-list
 entry (x, y, z, d) of that object, executed by ``lbm355_boundary_links`` between sweeps.  Links
@@ -81,6 +81,19 @@ class FreeSlip(LbBoundary):
         super().__init__(name)
 
 
+class FixedDensity(LbBoundary):
+    """Prescribed density (pressure) at the boundary: f_inv(d)(x_b) = 2 f_eq,sym(d; rho_w, u) - f*_d(x), with
+    f_eq,sym the part of the method's equilibrium even in u and u the fluid cell's velocity computed with
+    rho_w (boundaryconditions.py:1044-1102).  Moment-family methods on the GPU path."""
+    kind = nat.BC_FIXED_DENSITY
+
+    def __init__(self, density, name=None):
+        if name is None:
+            name = "Fixed Density " + str(density)
+        self.density = float(density)
+        super().__init__(name)
+
+
 def slice_from_direction(direction_name, dim, normal_offset=0, tangential_offset=0):
     """Slice of the ghost layer in a compass direction ('N', 'S', 'E', 'W', 'T', 'B'),
     as pystencils' ``slice_from_direction`` returns it for ghost-layer inclusive arrays."""
@@ -122,9 +135,9 @@ class LatticeBoltzmannBoundaryHandling:
 
     def set_boundary(self, boundary_obj, slice_obj=None, mask_callback=None, ghost_layers=True,
                      inner_ghost_layers=True, replace=True, force_flag_value=None):
-        if not isinstance(boundary_obj, (NoSlip, UBB, FreeSlip)):
+        if not isinstance(boundary_obj, (NoSlip, UBB, FreeSlip, FixedDensity)):
             raise NotImplementedError(f"boundary {boundary_obj!r} is not available on the GPU path "
-                                      "(NoSlip, UBB and FreeSlip are)")
+                                      "(NoSlip, UBB, FreeSlip and FixedDensity are)")
         if getattr(boundary_obj, "dim", self.dim) != self.dim:
             raise ValueError(f"boundary dimension {boundary_obj.dim} does not match the domain ({self.dim})")
         bid = self._id_of(boundary_obj)

@@ -186,11 +186,13 @@ __global__ void noslip_links_kernel(T* pdf, int64_t sy, int64_t sz, int64_t sq, 
 // ------------------------------------------------------------------ general boundary links
 struct BoundaryArgs {
   int64_t sy, sz, sq;
-  int n[3], g, periodic[3];
-  int zc, comp;
+  int n[3], g, periodic[3], fused[3];
+  int zc, comp, eq_shift;
   int normal[3], mirror_axis;
   double velocity[3];
   double density;
+  double force[3];
+  const double* eq;
 };
 
 __device__ __forceinline__ int wrap_axis(int c, int n, int g, int periodic) {
@@ -205,9 +207,18 @@ template <class S, int PAT>
 __device__ __forceinline__ int64_t bc_out(const BoundaryArgs& b, int i, int cx, int cy, int cz) {
   int slot = i;
   if (PAT == LBM355_AA_ODD) {
-    cx += S::c[i][0];
-    cy += S::c[i][1];
-    cz += S::c[i][2];
+    const int tx = cx + S::c[i][0], ty = cy + S::c[i][1], tz = cz + S::c[i][2];
+    // a population the odd sweep bounced at a fused NoSlip face already sits in the cell's slot inv(i)
+    const bool bounced = (b.fused[0] && (tx < b.g || tx >= b.g + b.n[0])) ||
+                         (b.fused[1] && (ty < b.g || ty >= b.g + b.n[1])) ||
+                         (S::D == 3 && b.fused[2] && (tz < b.g || tz >= b.g + b.n[2]));
+    if (bounced) {
+      slot = S::inv[i];
+    } else {
+      cx = tx;
+      cy = ty;
+      cz = tz;
+    }
   } else if (PAT == LBM355_AA_EVEN) {
     slot = S::inv[i];
   }
@@ -269,6 +280,27 @@ __global__ void boundary_links_kernel(T* pdf, const int32_t* links, int64_t n, c
     for (int i = 0; i < S::Q; ++i)
       if (i == d) w = (TC)weight<S>(i);
     pdf[target] = (T)((TC)pdf[bc_out<S, PAT>(b, d, x, y, z)] - TC(6) * w * cu * rho);
+  } else if (KIND == LBM355_BC_FIXED_DENSITY) {
+    // 2 * (symmetric part of the equilibrium at rho_w and the fluid cell's velocity) - f_out(d); the
+    // velocity comes from the fluid cell's populations with rho replaced by rho_w
+    TC j[3] = {0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < S::Q; ++i) {
+      const TC v = (TC)pdf[bc_out<S, PAT>(b, i, x, y, z)];
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+        if (S::c[i][a] != 0) j[a] += S::c[i][a] > 0 ? v : -v;
+    }
+    const TC rho_w = (TC)b.density;
+    const TC ref = b.comp ? rho_w : TC(1);
+    TC u[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) u[a] = j[a] / ref + (b.eq_shift ? (TC)b.force[a] / (TC(2) * ref) : TC(0));
+    const double* e = b.eq + 8 * d;
+    const TC quad = (TC)e[2] * u[0] * u[0] + (TC)e[3] * u[1] * u[1] + (TC)e[4] * u[2] * u[2] +
+                    (TC)e[5] * u[0] * u[1] + (TC)e[6] * u[0] * u[2] + (TC)e[7] * u[1] * u[2];
+    const TC eq_sym = (TC)e[0] + (TC)e[1] * (rho_w - TC(1)) + (b.comp ? rho_w : TC(1)) * quad;
+    pdf[target] = (T)(TC(2) * eq_sym - (TC)pdf[bc_out<S, PAT>(b, d, x, y, z)]);
   } else {
     // specular reflection at the half-way plane: mirror d along the wall normal, take the inverse
     int mc[3] = {cdx, cdy, cdz};
@@ -527,6 +559,9 @@ int launch_bc_kind(const lbm355_boundary_desc* d, T* pdf, const int32_t* links, 
   else if (d->kind == LBM355_BC_UBB)
     hipLaunchKernelGGL((boundary_links_kernel<S, PAT, T, LBM355_BC_UBB>), dim3(grid), dim3(256), 0, s, pdf, links, n, vel,
                        b);
+  else if (d->kind == LBM355_BC_FIXED_DENSITY)
+    hipLaunchKernelGGL((boundary_links_kernel<S, PAT, T, LBM355_BC_FIXED_DENSITY>), dim3(grid), dim3(256), 0, s, pdf,
+                       links, n, vel, b);
   else
     hipLaunchKernelGGL((boundary_links_kernel<S, PAT, T, LBM355_BC_FREESLIP>), dim3(grid), dim3(256), 0, s, pdf, links,
                        n, vel, b);
@@ -631,7 +666,9 @@ int lbm355_boundary_links(const lbm355_boundary_desc* d, void* pdf, const int32_
   if (!pdf || !links || n_links < 0) return fail(LBM355_ERR_ARG, "bad pointer or link count");
   if (d->pattern < 0 || d->pattern > 2) return fail(LBM355_ERR_ARG, "bad pattern");
   if (d->store_dtype < 0 || d->store_dtype > 2) return fail(LBM355_ERR_ARG, "bad dtype");
-  if (d->kind < LBM355_BC_NOSLIP || d->kind > LBM355_BC_FREESLIP) return fail(LBM355_ERR_ARG, "bad boundary kind");
+  if (d->kind < LBM355_BC_NOSLIP || d->kind > LBM355_BC_FIXED_DENSITY) return fail(LBM355_ERR_ARG, "bad boundary kind");
+  if (d->kind == LBM355_BC_FIXED_DENSITY && (!d->equilibrium || !(d->density > 0)))
+    return fail(LBM355_ERR_ARG, "FixedDensity needs a positive density and the equilibrium table");
   const int D = stencil_d(d->stencil);
   if (!stencil_q(d->stencil)) return fail(LBM355_ERR_ARG, "unknown stencil");
   if (d->nx <= 0 || d->ny <= 0 || d->nz <= 0 || d->ghost_layers < 0) return fail(LBM355_ERR_ARG, "bad sizes");
@@ -645,12 +682,16 @@ int lbm355_boundary_links(const lbm355_boundary_desc* d, void* pdf, const int32_
   b.g = d->ghost_layers;
   for (int a = 0; a < 3; ++a) {
     b.periodic[a] = (a < D) ? (d->periodic[a] != 0) : 0;
+    b.fused[a] = (a < D) ? (d->fused_noslip[a] != 0) : 0;
     b.normal[a] = (a < D) ? d->normal[a] : 0;
     b.velocity[a] = d->velocity[a];
   }
   b.zc = (d->flags & LBM355_FLAG_ZERO_CENTERED) ? 1 : 0;
   b.comp = (d->flags & LBM355_FLAG_COMPRESSIBLE) ? 1 : 0;
   b.density = d->density;
+  b.eq_shift = (d->flags & LBM355_FLAG_EQ_SHIFT) ? 1 : 0;
+  for (int a = 0; a < 3; ++a) b.force[a] = (a < D) ? d->force[a] : 0.0;
+  b.eq = d->equilibrium;
   b.mirror_axis = 0;
   if (d->kind == LBM355_BC_FREESLIP) {
     int nz = 0;

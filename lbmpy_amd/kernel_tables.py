@@ -314,6 +314,19 @@ def central_table(method, bindings, dtype=np.float64):
     return tab.astype(dtype), n
 
 
+def fixed_density_table(method):
+    """(Q, 8) FP64 table of the storage-form equilibrium per direction, A, B, Qd[3], Qo[3] (xy, xz, yz), for the
+    FixedDensity boundary (its symmetric part: A + B drho + lam (Qd.u^2 + Qo.uu)).  Moment family only."""
+    if family_of(method) != FAMILY_MOMENT:
+        raise UnsupportedMethod("FixedDensity on the GPU path needs a moment-family method (SRT, TRT, MRT)")
+    feq, _, _ = _storage_equilibrium(method)
+    tab = np.zeros((method.stencil.Q, 8))
+    for i, p in enumerate(feq):
+        A, B, _, qd, qo = _split_equilibrium(p, method.compressible)
+        tab[i] = [float(A), float(B)] + [float(v) for v in qd] + [float(v) for v in qo]
+    return tab
+
+
 def method_table(method, bindings, dtype=np.float64):
     fam = family_of(method)
     if fam == FAMILY_MOMENT:

@@ -23,10 +23,10 @@ import numpy as np
 import torch
 
 from . import _native as nat
-from .boundaries import LatticeBoltzmannBoundaryHandling, UBB, FreeSlip
+from .boundaries import LatticeBoltzmannBoundaryHandling, UBB, FreeSlip, FixedDensity
 from .creationfunctions import (update_with_default_parameters, create_lb_function, Timestep, CreateKernelConfig)
 from .enums import Stencil, Target
-from .kernel_tables import method_table
+from .kernel_tables import method_table, fixed_density_table
 from .stencils import LBStencil
 from .sweep import create_pdf_array, TORCH_DTYPES
 
@@ -243,8 +243,10 @@ class LatticeBoltzmannStep:
             sizes = list(self._domain_size) + [1] * (3 - D)
             d.nx, d.ny, d.nz = sizes
             d.ghost_layers = 1
+            faces = self._boundary_handling.faces
             for ax in range(3):
                 d.periodic[ax] = int(ax < D and bool(self.periodicity[ax]))
+                d.fused_noslip[ax] = int(ax < D and faces[ax] == "noslip")
             d.flags = (nat.FLAG_ZERO_CENTERED if m.zero_centered_pdfs else 0) | \
                 (nat.FLAG_COMPRESSIBLE if m.compressible else 0)
             d.stride_y = a.stride(1)
@@ -258,6 +260,17 @@ class LatticeBoltzmannStep:
             elif isinstance(obj, FreeSlip):
                 for ax, v in enumerate(obj.normal_direction):
                     d.normal[ax] = int(v)
+            elif isinstance(obj, FixedDensity):
+                if getattr(self, "_fd_table", None) is None:
+                    self._fd_table = torch.from_numpy(fixed_density_table(m)).cuda()
+                d.density = obj.density
+                d.equilibrium = self._fd_table.data_ptr()
+                fm = m.force_model
+                if fm is not None:
+                    if fm.equilibrium_velocity_shift:
+                        d.flags |= nat.FLAG_EQ_SHIFT
+                    for ax in range(D):
+                        d.force[ax] = float(fm.force[ax])
             nat.check(lib.lbm355_boundary_links(ctypes.byref(d), ctypes.c_void_p(a.data_ptr()),
                                                 ctypes.c_void_p(links.data_ptr()), links.shape[0],
                                                 ctypes.c_void_p(vel.data_ptr() if vel is not None else 0),

@@ -570,8 +570,34 @@ class OracleDomain:
                 mi = np.array([self._dir_index(tuple(v)) for v in mirrored])
                 src_cell = [cell[a] + self.c[d, a] + nrm[a] for a in range(D)]
                 arr[target] = arr[self._out_addr(prev, self.inv[mi], src_cell)]
+            elif spec["kind"] == "fixeddensity":
+                arr[target] = self._fixed_density_value(arr, prev, d, cell, float(spec["density"]))
             else:
                 raise NotImplementedError(spec["kind"])
+
+    def _fixed_density_value(self, arr, prev, d, cell, rho_w):
+        """2 f_eq,sym(d; rho_w, u) - f_out(d) (boundaryconditions.py:1044-1102): u from the fluid cell's
+        populations with the density replaced by rho_w; the symmetric part (f_eq(u) + f_eq(-u)) / 2 of the
+        method's equilibrium (moment family)."""
+        co = self.coll
+        if self.method.kind != "moment":
+            raise NotImplementedError("the oracle's FixedDensity covers the moment family")
+        fo = np.stack([arr[self._out_addr(prev, np.full_like(d, i), cell)] for i in range(self.Q)])
+        j = self.c[:, :self.D].T @ fo                                       # (D, n)
+        n = len(d)
+        rho = np.full(n, rho_w)
+        ref = rho if self.method.compressible else np.ones(n)
+        u = j / ref
+        if co.fm is not None and co.fm.equilibrium_velocity_shift:
+            u = u + co.F[:self.D, None] / (2.0 * ref)
+
+        def feq(uu):
+            meq = np.stack([_eval_eq_poly(p, rho, rho - 1.0, list(uu)) for p in co.meq_polys])
+            if co.bg is not None:
+                meq = meq - co.bg[:, None]
+            return co.Minv @ meq
+        sym = 0.5 * (feq(u) + feq(-u))
+        return 2.0 * sym[d, np.arange(n)] - fo[d, np.arange(n)]
 
     def _dir_index(self, v):
         for i in range(self.Q):

@@ -138,3 +138,35 @@ def test_freeslip_fp32_matches_fp64_closely():
         step.run(50)
         res.append(_natural(step))
     assert np.abs(res[0] - res[1]).max() <= 1e-6
+
+
+@pytest.mark.parametrize("pattern", ["pull", "aa"])
+@pytest.mark.parametrize("compressible", [False, True])
+def test_fixed_density_pressure_channel(pattern, compressible):
+    """FixedDensity inlet/outlet (rho 1.002 / 0.998) with NoSlip walls, D2Q9 TRT with a small body force (so the
+    equilibrium velocity shift is part of the boundary's velocity)."""
+    from lbmpy_amd.boundaries import FixedDensity
+    lc = LBMConfig(stencil=LBStencil(Stencil.D2Q9), method=Method.TRT, relaxation_rates=[1.6, 1.2],
+                   compressible=compressible, force=(1e-6, 0.0), streaming_pattern=pattern)
+    step = _make((20, 10), (False, False), lc)
+    step.boundary_handling.set_boundary(FixedDensity(1.002), slice_from_direction("W", 2))
+    step.boundary_handling.set_boundary(FixedDensity(0.998), slice_from_direction("E", 2))
+    for w in ("N", "S"):
+        step.boundary_handling.set_boundary(NoSlip(), slice_from_direction(w, 2))
+    print(f"fixed density {pattern} compressible={compressible}", flush=True)
+    step.run(60)
+    got = _natural(step)
+    bh = step.boundary_handling
+    specs = []
+    for k, o in enumerate(bh._objects):
+        specs.append((bh.flags == k + 1, {"kind": "fixeddensity", "density": o.density} if isinstance(o, FixedDensity)
+                      else _spec(o)))
+    m = step.method
+    dom = OracleDomain(m, (20, 10), faces={0: "ghost", 1: "ghost"}, pattern=pattern, boundaries=specs)
+    f0 = equilibrium_pdfs(m, np.ones((10, 20)), np.zeros((2, 10, 20)))
+    dom.set_pdfs_interior(f0[dom.inv] if pattern == "aa" else f0)
+    for _ in range(60):
+        dom.step()
+    ref = dom.pdfs_in_natural_layout()
+    assert np.abs(got - ref).max() <= 1e-12
+    assert np.abs(got - f0).max() > 1e-5
