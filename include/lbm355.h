@@ -148,6 +148,10 @@ int lbm355_stream_copy(const void* src, void* dst, int64_t n_floats, int32_t non
 #define LBM355_BC_UBB 1      /* f_in(inv d) = f_out(d) - 6 w_d rho c_d.u  boundaryconditions.py:724-842 */
 #define LBM355_BC_FREESLIP 2 /* f_in(inv d) = f_out(inv mirror(d)) at x + c_d + n, boundaryconditions.py:360-487 */
 #define LBM355_BC_FIXED_DENSITY 3 /* f_in(inv d) = 2 f_eq,sym(d; rho_w, u) - f_out(d), boundaryconditions.py:1044-1102 */
+#define LBM355_BC_OUTFLOW 4  /* extrapolation outflow with per-link state, boundaryconditions.py:894-1042:
+                                f_in(inv d) = c s0 + (1 - c) s1, c = 1/sqrt(3); then s0 = f_out(inv d) at
+                                x + c_d - n and s1 = the value just written */
+#define LBM355_BC_OUTFLOW_INIT 5 /* sets s0 = s1 = f_out(inv d) at x + c_d - n (before the first step) */
 
 typedef struct lbm355_boundary_desc {
   int32_t stencil, pattern, store_dtype, kind; /* pattern: the kernel that ran last (PULL, AA_EVEN, AA_ODD) */
@@ -170,9 +174,11 @@ typedef struct lbm355_boundary_desc {
                                                   A, B, Qd[3], Qo[3] (xy, xz, yz) per direction */
 } lbm355_boundary_desc;
 
-/* Applies one boundary condition on n_links links; link_velocity: NULL or n_links x 3 doubles (UBB). */
+/* Applies one boundary condition on n_links links.  link_data: UBB: NULL or n_links x 3 wall velocities;
+ * outflow: n_links x 2 doubles of per-link state (s0, s1), read and updated (normal in desc->normal,
+ * pointing out of the fluid). */
 int lbm355_boundary_links(const lbm355_boundary_desc* desc, void* pdf, const int32_t* links, int64_t n_links,
-                          const double* link_velocity, void* stream);
+                          double* link_data, void* stream);
 
 /* ---- Batched ensembles -------------------------------------------------------------------------
  * Many independent domains ("members") advanced by one launch per time step.  lbmpy has no batched

@@ -26,7 +26,7 @@ DTYPE_CODES = {"float64": F64, "float32": F32, "float16": F16}
 STENCIL_NAMES = {"D2Q9": D2Q9, "D3Q19": D3Q19, "D3Q27": D3Q27}
 
 
-BC_NOSLIP, BC_UBB, BC_FREESLIP, BC_FIXED_DENSITY = 0, 1, 2, 3
+BC_NOSLIP, BC_UBB, BC_FREESLIP, BC_FIXED_DENSITY, BC_OUTFLOW, BC_OUTFLOW_INIT = 0, 1, 2, 3, 4, 5
 
 
 class BoundaryDesc(ctypes.Structure):

@@ -1,7 +1,8 @@
-"""Boundaries for the GPU step driver: NoSlip, UBB, FreeSlip and FixedDensity.
+"""Boundaries for the GPU step driver: NoSlip, UBB, FreeSlip, FixedDensity and ExtrapolationOutflow.
 
 Mirrors lbmpy ``boundaries/boundaryconditions.py`` (``NoSlip`` :104-139, ``UBB`` :724-842,
-``FreeSlip`` :360-487, ``FixedDensity`` :1044-1102) and the index-list machinery of ``boundaries/boundaryhandling.py:15-199``:
+``FreeSlip`` :360-487, ``FixedDensity`` :1044-1102, ``ExtrapolationOutflow`` :894-1042) and the index-list
+machinery of ``boundaries/boundaryhandling.py:15-199``:
 boundary cells are flagged on a (ghost-layer inclusive) flag field with the id of their boundary
 object; every link from a fluid cell x into a boundary cell in direction d becomes an index-list
 entry (x, y, z, d) of that object, executed by ``lbm355_boundary_links`` between sweeps.  Links
@@ -94,6 +95,37 @@ class FixedDensity(LbBoundary):
         super().__init__(name)
 
 
+_COMPASS = {"E": (1, 0, 0), "W": (-1, 0, 0), "N": (0, 1, 0), "S": (0, -1, 0), "T": (0, 0, 1), "B": (0, 0, -1)}
+
+
+class ExtrapolationOutflow(LbBoundary):
+    """Outflow by extrapolation along the normal with the previous step's populations
+    (boundaryconditions.py:894-1042, Geier et al. 2015 eq. F.2 with u neglected): per link
+    f_inv(d)(x_b) = c s0 + (1 - c) s1 with c = sqrt(1/3) dt/dx, s0 the previous step's f*_inv(d) at x + c_d - n and
+    s1 the previous interpolated value.  ``normal_direction`` points out of the fluid (e.g. (1, 0, 0) or 'E').
+    The state starts from the populations present when the links are built; dt == dx on the GPU path."""
+    kind = nat.BC_OUTFLOW
+
+    def __init__(self, normal_direction, lb_method, dt=1, dx=1, name=None, streaming_pattern='pull',
+                 zeroth_timestep=None, initial_density=None, initial_velocity=None, data_type='double'):
+        if dt != dx:
+            raise NotImplementedError("the GPU outflow boundary uses dt == dx")
+        if initial_density is not None or initial_velocity is not None:
+            raise NotImplementedError("initial_density / initial_velocity are not available on the GPU path")
+        dim = lb_method.stencil.D
+        if isinstance(normal_direction, str):
+            v = [0, 0, 0]
+            for ch in normal_direction:
+                v = [a + b for a, b in zip(v, _COMPASS[ch])]
+            normal_direction = tuple(v[:dim])
+        self.normal_direction = tuple(int(n) for n in normal_direction)
+        if any(n not in (-1, 0, 1) for n in self.normal_direction):
+            raise ValueError("Only -1, 0 and 1 allowed for defining the normal direction")
+        self.dim = dim
+        self.lb_method = lb_method
+        super().__init__(name or f"Outflow: {normal_direction}")
+
+
 def slice_from_direction(direction_name, dim, normal_offset=0, tangential_offset=0):
     """Slice of the ghost layer in a compass direction ('N', 'S', 'E', 'W', 'T', 'B'),
     as pystencils' ``slice_from_direction`` returns it for ghost-layer inclusive arrays."""
@@ -135,9 +167,9 @@ class LatticeBoltzmannBoundaryHandling:
 
     def set_boundary(self, boundary_obj, slice_obj=None, mask_callback=None, ghost_layers=True,
                      inner_ghost_layers=True, replace=True, force_flag_value=None):
-        if not isinstance(boundary_obj, (NoSlip, UBB, FreeSlip, FixedDensity)):
+        if not isinstance(boundary_obj, (NoSlip, UBB, FreeSlip, FixedDensity, ExtrapolationOutflow)):
             raise NotImplementedError(f"boundary {boundary_obj!r} is not available on the GPU path "
-                                      "(NoSlip, UBB, FreeSlip and FixedDensity are)")
+                                      "(NoSlip, UBB, FreeSlip, FixedDensity and ExtrapolationOutflow are)")
         if getattr(boundary_obj, "dim", self.dim) != self.dim:
             raise ValueError(f"boundary dimension {boundary_obj.dim} does not match the domain ({self.dim})")
         bid = self._id_of(boundary_obj)

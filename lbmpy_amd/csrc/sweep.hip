@@ -239,7 +239,7 @@ __device__ __forceinline__ int64_t bc_in(const BoundaryArgs& b, int i, int cx, i
 }
 
 template <class S, int PAT, class T, int KIND>
-__global__ void boundary_links_kernel(T* pdf, const int32_t* links, int64_t n, const double* link_vel, BoundaryArgs b) {
+__global__ void boundary_links_kernel(T* pdf, const int32_t* links, int64_t n, double* link_vel, BoundaryArgs b) {
   using TC = typename std::conditional<sizeof(T) == 8, double, float>::type;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
@@ -255,6 +255,22 @@ __global__ void boundary_links_kernel(T* pdf, const int32_t* links, int64_t n, c
       cdy = S::c[i][1];
       cdz = S::c[i][2];
     }
+  if (KIND == LBM355_BC_OUTFLOW_INIT || KIND == LBM355_BC_OUTFLOW) {
+    // the fluid cell one step back along the normal from the boundary cell: x + c_d - n
+    const T v = pdf[bc_out<S, PAT>(b, id, x + cdx - b.normal[0], y + cdy - b.normal[1], z + cdz - b.normal[2])];
+    double* st = link_vel + 2 * t;
+    if (KIND == LBM355_BC_OUTFLOW_INIT) {
+      st[0] = (double)v;
+      st[1] = (double)v;
+    } else {
+      const TC c = (TC)0.57735026918962576451;   // sqrt(1/3) dt/dx
+      const TC inter = (TC)st[0] * c + (TC(1) - c) * (TC)st[1];
+      pdf[bc_in<S, PAT>(b, id, x, y, z)] = (T)inter;
+      st[0] = (double)v;
+      st[1] = (double)inter;
+    }
+    return;
+  }
   const int64_t target = bc_in<S, PAT>(b, id, x, y, z);
   if (KIND == LBM355_BC_NOSLIP) {
     pdf[target] = pdf[bc_out<S, PAT>(b, d, x, y, z)];
@@ -550,7 +566,7 @@ int links_dtype(int dtype, int pattern, void* pdf, int64_t sy, int64_t sz, int64
 }
 
 template <class S, int PAT, class T>
-int launch_bc_kind(const lbm355_boundary_desc* d, T* pdf, const int32_t* links, int64_t n, const double* vel,
+int launch_bc_kind(const lbm355_boundary_desc* d, T* pdf, const int32_t* links, int64_t n, double* vel,
                    const BoundaryArgs& b, hipStream_t s) {
   const int grid = (int)((n + 255) / 256);
   if (d->kind == LBM355_BC_NOSLIP)
@@ -562,6 +578,12 @@ int launch_bc_kind(const lbm355_boundary_desc* d, T* pdf, const int32_t* links, 
   else if (d->kind == LBM355_BC_FIXED_DENSITY)
     hipLaunchKernelGGL((boundary_links_kernel<S, PAT, T, LBM355_BC_FIXED_DENSITY>), dim3(grid), dim3(256), 0, s, pdf,
                        links, n, vel, b);
+  else if (d->kind == LBM355_BC_OUTFLOW)
+    hipLaunchKernelGGL((boundary_links_kernel<S, PAT, T, LBM355_BC_OUTFLOW>), dim3(grid), dim3(256), 0, s, pdf, links,
+                       n, vel, b);
+  else if (d->kind == LBM355_BC_OUTFLOW_INIT)
+    hipLaunchKernelGGL((boundary_links_kernel<S, PAT, T, LBM355_BC_OUTFLOW_INIT>), dim3(grid), dim3(256), 0, s, pdf,
+                       links, n, vel, b);
   else
     hipLaunchKernelGGL((boundary_links_kernel<S, PAT, T, LBM355_BC_FREESLIP>), dim3(grid), dim3(256), 0, s, pdf, links,
                        n, vel, b);
@@ -571,7 +593,7 @@ int launch_bc_kind(const lbm355_boundary_desc* d, T* pdf, const int32_t* links, 
 }
 
 template <class S, class T>
-int launch_bc(const lbm355_boundary_desc* d, void* pdf, const int32_t* links, int64_t n, const double* vel,
+int launch_bc(const lbm355_boundary_desc* d, void* pdf, const int32_t* links, int64_t n, double* vel,
               const BoundaryArgs& b, hipStream_t s) {
   if (d->pattern == LBM355_PULL) return launch_bc_kind<S, LBM355_PULL, T>(d, (T*)pdf, links, n, vel, b, s);
   if (d->pattern == LBM355_AA_EVEN) return launch_bc_kind<S, LBM355_AA_EVEN, T>(d, (T*)pdf, links, n, vel, b, s);
@@ -579,7 +601,7 @@ int launch_bc(const lbm355_boundary_desc* d, void* pdf, const int32_t* links, in
 }
 
 template <class S>
-int launch_bc_dtype(const lbm355_boundary_desc* d, void* pdf, const int32_t* links, int64_t n, const double* vel,
+int launch_bc_dtype(const lbm355_boundary_desc* d, void* pdf, const int32_t* links, int64_t n, double* vel,
                     const BoundaryArgs& b, hipStream_t s) {
   if (d->store_dtype == LBM355_F64) return launch_bc<S, double>(d, pdf, links, n, vel, b, s);
   if (d->store_dtype == LBM355_F32) return launch_bc<S, float>(d, pdf, links, n, vel, b, s);
@@ -660,13 +682,15 @@ int lbm355_noslip_links(int32_t stencil, int32_t pattern, int32_t dtype, void* p
 }
 
 int lbm355_boundary_links(const lbm355_boundary_desc* d, void* pdf, const int32_t* links, int64_t n_links,
-                          const double* link_velocity, void* stream) {
+                          double* link_velocity, void* stream) {
   if (!d) return fail(LBM355_ERR_ARG, "desc is NULL");
   if (n_links == 0) return LBM355_OK;
   if (!pdf || !links || n_links < 0) return fail(LBM355_ERR_ARG, "bad pointer or link count");
   if (d->pattern < 0 || d->pattern > 2) return fail(LBM355_ERR_ARG, "bad pattern");
   if (d->store_dtype < 0 || d->store_dtype > 2) return fail(LBM355_ERR_ARG, "bad dtype");
-  if (d->kind < LBM355_BC_NOSLIP || d->kind > LBM355_BC_FIXED_DENSITY) return fail(LBM355_ERR_ARG, "bad boundary kind");
+  if (d->kind < LBM355_BC_NOSLIP || d->kind > LBM355_BC_OUTFLOW_INIT) return fail(LBM355_ERR_ARG, "bad boundary kind");
+  if ((d->kind == LBM355_BC_OUTFLOW || d->kind == LBM355_BC_OUTFLOW_INIT) && !link_velocity)
+    return fail(LBM355_ERR_ARG, "the outflow boundary needs its per-link state array");
   if (d->kind == LBM355_BC_FIXED_DENSITY && (!d->equilibrium || !(d->density > 0)))
     return fail(LBM355_ERR_ARG, "FixedDensity needs a positive density and the equilibrium table");
   const int D = stencil_d(d->stencil);

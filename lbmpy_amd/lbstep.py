@@ -23,7 +23,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
-from .boundaries import LatticeBoltzmannBoundaryHandling, UBB, FreeSlip, FixedDensity
+from .boundaries import LatticeBoltzmannBoundaryHandling, UBB, FreeSlip, FixedDensity, ExtrapolationOutflow
 from .creationfunctions import (update_with_default_parameters, create_lb_function, Timestep, CreateKernelConfig)
 from .enums import Stencil, Target
 from .kernel_tables import method_table, fixed_density_table
@@ -221,9 +221,13 @@ class LatticeBoltzmannStep:
         key = self._boundary_handling.version
         if self._links_version != key:
             self._links_version = key
-            self._links_dev = [(obj, torch.from_numpy(np.ascontiguousarray(l)).cuda(),
-                                None if v is None else torch.from_numpy(np.ascontiguousarray(v)).cuda())
-                               for obj, l, v in groups]
+            self._links_dev = []
+            for obj, l, v in groups:
+                data = None if v is None else torch.from_numpy(np.ascontiguousarray(v)).cuda()
+                if isinstance(obj, ExtrapolationOutflow):
+                    data = torch.zeros((len(l), 2), dtype=torch.float64, device="cuda")    # set before first use
+                self._links_dev.append([obj, torch.from_numpy(np.ascontiguousarray(l)).cuda(), data,
+                                        not isinstance(obj, ExtrapolationOutflow)])
         return self._links_dev
 
     def _apply_links(self, pattern):
@@ -234,7 +238,8 @@ class LatticeBoltzmannStep:
         D = self.dim
         m = self.method
         lib = nat.load()
-        for obj, links, vel in groups:
+        for group in groups:
+            obj, links, vel, ready = group
             d = nat.BoundaryDesc()
             d.stencil = nat.STENCIL_NAMES[m.stencil.name]
             d.pattern = pattern
@@ -260,6 +265,18 @@ class LatticeBoltzmannStep:
             elif isinstance(obj, FreeSlip):
                 for ax, v in enumerate(obj.normal_direction):
                     d.normal[ax] = int(v)
+            elif isinstance(obj, ExtrapolationOutflow):
+                for ax, v in enumerate(obj.normal_direction):
+                    d.normal[ax] = int(v)
+                if not ready:
+                    # the state starts from the populations present now (read through the same accessor)
+                    d.kind = nat.BC_OUTFLOW_INIT
+                    nat.check(lib.lbm355_boundary_links(ctypes.byref(d), ctypes.c_void_p(a.data_ptr()),
+                                                        ctypes.c_void_p(links.data_ptr()), links.shape[0],
+                                                        ctypes.c_void_p(vel.data_ptr()),
+                                                        nat.current_stream_handle()))
+                    d.kind = nat.BC_OUTFLOW
+                    group[3] = True
             elif isinstance(obj, FixedDensity):
                 if getattr(self, "_fd_table", None) is None:
                     self._fd_table = torch.from_numpy(fixed_density_table(m)).cuda()

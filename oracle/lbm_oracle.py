@@ -421,8 +421,11 @@ class OracleDomain:
         (what ``set_boundary(NoSlip(), mask_callback=...)`` flags).  ``boundaries``: optional list of
         (flags, spec) with flags a bool array [x, y, (z)] over the ghost-inclusive domain and spec
         {"kind": "noslip"}, {"kind": "ubb", "velocity": u_w, "density": rho_w or None} or
-        {"kind": "freeslip", "normal": n} (n points from the wall into the fluid)."""
+        {"kind": "freeslip", "normal": n} (n points from the wall into the fluid),
+        {"kind": "fixeddensity", "density": rho_w} or {"kind": "outflow", "normal": n} (n points out of the
+        fluid)."""
         self.boundaries = list(boundaries or [])
+        self._outflow_state = {}
         self.coll = OracleCollision(method, bindings)
         self.obstacle = None if obstacle is None else np.asarray(obstacle, dtype=bool)
         self.method = method
@@ -570,6 +573,18 @@ class OracleDomain:
                 mi = np.array([self._dir_index(tuple(v)) for v in mirrored])
                 src_cell = [cell[a] + self.c[d, a] + nrm[a] for a in range(D)]
                 arr[target] = arr[self._out_addr(prev, self.inv[mi], src_cell)]
+            elif spec["kind"] == "outflow":
+                # extrapolation outflow (boundaryconditions.py:894-1042) with per-link state (s0, s1)
+                nrm = np.asarray(spec["normal"], dtype=int)
+                src = [cell[a] + self.c[d, a] - nrm[a] for a in range(D)]
+                v = arr[self._out_addr(prev, self.inv[d], src)]
+                st = self._outflow_state.get(k)
+                if st is None:
+                    st = np.stack([v, v])
+                c = np.sqrt(1.0 / 3.0)
+                inter = st[0] * c + (1.0 - c) * st[1]
+                arr[target] = inter
+                self._outflow_state[k] = np.stack([v, inter])
             elif spec["kind"] == "fixeddensity":
                 arr[target] = self._fixed_density_value(arr, prev, d, cell, float(spec["density"]))
             else:

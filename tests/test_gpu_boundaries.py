@@ -170,3 +170,34 @@ def test_fixed_density_pressure_channel(pattern, compressible):
     ref = dom.pdfs_in_natural_layout()
     assert np.abs(got - ref).max() <= 1e-12
     assert np.abs(got - f0).max() > 1e-5
+
+
+@pytest.mark.parametrize("pattern", ["pull", "aa"])
+def test_extrapolation_outflow_channel(pattern):
+    """Channel with a UBB inlet (W), NoSlip walls (N, S) and an extrapolation outflow (E); the outflow keeps its
+    per-link state on the GPU.  Compared with the oracle after 60 steps."""
+    from lbmpy_amd.boundaries import ExtrapolationOutflow
+    lc = LBMConfig(stencil=LBStencil(Stencil.D2Q9), method=Method.TRT, relaxation_rates=[1.7, 1.2],
+                   compressible=True, streaming_pattern=pattern)
+    step = _make((24, 10), (False, False), lc)
+    step.boundary_handling.set_boundary(UBB((0.03, 0.0)), slice_from_direction("W", 2))
+    step.boundary_handling.set_boundary(ExtrapolationOutflow((1, 0), step.method), slice_from_direction("E", 2))
+    for w in ("N", "S"):
+        step.boundary_handling.set_boundary(NoSlip(), slice_from_direction(w, 2))
+    print(f"outflow {pattern}", flush=True)
+    step.run(60)
+    got = _natural(step)
+    bh = step.boundary_handling
+    specs = []
+    for k, o in enumerate(bh._objects):
+        spec = {"kind": "outflow", "normal": o.normal_direction} if isinstance(o, ExtrapolationOutflow) else _spec(o)
+        specs.append((bh.flags == k + 1, spec))
+    m = step.method
+    dom = OracleDomain(m, (24, 10), faces={0: "ghost", 1: "ghost"}, pattern=pattern, boundaries=specs)
+    f0 = equilibrium_pdfs(m, np.ones((10, 24)), np.zeros((2, 10, 24)))
+    dom.set_pdfs_interior(f0[dom.inv] if pattern == "aa" else f0)
+    for _ in range(60):
+        dom.step()
+    ref = dom.pdfs_in_natural_layout()
+    assert np.abs(got - ref).max() <= 1e-12
+    assert np.abs(got - f0).max() > 1e-4

@@ -158,3 +158,25 @@ def test_fixed_density_pressure_driven_channel():
     np.testing.assert_allclose(ux[:, nx // 2], want, rtol=0.02)
     assert abs(abs(grad.mean()) * nx - 0.002) <= 0.1 * 0.002
     np.testing.assert_allclose(rmid - 1.0, -(rmid[::-1] - 1.0), atol=5e-6)
+
+
+@pytest.mark.parametrize("pattern", ["pull", "aa"])
+def test_outflow_passes_uniform_flow(pattern):
+    """UBB inlet (W) at U and extrapolation outflow (E), periodic y: the uniform equilibrium flow at U is a steady
+    state of the scheme -- the outflow's extrapolation reproduces it exactly."""
+    U = 0.03
+    m = create_lb_method(LBMConfig(stencil=LBStencil(Stencil.D2Q9), method=Method.TRT, relaxation_rates=[1.6, 1.2],
+                                   streaming_pattern=pattern))
+    nx, ny = 12, 6
+    gs = (nx + 2, ny + 2)
+    bnd = [(_plane(gs, 0, 0), {"kind": "ubb", "velocity": (U, 0.0), "density": None}),
+           (_plane(gs, 0, -1), {"kind": "outflow", "normal": (1, 0)})]
+    from oracle.lbm_oracle import equilibrium_pdfs
+    u = np.zeros((2, ny, nx))
+    u[0] = U
+    f0 = equilibrium_pdfs(m, np.ones((ny, nx)), u)
+    dom = OracleDomain(m, (nx, ny), faces={0: "ghost", 1: "periodic"}, pattern=pattern, boundaries=bnd)
+    dom.set_pdfs_interior(f0[dom.inv] if pattern == "aa" else f0)
+    for _ in range(30):
+        dom.step()
+    assert np.abs(dom.pdfs_in_natural_layout() - f0).max() <= 1e-15
