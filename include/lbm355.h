@@ -60,6 +60,9 @@ extern "C" {
 #define LBM355_FLAG_FORCE 8         /* a force model is present (macroscopic velocity shift) */
 #define LBM355_FLAG_SMAGORINSKY 16  /* Smagorinsky subgrid model on the shear rate */
 #define LBM355_FLAG_LEARNED_CLOSURE 32 /* learned per-cell eddy viscosity on the shear rate (cumulant, FP32) */
+#define LBM355_FLAG_GALILEAN 64     /* D3Q27 cumulant: Galilean correction (galilean_correction.py:21-91) */
+#define LBM355_FLAG_FOURTH_ORDER 128 /* D3Q27 cumulant: fourth-order correction with limited third-order rates
+                                        (fourth_order_correction.py:21-237); parameters in the table */
 
 /* error codes */
 #define LBM355_OK 0

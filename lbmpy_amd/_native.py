@@ -20,7 +20,7 @@ PULL, AA_EVEN, AA_ODD = 0, 1, 2
 F64, F32, F16 = 0, 1, 2
 FACE_GHOST, FACE_PERIODIC, FACE_NOSLIP = 0, 1, 2
 FLAG_ZERO_CENTERED, FLAG_COMPRESSIBLE, FLAG_EQ_SHIFT, FLAG_FORCE, FLAG_SMAGORINSKY = 1, 2, 4, 8, 16
-FLAG_LEARNED_CLOSURE = 32
+FLAG_LEARNED_CLOSURE, FLAG_GALILEAN, FLAG_FOURTH_ORDER = 32, 64, 128
 
 DTYPE_CODES = {"float64": F64, "float32": F32, "float16": F16}
 STENCIL_NAMES = {"D2Q9": D2Q9, "D3Q19": D3Q19, "D3Q27": D3Q27}

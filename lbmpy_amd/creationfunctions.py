@@ -328,8 +328,17 @@ def create_lb_method(lbm_config=None, **params):
         method = create_central_moment(lbm_config.stencil, rates, nested_moments=lbm_config.nested_moments,
                                        conserved_moments=lbm_config.conserved_moments, **common)
     elif m == Method.CUMULANT:
-        if lbm_config.fourth_order_correction or lbm_config.galilean_correction:
-            raise NotImplementedError("Galilean / fourth-order cumulant corrections are not implemented")
+        if lbm_config.fourth_order_correction:
+            # lbmpy's own condition (creationfunctions.py:835-846); a non-D3Q27 stencil that passes it
+            # fails at the collision rule, as in lbmpy
+            if lbm_config.stencil.D != 3 and lbm_config.stencil.Q != 27:
+                raise ValueError("Fourth-order correction can only be applied to D3Q27 cumulant methods.")
+            assert len(rates) <= 2, "Optimal parametrisation for fourth-order cumulants needs either one " \
+                                    "or two relaxation rates, associated with the shear (and bulk) " \
+                                    "viscosity. All other relaxation rates are automatically chosen optimally"
+            # the method carries symbolic rates; their values are assigned with the collision rule
+            from .kernel_tables import CORRECTION_RATE_NAMES
+            rates = [_symbol(n) for n in CORRECTION_RATE_NAMES]
         kw = dict(zero_centered=lbm_config.zero_centered, force_model=lbm_config.force_model,
                   c_s_sq=Fraction(lbm_config.c_s_sq))
         if lbm_config.nested_moments is not None:
@@ -368,9 +377,26 @@ def create_lb_collision_rule(lb_method=None, lbm_config=None, lbm_optimisation=N
         lb_method = lbm_config.lb_method
     if lb_method is None:
         lb_method = create_lb_method(lbm_config)
+    _check_cumulant_corrections(lb_method, lbm_config)
     rule = CollisionRule(lb_method, lbm_config)
     lbm_config.collision_rule = rule
     return rule
+
+
+def _check_cumulant_corrections(method, lbm_config):
+    """The checks lbmpy makes when it adds the cumulant corrections to a collision rule
+    (creationfunctions.py:692-707, galilean_correction.py:21-37, fourth_order_correction.py:21-36); the
+    fourth-order correction gets a bulk rate of 1 when only the shear rate is given."""
+    from .kernel_tables import cumulant_corrections
+    if lbm_config.galilean_correction:
+        if method.kind != "cumulant" or method.stencil.name != "D3Q27":
+            raise ValueError("Galilean correction is only defined for D3Q27 cumulant methods.")
+    if lbm_config.fourth_order_correction:
+        if len(lbm_config.relaxation_rates) == 1:
+            lbm_config.relaxation_rates = list(lbm_config.relaxation_rates) + [1]
+        if method.kind != "cumulant" or method.stencil.name != "D3Q27":
+            raise ValueError("Fourth-order correction is only defined for D3Q27 cumulant methods.")
+    return cumulant_corrections(lbm_config)
 
 
 def create_lb_update_rule(collision_rule=None, lbm_config=None, lbm_optimisation=None, config=None,

@@ -81,14 +81,23 @@ __device__ __forceinline__ void shift_bwd(T& k0, T& k1, T& k2, T u) {
 }
 
 // cumulant-table layout: 27 per-polynomial relaxation rates in the order of the default
-// polynomial cumulant set (density, x, y, z, xy, xz, yz, x2-y2, x2-z2, x2+y2+z2, ...).
+// polynomial cumulant set (density, x, y, z, xy, xz, yz, x2-y2, x2-z2, x2+y2+z2, ...), then the
+// fourth-order correction's parameters a, b (fourth_order_correction.py:70-94), its unlimited rates
+// omega_3, omega_4, omega_5 (:127-139) and the limiter (zeros without the correction).
 struct CumulantTab {
-  static constexpr int RATES = 0, SIZE = 27;
+  static constexpr int RATES = 0, EXTRA = 27, SIZE = 33;
+  static constexpr int A = 0, B = 1, W3 = 2, W4 = 3, W5 = 4, LIMITER = 5;
 };
 
 // LEARNED: the shear rate comes from the learned closure (a separate kernel variant, so that the plain
-// sweeps carry none of its MFMA register state)
-template <class T, bool LEARNED = false>
+// sweeps carry none of its MFMA register state).  CORRECTED: the variant that adds the Galilean
+// correction (galilean_correction.py:56-91) and / or the fourth-order correction with its limited
+// third-order rates (fourth_order_correction.py:97-237), per LBM355_FLAG_GALILEAN /
+// LBM355_FLAG_FOURTH_ORDER.  Both corrections are written for normalised deviations: lbmpy's
+// cumulants are rho-scaled (C = rho k, cumulanttransforms.py:181), so a correction term cor added
+// to a post-collision polynomial cumulant enters here as cor / rho, and its derivative estimates,
+// built from c_ab / rho = gamma_ab + mu'_ab, take the deviations mu'_ab directly.
+template <class T, bool LEARNED = false, bool CORRECTED = false>
 __device__ __forceinline__ void collide_cumulant_d3q27(T (&f)[27], const SweepArgs<T>& a, T& rho_out_v,
                                                        T (&u_out_v)[3]) {
   using S = D3Q27;
@@ -207,26 +216,72 @@ __device__ __forceinline__ void collide_cumulant_d3q27(T (&f)[27], const SweepAr
                  T(4) * (mu011 * mu211 + mu101 * mu121 + mu110 * mu112) -
                  T(2) * (mu012 * mu210 + mu021 * mu201 + mu102 * mu120) - T(4) * mu111 * mu111;
 
+  // third-order rates; the fourth-order correction limits them per cell with the x components of the
+  // pre-collision third-order cumulants (fourth_order_correction.py:191-198, one value per group):
+  // w + (1 - w) |C| / (rho lim + |C|), and with C = rho mu' the density cancels
+  T w3[3] = {r[10], r[11], r[12]}, w4[3] = {r[13], r[14], r[15]};
+  T w5 = r[16];
+  if constexpr (CORRECTED) {
+    if (a.flags & LBM355_FLAG_FOURTH_ORDER) {
+      const T* __restrict__ x = a.tab + CumulantTab::EXTRA;
+      const T lim = x[CumulantTab::LIMITER];
+      const T sp = fabs(mu120 + mu102), sm = fabs(mu120 - mu102), s1 = fabs(mu111);
+      const T l3 = x[CumulantTab::W3] + (T(1) - x[CumulantTab::W3]) * sp / (lim + sp);
+      const T l4 = x[CumulantTab::W4] + (T(1) - x[CumulantTab::W4]) * sm / (lim + sm);
+      w3[0] = w3[1] = w3[2] = l3;
+      w4[0] = w4[1] = w4[2] = l4;
+      w5 = x[CumulantTab::W5] + (T(1) - x[CumulantTab::W5]) * s1 / (lim + s1);
+    }
+  }
+
   // ---- relaxation of the polynomial cumulants (equilibrium deviation is zero)
   const T c110 = (T(1) - rs[0]) * mu110, c101 = (T(1) - rs[1]) * mu101, c011 = (T(1) - rs[2]) * mu011;
-  const T d1 = (T(1) - rs[3]) * (mu200 - mu020);
-  const T d2 = (T(1) - rs[4]) * (mu200 - mu002);
-  const T tr = (T(1) - r[9]) * (mu200 + mu020 + mu002);
+  T d1 = (T(1) - rs[3]) * (mu200 - mu020);
+  T d2 = (T(1) - rs[4]) * (mu200 - mu002);
+  T tr = (T(1) - r[9]) * (mu200 + mu020 + mu002);
+  T p1 = (T(1) - r[17]) * (k220 - T(2) * k202 + k022);
+  T p2 = (T(1) - r[18]) * (k220 + k202 - T(2) * k022);
+  T s4 = (T(1) - r[19]) * (k220 + k202 + k022);
+  T c211 = (T(1) - r[20]) * k211, c121 = (T(1) - r[21]) * k121, c112 = (T(1) - r[22]) * k112;
+  if constexpr (CORRECTED) {
+    // velocity-gradient estimates from the pre-collision second-order cumulants (shear rate w1 of
+    // x2-y2, the Smagorinsky rate when that model is on; bulk rate w2)
+    const T w1 = rs[3], w2 = r[9];
+    const T dxu = -T(0.5) * w1 * (T(2) * mu200 - mu020 - mu002) - T(0.5) * w2 * (mu200 + mu020 + mu002);
+    const T dyv = dxu + T(1.5) * w1 * (mu200 - mu020);
+    const T dzw = dxu + T(1.5) * w1 * (mu200 - mu002);
+    if (a.flags & LBM355_FLAG_GALILEAN) {
+      // cor = -3 rho (1 - w/2) (u_a^2 D_a - u_b^2 D_b) on x2-y2, x2-z2; (1 - w2/2) and the sum on x2+y2+z2
+      const T g1 = T(-3) * (T(1) - T(0.5) * w1), g2 = T(-3) * (T(1) - T(0.5) * w2);
+      const T ex = u2[0] * dxu, ey = u2[1] * dyv, ez = u2[2] * dzw;
+      d1 += g1 * (ex - ey);
+      d2 += g1 * (ex - ez);
+      tr += g2 * (ex + ey + ez);
+    }
+    if (a.flags & LBM355_FLAG_FOURTH_ORDER) {
+      const T* __restrict__ x = a.tab + CumulantTab::EXTRA;
+      const T pre = T(1) / w1 - T(0.5);
+      const T pa = pre * x[CumulantTab::A], pb = pre * x[CumulantTab::B];
+      p1 += T(2.0 / 3.0) * pa * r[17] * (dxu - T(2) * dyv + dzw);
+      p2 += T(2.0 / 3.0) * pa * r[17] * (dxu + dyv - T(2) * dzw);
+      s4 -= T(4.0 / 3.0) * pa * r[19] * (dxu + dyv + dzw);
+      // -1/3 pre w8 b (D_bc = -3 w1 mu'_bc)
+      c211 += pb * r[20] * w1 * mu011;
+      c121 += pb * r[20] * w1 * mu101;
+      c112 += pb * r[20] * w1 * mu110;
+    }
+  }
   const T c200 = (d1 + d2 + tr) * third;
   const T c020 = (tr - T(2) * d1 + d2) * third;
   const T c002 = (tr + d1 - T(2) * d2) * third;
-  const T ax = (T(1) - r[10]) * (mu120 + mu102), bx = (T(1) - r[13]) * (mu120 - mu102);
-  const T ay = (T(1) - r[11]) * (mu210 + mu012), by = (T(1) - r[14]) * (mu210 - mu012);
-  const T az = (T(1) - r[12]) * (mu201 + mu021), bz = (T(1) - r[15]) * (mu201 - mu021);
+  const T ax = (T(1) - w3[0]) * (mu120 + mu102), bx = (T(1) - w4[0]) * (mu120 - mu102);
+  const T ay = (T(1) - w3[1]) * (mu210 + mu012), by = (T(1) - w4[1]) * (mu210 - mu012);
+  const T az = (T(1) - w3[2]) * (mu201 + mu021), bz = (T(1) - w4[2]) * (mu201 - mu021);
   const T c120 = T(0.5) * (ax + bx), c102 = T(0.5) * (ax - bx);
   const T c210 = T(0.5) * (ay + by), c012 = T(0.5) * (ay - by);
   const T c201 = T(0.5) * (az + bz), c021 = T(0.5) * (az - bz);
-  const T c111 = (T(1) - r[16]) * mu111;
-  const T p1 = (T(1) - r[17]) * (k220 - T(2) * k202 + k022);
-  const T p2 = (T(1) - r[18]) * (k220 + k202 - T(2) * k022);
-  const T s4 = (T(1) - r[19]) * (k220 + k202 + k022);
+  const T c111 = (T(1) - w5) * mu111;
   const T c220 = (s4 + p1 + p2) * third, c202 = (s4 - p1) * third, c022 = (s4 - p2) * third;
-  const T c211 = (T(1) - r[20]) * k211, c121 = (T(1) - r[21]) * k121, c112 = (T(1) - r[22]) * k112;
   const T c221 = (T(1) - r[23]) * k221, c212 = (T(1) - r[24]) * k212, c122 = (T(1) - r[25]) * k122;
   const T c222 = (T(1) - r[26]) * k222;
 

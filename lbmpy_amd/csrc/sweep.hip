@@ -472,6 +472,8 @@ void* pick_kernel(const lbm355_sweep_desc* d, bool walls, bool rowlds) {
       }
       return nullptr;
     }
+    if (d->flags & (LBM355_FLAG_GALILEAN | LBM355_FLAG_FOURTH_ORDER))
+      return pick_store<D3Q27, FAMILY_CUMULANT_CORRECTED, TC>(d->store_dtype, d->pattern, walls, rowlds);
     return pick_store<D3Q27, LBM355_FAMILY_CUMULANT, TC>(d->store_dtype, d->pattern, walls, rowlds);
   } else if (d->family == LBM355_FAMILY_CENTRAL) {
     if (d->stencil == LBM355_D2Q9)
@@ -644,6 +646,14 @@ int lbm355_sweep(const lbm355_sweep_desc* d, const void* pdf_src, void* pdf_dst,
     return fail(LBM355_ERR_UNSUPPORTED, "cumulant sweep is implemented for D3Q27");
   if ((d->flags & LBM355_FLAG_SMAGORINSKY) && d->family != LBM355_FAMILY_CUMULANT)
     return fail(LBM355_ERR_UNSUPPORTED, "Smagorinsky is implemented for the cumulant sweep");
+  if (d->flags & (LBM355_FLAG_GALILEAN | LBM355_FLAG_FOURTH_ORDER)) {
+    if (d->family != LBM355_FAMILY_CUMULANT)
+      return fail(LBM355_ERR_UNSUPPORTED, "the Galilean / fourth-order corrections are D3Q27 cumulant options");
+    if (d->flags & LBM355_FLAG_LEARNED_CLOSURE)
+      return fail(LBM355_ERR_UNSUPPORTED, "the cumulant corrections are not combined with the learned closure");
+    if ((d->flags & LBM355_FLAG_FOURTH_ORDER) && (d->flags & LBM355_FLAG_SMAGORINSKY))
+      return fail(LBM355_ERR_UNSUPPORTED, "the fourth-order correction is not combined with the Smagorinsky model");
+  }
   if (d->flags & LBM355_FLAG_LEARNED_CLOSURE) {
     if (d->family != LBM355_FAMILY_CUMULANT || d->compute_dtype != LBM355_F32 || d->store_dtype != LBM355_F32)
       return fail(LBM355_ERR_UNSUPPORTED, "the learned closure runs in the FP32 cumulant sweep (FP32 storage)");

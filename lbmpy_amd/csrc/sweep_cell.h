@@ -34,6 +34,17 @@ struct Collider<D3Q27, FAMILY_CUMULANT_LEARNED, TC> {
   }
 };
 
+// kernel-variant family code of the cumulant sweep with the Galilean / fourth-order corrections (internal)
+constexpr int FAMILY_CUMULANT_CORRECTED = 0x103;
+
+template <class TC>
+struct Collider<D3Q27, FAMILY_CUMULANT_CORRECTED, TC> {
+  static constexpr bool deviation_form = true;
+  static __device__ __forceinline__ void run(TC (&f)[27], const SweepArgs<TC>& a, TC& rho, TC (&u)[3]) {
+    collide_cumulant_d3q27<TC, false, true>(f, a, rho, u);
+  }
+};
+
 template <class TC>
 struct Collider<D3Q27, LBM355_FAMILY_CUMULANT, TC> {
   static constexpr bool deviation_form = true;

@@ -202,8 +202,80 @@ def moment_table(method, bindings, dtype=np.float64):
 
 # ------------------------------------------------------------------------------- cumulant family
 
-def cumulant_table(method, bindings, dtype=np.float64):
-    """Per-polynomial rates in the order of the default D3Q27 polynomial cumulant set."""
+# ------------------------------------------------------------------------------- cumulant corrections
+
+CORRECTION_RATE_NAMES = tuple(f"corr_rr_{i}" for i in range(10))
+# which of the ten fourth-order rate symbols each polynomial of the default D3Q27 set carries, as
+# _get_relaxation_info_dict assigns ten rates (shear, bulk, then one per remaining group);
+# None: conserved (rate 0)
+_CORRECTION_RATE_SLOTS = (None,) * 4 + (0,) * 5 + (1,) + (2,) * 3 + (3,) * 3 + (4,) + (5,) * 2 + (6,) + \
+    (7,) * 3 + (8,) * 3 + (9,)
+
+
+class CumulantCorrections:
+    """The D3Q27 cumulant corrections an LBMConfig asks for (lbmpy ``creationfunctions.py:692-707``):
+    the Galilean correction (``methods/cumulantbased/galilean_correction.py:21-91``) and the fourth-order
+    correction (``methods/cumulantbased/fourth_order_correction.py:21-237``) with its limiter (1e6 for
+    ``True``) and the shear / bulk rates its optimal parametrisation is computed from."""
+
+    def __init__(self, galilean=False, fourth_order=False, limiter=1e6, shear=None, bulk=1):
+        self.galilean = bool(galilean)
+        self.fourth_order = bool(fourth_order)
+        self.limiter = float(limiter)
+        self.shear = shear
+        self.bulk = bulk
+
+    def free_symbols(self):
+        out = []
+        for r in (self.shear, self.bulk) if self.fourth_order else ():
+            for s in getattr(r, "free_symbols", ()):
+                if s.name not in out:
+                    out.append(s.name)
+        return out
+
+    def __repr__(self):
+        return (f"CumulantCorrections(galilean={self.galilean}, fourth_order={self.fourth_order}, "
+                f"limiter={self.limiter}, shear={self.shear}, bulk={self.bulk})")
+
+
+def cumulant_corrections(lbm_config):
+    """The corrections of ``lbm_config`` or None.  The fourth-order correction takes the shear rate from
+    ``relaxation_rates[0]`` and the bulk rate from ``relaxation_rates[1]`` (1 when only one is given,
+    creationfunctions.py:699-701)."""
+    gal = bool(lbm_config.galilean_correction)
+    foc = lbm_config.fourth_order_correction
+    if not gal and not foc:
+        return None
+    rr = list(lbm_config.relaxation_rates or [])
+    if foc:
+        limiter = 1e6 if foc is True else float(foc)
+        return CumulantCorrections(gal, True, limiter, rr[0], rr[1] if len(rr) > 1 else 1)
+    return CumulantCorrections(gal, False)
+
+
+def fourth_order_parameters(w1, w2):
+    """Optimal a, b (Geier et al. 2017, eq. 115-116) and unlimited omega_3, omega_4, omega_5 (eq. 112-114)
+    for shear rate w1 and bulk rate w2, as lbmpy evaluates them (fourth_order_correction.py:70-94,
+    127-139)."""
+    d = (w1 - w2) * (w2 * (2.0 + 3.0 * w1) - 8.0 * w1)
+    if d == 0.0:
+        raise ValueError("the fourth-order correction needs distinct shear and bulk rates with "
+                         "w2 (2 + 3 w1) != 8 w1")
+    a = (4.0 * w1 * w1 + 2.0 * w1 * w2 * (w1 - 6.0) + w2 * w2 * (w1 * (10.0 - 3.0 * w1) - 4.0)) / d
+    b = (4.0 * w1 * w2 * (9.0 * w1 - 16.0) - 4.0 * w1 * w1 - 2.0 * w2 * w2 * (2.0 + 9.0 * w1 * (w1 - 2.0))) / (3.0 * d)
+    w3 = 8.0 * (w1 - 2.0) * (w2 * (3.0 * w1 - 1.0) - 5.0 * w1) / \
+        (8.0 * (5.0 - 2.0 * w1) * w1 + w2 * (8.0 + w1 * (9.0 * w1 - 26.0)))
+    w4 = 8.0 * (w1 - 2.0) * (w1 + w2 * (3.0 * w1 - 7.0)) / (w2 * (56.0 - 42.0 * w1 + 9.0 * w1 * w1) - 8.0 * w1)
+    w5 = 24.0 * (w1 - 2.0) * (4.0 * w1 * w1 + w1 * w2 * (18.0 - 13.0 * w1) + w2 * w2 * (2.0 + w1 * (6.0 * w1 - 11.0))) / \
+        (16.0 * w1 * w1 * (w1 - 6.0) - 2.0 * w1 * w2 * (216.0 + 5.0 * w1 * (9.0 * w1 - 46.0)) +
+         w2 * w2 * (w1 * (3.0 * w1 - 10.0) * (15.0 * w1 - 28.0) - 48.0))
+    return a, b, w3, w4, w5
+
+
+def cumulant_table(method, bindings, dtype=np.float64, corrections=None):
+    """Per-polynomial rates in the order of the default D3Q27 polynomial cumulant set, then the
+    fourth-order correction's a, b, omega_3..5 and limiter (layout CumulantTab in
+    csrc/collide_cumulant.h)."""
     st = method.stencil
     if st.name != "D3Q27":
         raise UnsupportedMethod("the cumulant sweep is implemented for D3Q27")
@@ -214,10 +286,27 @@ def cumulant_table(method, bindings, dtype=np.float64):
     fm = method.force_model
     if fm is not None and type(fm).__name__ != "CentralMoment":
         raise UnsupportedMethod("cumulant sweep supports the CentralMoment force model")
-    rates = np.array([bind_rate(r, bindings) for r in method.relaxation_rates])
+    extra = np.zeros(6)
+    named = {}
+    if corrections is not None and corrections.fourth_order:
+        w1 = bind_rate(corrections.shear, bindings)
+        w2 = bind_rate(corrections.bulk, bindings)
+        a, b, w3, w4, w5 = fourth_order_parameters(w1, w2)
+        extra[:] = (a, b, w3, w4, w5, corrections.limiter)
+        named = dict(zip(CORRECTION_RATE_NAMES, (w1, w2, w3, w4, w5, 1.0, 1.0, 1.0, 1.0, 1.0)))
+        for slot, r in zip(_CORRECTION_RATE_SLOTS, method.relaxation_rates):
+            want = 0.0 if slot is None else CORRECTION_RATE_NAMES[slot]
+            got = getattr(r, "name", None) if slot is not None else float(r)
+            if got != want:
+                raise UnsupportedMethod("fourth-order correction: the method's rates are not the ten "
+                                        "corr_rr symbols in the default polynomial groups")
+    rates = np.array([named[r.name] if getattr(r, "name", None) in named else bind_rate(r, bindings)
+                      for r in method.relaxation_rates])
     if rates[0] != 0.0:
         raise UnsupportedMethod("the zeroth cumulant must not be relaxed")
-    return rates.astype(dtype), 0
+    if corrections is not None and corrections.galilean and rates[7] != rates[8]:
+        raise ValueError("Cumulants (x^2 - y^2) and (x^2 - z^2) must have the same relaxation rate")
+    return np.concatenate([rates, extra]).astype(dtype), 0
 
 
 # ------------------------------------------------------------------------------- central-moment family
@@ -327,10 +416,12 @@ def fixed_density_table(method):
     return tab
 
 
-def method_table(method, bindings, dtype=np.float64):
+def method_table(method, bindings, dtype=np.float64, corrections=None):
     fam = family_of(method)
+    if corrections is not None and fam != FAMILY_CUMULANT:
+        raise ValueError("Galilean / fourth-order corrections are only defined for D3Q27 cumulant methods.")
     if fam == FAMILY_MOMENT:
         return moment_table(method, bindings, dtype)
     if fam == FAMILY_CUMULANT:
-        return cumulant_table(method, bindings, dtype)
+        return cumulant_table(method, bindings, dtype, corrections)
     return central_table(method, bindings, dtype)

@@ -204,7 +204,8 @@ class LatticeBoltzmannStep:
         for a in range(D):
             self._u_dev[a][inner] = torch.from_numpy(np.ascontiguousarray(self._u_host[..., a].T)).cuda()
         d = self._descriptor()
-        tab, _ = method_table(self.method, self.kernel_params, np.float64)
+        tab, _ = method_table(self.method, self.kernel_params, np.float64,
+                              corrections=self._kernels[0].corrections)
         tab_dev = torch.from_numpy(tab).cuda()
         prev = nat.PULL if self._pattern == "pull" else nat.AA_EVEN
         lib = nat.load()

@@ -16,7 +16,8 @@ import numpy as np
 import torch
 
 from . import _native as nat
-from .kernel_tables import method_table, family_of, STENCIL_CODES, UnsupportedMethod, bind_rate
+from .kernel_tables import method_table, family_of, STENCIL_CODES, UnsupportedMethod, bind_rate, \
+    cumulant_corrections
 from .relaxationrates import get_shear_relaxation_rate
 from .enums import SubgridScaleModel, Target
 
@@ -110,9 +111,18 @@ class SweepKernel:
         self.closure = None
         self._shear_rate = None          # (bindings key, bound shear rate): the lookup is symbolic and slow
         self._tables = {}
-        self.free_symbols = m.relaxation_rate_symbols()
+        self.corrections = cumulant_corrections(lc)
+        if self.corrections is not None and self.corrections.fourth_order:
+            if self.smagorinsky is not None:
+                raise NotImplementedError("the fourth-order correction is not combined with a subgrid model")
+            # the method's rates are the corr_rr symbols; the free ones are those of the shear / bulk rates
+            self.free_symbols = self.corrections.free_symbols()
+        else:
+            self.free_symbols = m.relaxation_rate_symbols()
         # validate the method once with dummy bindings so unsupported configurations fail here
-        method_table(m, {s: 1.0 for s in self.free_symbols})
+        # (rates of 1.7 / 1.1 keep the fourth-order parametrisation regular)
+        probe = {s: (1.7 if k == 0 else 1.1) for k, s in enumerate(self.free_symbols)}
+        method_table(m, probe, corrections=self.corrections)
         nat.load()
 
     # ------------------------------------------------------------------ configuration
@@ -124,6 +134,8 @@ class SweepKernel:
                 raise NotImplementedError("the learned closure runs in the FP32 cumulant sweep with FP32 storage")
             if self.smagorinsky is not None:
                 raise ValueError("a sweep uses either the Smagorinsky model or a learned closure")
+            if self.corrections is not None:
+                raise NotImplementedError("the learned closure is not combined with the cumulant corrections")
         self.closure = model
         return self
 
@@ -137,7 +149,7 @@ class SweepKernel:
         key = (tuple(sorted(bindings.items())), str(device))
         t = self._tables.get(key)
         if t is None:
-            arr, nact = method_table(self.method, bindings)
+            arr, nact = method_table(self.method, bindings, corrections=self.corrections)
             dt = torch.float64 if self.compute == nat.F64 else torch.float32
             t = (torch.tensor(arr, dtype=dt, device=device), nact)
             self._tables[key] = t
@@ -193,6 +205,11 @@ class SweepKernel:
             flags |= nat.FLAG_SMAGORINSKY
             d.smagorinsky_constant = self.smagorinsky
             d.omega_shear = self.shear_rate(bindings)
+        if self.corrections is not None:
+            if self.corrections.galilean:
+                flags |= nat.FLAG_GALILEAN
+            if self.corrections.fourth_order:
+                flags |= nat.FLAG_FOURTH_ORDER
         if self.closure is not None:
             flags |= nat.FLAG_LEARNED_CLOSURE
             d.closure = self.closure.params.data_ptr()

@@ -15,6 +15,9 @@ form in which lbmpy derives it, vectorised over cells with numpy:
 * cumulant methods: ``methods/cumulantbased/cumulantbasedmethod.py:296-402`` with
   the cumulant generating function of ``moment_transforms/cumulanttransforms.py:46-220``
   evaluated by the Leibniz recursion on K' = C' K (no hand-expanded formulas);
+* D3Q27 cumulant corrections: Galilean (``methods/cumulantbased/galilean_correction.py:21-91``) and
+  fourth-order with limited third-order rates (``methods/cumulantbased/fourth_order_correction.py:21-237``),
+  added to the post-collision polynomial cumulants in lbmpy's rho-scaled form;
 * force models: ``forcemodels.py:179-316`` (Simple, Luo, Guo, CentralMoment);
 * streaming: pull ``fieldaccess.py:66-75`` and AA ``fieldaccess.py:136-157`` on
   arrays with one ghost layer, periodic ghost copies (``lbstep.py:148-151``)
@@ -72,13 +75,26 @@ def _mono_exponents(polys, dim):
 class OracleCollision:
     """Collision operator of one method on arrays shaped (Q, ncells)."""
 
-    def __init__(self, method, bindings=None, smagorinsky=None, closure=None):
+    def __init__(self, method, bindings=None, smagorinsky=None, closure=None, galilean=False, fourth_order=None):
         """``smagorinsky``: Smagorinsky constant Cs (turbulence_models.py:25-84) or None;
-        ``closure``: callable (pi_neq (n, 6), rho (n,), omega_shear) -> shear rate (n,), or None."""
+        ``closure``: callable (pi_neq (n, 6), rho (n,), omega_shear) -> shear rate (n,), or None;
+        ``galilean``: the D3Q27 cumulant Galilean correction (galilean_correction.py:21-91);
+        ``fourth_order``: None or (limiter, shear rate, bulk rate) for the fourth-order correction
+        (fourth_order_correction.py:21-237); the method then carries the rate symbols corr_rr_0..9
+        (creationfunctions.py:835-846), bound here as lbmpy's collision rule assigns them."""
         self.m = method
         self.smagorinsky = smagorinsky
         self.closure = closure
+        self.galilean = bool(galilean)
+        self.fourth = fourth_order
         self.bind = dict(bindings or {})
+        if fourth_order is not None:
+            lim, shear, bulk = fourth_order
+            w1, w2 = _rate(shear, self.bind), _rate(bulk, self.bind)
+            self.fourth_ab = _fourth_order_ab(w1, w2)
+            self.fourth_w = _fourth_order_omegas(w1, w2)
+            for i, v in enumerate((w1, w2) + self.fourth_w + (1.0,) * 5):
+                self.bind[f"corr_rr_{i}"] = v
         self.stencil = method.stencil
         self.Q = method.stencil.Q
         self.D = method.stencil.D
@@ -268,11 +284,17 @@ class OracleCollision:
                 if p is None:
                     assert rates[i] == 0.0, "the zeroth cumulant must not be relaxed"
             rates = np.repeat(rates[:, None], n, axis=1)
+            if self.fourth is not None:
+                rates = self._limited_rates(c, rho, rates)
             if self.smagorinsky is not None:
                 rates = self._smagorinsky_rates(f, rho, drho, u, rates)
             if self.closure is not None:
                 rates = self._closure_rates(f, rho, drho, u, rates)
             Cp = C + rates * (ceq - C)
+            if self.galilean:
+                self._add_galilean_correction(Cp, c, rho, u3, rates)
+            if self.fourth is not None:
+                self._add_fourth_order_correction(Cp, c, rho, rates)
             cpost = self.Pinv @ Cp
             kpost = self._central_from_cumulants(cpost, rho, u3, kappa[self.midx[(0, 0, 0)]])
             if self.fm is not None:
@@ -323,6 +345,80 @@ class OracleCollision:
         for i in shear:
             out[i] = w
         return out
+
+    # ---- cumulant corrections ------------------------------------------------------------
+    def _poly_row(self, terms):
+        """Index of the polynomial with the given {exponent: coefficient} terms in the method's list."""
+        want = {tuple(e): Fraction(v) for e, v in terms.items()}
+        for i, p in enumerate(self.polys):
+            if {tuple(e[:3]): Fraction(v) for e, v in p.terms.items()} == want:
+                return i
+        raise ValueError(f"polynomial {want} is not among the method's cumulants")
+
+    def _c(self, c, e):
+        return c[self.midx[e]]
+
+    def _velocity_gradients(self, c, rho, w1, w2):
+        """D_x, D_y, D_z from the pre-collision second-order cumulants (galilean_correction.py:78-84,
+        fourth_order_correction.py:137-146)."""
+        cxx, cyy, czz = self._c(c, (2, 0, 0)), self._c(c, (0, 2, 0)), self._c(c, (0, 0, 2))
+        dx = -w1 / (2 * rho) * (2 * cxx - cyy - czz) - w2 / (2 * rho) * (cxx + cyy + czz - rho)
+        dy = dx + 3 * w1 / (2 * rho) * (cxx - cyy)
+        dz = dx + 3 * w1 / (2 * rho) * (cxx - czz)
+        return dx, dy, dz
+
+    def _add_galilean_correction(self, Cp, c, rho, u3, rates):
+        """Post-collision x2-y2, x2-z2, x2+y2+z2 plus the terms of galilean_correction.py:86-91."""
+        i1 = self._poly_row({(2, 0, 0): 1, (0, 2, 0): -1})
+        i2 = self._poly_row({(2, 0, 0): 1, (0, 0, 2): -1})
+        i3 = self._poly_row({(2, 0, 0): 1, (0, 2, 0): 1, (0, 0, 2): 1})
+        w1, w2 = rates[i1], rates[i3]
+        if not np.array_equal(w1, rates[i2]):
+            raise ValueError("Cumulants (x^2 - y^2) and (x^2 - z^2) must have the same relaxation rate")
+        dx, dy, dz = self._velocity_gradients(c, rho, w1, w2)
+        ux2, uy2, uz2 = u3[0] ** 2, u3[1] ** 2, u3[2] ** 2
+        Cp[i1] = Cp[i1] - 3 * rho * (1 - w1 / 2) * (ux2 * dx - uy2 * dy)
+        Cp[i2] = Cp[i2] - 3 * rho * (1 - w1 / 2) * (ux2 * dx - uz2 * dz)
+        Cp[i3] = Cp[i3] - 3 * rho * (1 - w2 / 2) * (ux2 * dx + uy2 * dy + uz2 * dz)
+
+    def _limited_rates(self, c, rho, rates):
+        """corr_rr_2..4 per cell: w + (1 - w) |C| / (rho lim + |C|) with C = c_xyy + c_xzz, c_xyy - c_xzz,
+        c_xyz (fourth_order_correction.py:185-209); every polynomial relaxed with that symbol takes it."""
+        lim = float(self.fourth[0])
+        sums = (self._c(c, (1, 2, 0)) + self._c(c, (1, 0, 2)), self._c(c, (1, 2, 0)) - self._c(c, (1, 0, 2)),
+                self._c(c, (1, 1, 1)))
+        out = rates.copy()
+        for k, (w, sc) in enumerate(zip(self.fourth_w, sums)):
+            lw = w + (1 - w) * np.abs(sc) / (rho * lim + np.abs(sc))
+            for i, r in enumerate(self.m.relaxation_rates):
+                if getattr(r, "name", None) == f"corr_rr_{2 + k}":
+                    out[i] = lw
+        return out
+
+    def _add_fourth_order_correction(self, Cp, c, rho, rates):
+        """Post-collision fourth-order polynomials plus the terms of fourth_order_correction.py:148-167."""
+        x2y2 = (2, 2, 0)
+        x2z2 = (2, 0, 2)
+        y2z2 = (0, 2, 2)
+        rows = [self._poly_row({x2y2: 1, x2z2: -2, y2z2: 1}), self._poly_row({x2y2: 1, x2z2: 1, y2z2: -2}),
+                self._poly_row({x2y2: 1, x2z2: 1, y2z2: 1}), self._poly_row({(2, 1, 1): 1}),
+                self._poly_row({(1, 2, 1): 1}), self._poly_row({(1, 1, 2): 1})]
+        ixy = self._poly_row({(1, 1, 0): 1})
+        ibulk = self._poly_row({(2, 0, 0): 1, (0, 2, 0): 1, (0, 0, 2): 1})
+        w1, w2 = rates[ixy], rates[ibulk]
+        w6, w7, w8 = rates[rows[0]], rates[rows[2]], rates[rows[3]]
+        a, b = self.fourth_ab
+        dxu, dyv, dzw = self._velocity_gradients(c, rho, w1, w2)
+        dxvdyu = -3 * w1 / rho * self._c(c, (1, 1, 0))
+        dxwdzu = -3 * w1 / rho * self._c(c, (1, 0, 1))
+        dywdzv = -3 * w1 / rho * self._c(c, (0, 1, 1))
+        pre = 1 / w1 - 0.5
+        Cp[rows[0]] = Cp[rows[0]] + 2 / 3 * pre * w6 * a * rho * (dxu - 2 * dyv + dzw)
+        Cp[rows[1]] = Cp[rows[1]] + 2 / 3 * pre * w6 * a * rho * (dxu + dyv - 2 * dzw)
+        Cp[rows[2]] = Cp[rows[2]] - 4 / 3 * pre * w7 * a * rho * (dxu + dyv + dzw)
+        Cp[rows[3]] = Cp[rows[3]] - 1 / 3 * pre * w8 * b * rho * dywdzv
+        Cp[rows[4]] = Cp[rows[4]] - 1 / 3 * pre * w8 * b * rho * dxwdzu
+        Cp[rows[5]] = Cp[rows[5]] - 1 / 3 * pre * w8 * b * rho * dxvdyu
 
     def _leibniz_terms(self, e):
         """Pick j with e_j > 0; return (j, list of (binomial, r)) for r <= e - e_j."""
@@ -396,6 +492,36 @@ class OracleCollision:
         return self._collide_central(f)
 
 
+def _fourth_order_ab(w1, w2):
+    """a, b of Geier et al. 2017 eq. 115-116 (fourth_order_correction.py:70-94)."""
+    den = (w1 - w2) * (w2 * (2 + 3 * w1) - 8 * w1)
+    a = (4 * w1 ** 2 + 2 * w1 * w2 * (w1 - 6) + w2 ** 2 * (w1 * (10 - 3 * w1) - 4)) / den
+    b = (4 * w1 * w2 * (9 * w1 - 16) - 4 * w1 ** 2 - 2 * w2 ** 2 * (2 + 9 * w1 * (w1 - 2))) / (3 * den)
+    return a, b
+
+
+def _fourth_order_omegas(w1, w2):
+    """Unlimited omega_3, omega_4, omega_5 of Geier et al. 2017 eq. 112-114
+    (fourth_order_correction.py:127-139)."""
+    o3 = 8 * (w1 - 2) * (w2 * (3 * w1 - 1) - 5 * w1) / (8 * (5 - 2 * w1) * w1 + w2 * (8 + w1 * (9 * w1 - 26)))
+    o4 = 8 * (w1 - 2) * (w1 + w2 * (3 * w1 - 7)) / (w2 * (56 - 42 * w1 + 9 * w1 ** 2) - 8 * w1)
+    o5 = 24 * (w1 - 2) * (4 * w1 ** 2 + w1 * w2 * (18 - 13 * w1) + w2 ** 2 * (2 + w1 * (6 * w1 - 11))) / \
+        (16 * w1 ** 2 * (w1 - 6) - 2 * w1 * w2 * (216 + 5 * w1 * (9 * w1 - 46)) +
+         w2 ** 2 * (w1 * (3 * w1 - 10) * (15 * w1 - 28) - 48))
+    return o3, o4, o5
+
+
+def corrections_from_config(lbm_config):
+    """OracleCollision keyword arguments for an LBMConfig's cumulant corrections
+    (creationfunctions.py:692-707: limiter 1e6 for True, bulk rate 1 when only the shear rate is given)."""
+    foc = lbm_config.fourth_order_correction
+    rr = list(lbm_config.relaxation_rates)
+    fourth = None
+    if foc:
+        fourth = (1e6 if foc is True else float(foc), rr[0], rr[1] if len(rr) > 1 else 1)
+    return {"galilean": bool(lbm_config.galilean_correction), "fourth_order": fourth}
+
+
 def _background_monomial(e, dim):
     """Raw monomial moment of the rest state rho=1, u=0 (Gaussian with cs2=1/3)."""
     v = 1.0
@@ -416,17 +542,18 @@ class OracleDomain:
     Arrays are numpy float64 with shape (Q, nz+2, ny+2, nx+2) for 3-D and
     (Q, ny+2, nx+2) for 2-D (the ``fzyx`` layout)."""
 
-    def __init__(self, method, shape, faces=None, pattern="pull", bindings=None, obstacle=None, boundaries=None):
+    def __init__(self, method, shape, faces=None, pattern="pull", bindings=None, obstacle=None, boundaries=None,
+                 collision=None):
         """``obstacle``: optional bool array [x, y, (z)] over the interior marking NoSlip cells
         (what ``set_boundary(NoSlip(), mask_callback=...)`` flags).  ``boundaries``: optional list of
         (flags, spec) with flags a bool array [x, y, (z)] over the ghost-inclusive domain and spec
         {"kind": "noslip"}, {"kind": "ubb", "velocity": u_w, "density": rho_w or None} or
         {"kind": "freeslip", "normal": n} (n points from the wall into the fluid),
         {"kind": "fixeddensity", "density": rho_w} or {"kind": "outflow", "normal": n} (n points out of the
-        fluid)."""
+        fluid).  ``collision``: extra OracleCollision keyword arguments (subgrid model, corrections)."""
         self.boundaries = list(boundaries or [])
         self._outflow_state = {}
-        self.coll = OracleCollision(method, bindings)
+        self.coll = OracleCollision(method, bindings, **(collision or {}))
         self.obstacle = None if obstacle is None else np.asarray(obstacle, dtype=bool)
         self.method = method
         self.D = method.stencil.D
@@ -725,8 +852,10 @@ def equilibrium_pdfs(method, rho, u, bindings=None):
     """Equilibrium populations for given (rho, u) fields, in the stored format
     (``macroscopic_value_kernels.py:29-45`` via ``conservedquantitycomputation.py:251-272``):
     the equilibrium velocity is u - F/(2 rho_ref) and collision with all rates 1
-    maps any state with these moments to the equilibrium."""
-    coll = OracleCollision(method, bindings)
+    maps any state with these moments to the equilibrium (so unbound rate symbols are set to 1)."""
+    bind = {name: 1.0 for name in method.relaxation_rate_symbols()}
+    bind.update(bindings or {})
+    coll = OracleCollision(method, bind)
     shape = rho.shape
     rho_f = rho.reshape(-1).astype(np.float64)
     u_f = u.reshape(u.shape[0], -1).astype(np.float64)

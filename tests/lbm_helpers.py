@@ -19,8 +19,8 @@ def seeded_state(method, shape, seed=0, du=0.02, drho=0.01, noise=1e-4):
     return f
 
 
-def oracle_run(method, shape, faces, pattern, f0, steps, bindings=None):
-    dom = OracleDomain(method, shape, faces=faces, pattern=pattern, bindings=bindings)
+def oracle_run(method, shape, faces, pattern, f0, steps, bindings=None, collision=None):
+    dom = OracleDomain(method, shape, faces=faces, pattern=pattern, bindings=bindings, collision=collision)
     if pattern == "aa":
         dom.set_pdfs_interior(f0[dom.inv])
     else:

@@ -1,0 +1,68 @@
+"""Host side of the cumulant corrections (no GPU): method creation with the fourth-order rate symbols, the
+checks lbmpy makes when it adds the corrections (creationfunctions.py:692-707, 835-846), and the kernel table
+against the oracle's independent restatement of the optimal parametrisation."""
+import numpy as np
+import pytest
+import sympy as sp
+
+import oracle.lbm_oracle as O
+from lbmpy_amd import LBMConfig, LBStencil, Stencil, Method, SubgridScaleModel, create_lb_method
+from lbmpy_amd.creationfunctions import create_lb_collision_rule
+from lbmpy_amd.kernel_tables import method_table, cumulant_corrections, CORRECTION_RATE_NAMES
+
+
+def _lc(**kw):
+    return LBMConfig(stencil=LBStencil(Stencil.D3Q27), method=Method.CUMULANT, compressible=True, **kw)
+
+
+def test_fourth_order_method_carries_rate_symbols():
+    m = create_lb_method(_lc(relaxation_rate=1.9, fourth_order_correction=True))
+    names = [r.name if hasattr(r, "name") else float(r) for r in m.relaxation_rates]
+    slots = [0.0] * 4 + [0] * 5 + [1] + [2] * 3 + [3] * 3 + [4] + [5] * 2 + [6] + [7] * 3 + [8] * 3 + [9]
+    want = [s if isinstance(s, float) else CORRECTION_RATE_NAMES[s] for s in slots]
+    assert names == want
+
+
+def test_fourth_order_needs_at_most_two_rates():
+    with pytest.raises(AssertionError):
+        create_lb_method(_lc(relaxation_rates=[1.9, 1.0, 1.0], fourth_order_correction=True))
+
+
+def test_collision_rule_appends_bulk_rate():
+    lc = _lc(relaxation_rate=1.9, fourth_order_correction=True)
+    create_lb_collision_rule(lbm_config=lc)
+    corr = cumulant_corrections(lc)
+    assert corr.fourth_order and corr.shear == 1.9 and corr.bulk == 1 and corr.limiter == 1e6
+
+
+def test_galilean_needs_d3q27_cumulant():
+    lc = LBMConfig(stencil=LBStencil(Stencil.D3Q19), method=Method.CUMULANT, compressible=True, relaxation_rate=1.8,
+                   galilean_correction=True)
+    with pytest.raises(ValueError):
+        create_lb_collision_rule(lbm_config=lc)
+
+
+@pytest.mark.parametrize("limiter", [True, 0.01])
+def test_table_matches_oracle_parametrisation(limiter):
+    lc = _lc(relaxation_rates=[sp.Symbol("omega"), 1.2], fourth_order_correction=limiter, galilean_correction=True)
+    m = create_lb_method(lc)
+    tab, _ = method_table(m, {"omega": 1.93}, corrections=cumulant_corrections(lc))
+    a, b = O._fourth_order_ab(1.93, 1.2)
+    w3, w4, w5 = O._fourth_order_omegas(1.93, 1.2)
+    np.testing.assert_allclose(tab[27:], [a, b, w3, w4, w5, 1e6 if limiter is True else limiter], rtol=1e-14)
+    np.testing.assert_allclose(tab[4:10], [1.93] * 5 + [1.2])
+    np.testing.assert_allclose(tab[10:17], [w3] * 3 + [w4] * 3 + [w5], rtol=1e-14)
+    np.testing.assert_array_equal(tab[17:27], 1.0)
+
+
+def test_plain_table_has_zero_correction_entries():
+    m = create_lb_method(_lc(relaxation_rate=1.9))
+    tab, _ = method_table(m, {})
+    assert tab.size == 33 and np.all(tab[27:] == 0.0)
+
+
+def test_fourth_order_with_smagorinsky_is_rejected():
+    from lbmpy_amd import create_lb_function, CreateKernelConfig, Target
+    lc = _lc(relaxation_rate=1.9, fourth_order_correction=True, subgrid_scale_model=SubgridScaleModel.SMAGORINSKY)
+    with pytest.raises(NotImplementedError):
+        create_lb_function(lbm_config=lc, config=CreateKernelConfig(target=Target.GPU))
