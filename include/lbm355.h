@@ -38,10 +38,29 @@ extern "C" {
 #define LBM355_FAMILY_CENTRAL 1  /* central-moment methods */
 #define LBM355_FAMILY_CUMULANT 2 /* cumulant methods (default polynomial cumulant set) */
 
-/* streaming patterns (lbmpy fieldaccess.py:66-75 pull, :136-157 AA) */
+/* streaming patterns (lbmpy fieldaccess.py:66-75 pull, :136-157 AA, :160-316 EsoTwist / EsoPull / EsoPush).
+ * Every in-place pattern alternates an EVEN and an ODD kernel on one array; the value after the name
+ * is the kernel.  Per direction i (c = c_i, inv = inv(i)) a kernel reads (cell offset, slot) and writes
+ * (cell offset, slot):
+ *   AA        EVEN  read (0, i)          write (0, inv)        ODD  read (-c, inv)        write (+c, i)
+ *   EsoTwist  EVEN  read (max(-c,0), i)  write (max(c,0), inv) ODD  read (max(-c,0), inv) write (max(c,0), i)
+ *   EsoPull   EVEN  E: read (-c, i) write (0, inv); O: read (0, i) write (+c, inv)
+ *             ODD   E: read (-c, inv) write (0, i); O: read (0, inv) write (+c, i)
+ *   EsoPush   EVEN  E: read (0, inv) write (+c, i); O: read (-c, inv) write (0, i)
+ *             ODD   E: read (0, i) write (+c, inv); O: read (-c, i) write (0, inv)
+ * where E are the directions at even positions of lbmpy's 'lehmann' ordering (the second of each
+ * +-pair, stencils.py:146-149, 214-218, 242-248) and O their inverses.  The Eso patterns and push run
+ * with periodic or ghost faces; walls are boundary links (lbm355_boundary_links). */
 #define LBM355_PULL 0
 #define LBM355_AA_EVEN 1
 #define LBM355_AA_ODD 2
+#define LBM355_ESOTWIST_EVEN 3
+#define LBM355_ESOTWIST_ODD 4
+#define LBM355_ESOPULL_EVEN 5
+#define LBM355_ESOPULL_ODD 6
+#define LBM355_ESOPUSH_EVEN 7
+#define LBM355_ESOPUSH_ODD 8
+#define LBM355_PUSH 9        /* two arrays: read (0, i) of src, write (+c, i) of dst (fieldaccess.py:78-87) */
 
 /* element types */
 #define LBM355_F64 0
@@ -73,7 +92,7 @@ extern "C" {
 typedef struct lbm355_sweep_desc {
   int32_t stencil;       /* LBM355_D2Q9 ... */
   int32_t family;        /* LBM355_FAMILY_* */
-  int32_t pattern;       /* LBM355_PULL / LBM355_AA_EVEN / LBM355_AA_ODD */
+  int32_t pattern;       /* LBM355_PULL, or the EVEN / ODD kernel of an in-place pattern */
   int32_t store_dtype;   /* element type of the PDF arrays */
   int32_t compute_dtype; /* arithmetic type: LBM355_F64 or LBM355_F32 */
   int32_t nx, ny, nz;    /* interior cells (nz = 1 for 2-D) */
@@ -121,8 +140,9 @@ int lbm355_reduce_conserved(const lbm355_sweep_desc* desc, const void* pdf, doub
  * lbstep.py:268-269).  density / velocity: FP64 device arrays laid out with desc->out_stride_*
  * (velocity component stride out_stride_c) and the same ghost-layer offset as the PDFs.  The
  * equilibrium velocity is u - F/(2 rho_ref) when a force model is present.  prev_pattern:
- * LBM355_PULL writes slot i; LBM355_AA_EVEN writes slot inv(i) (state "after an EVEN step", so
- * the next AA kernel is ODD).  table_f64: FP64 method table (moment family; ignored for cumulants).
+ * LBM355_PULL writes slot i; the EVEN kernel of an in-place pattern writes through that kernel's
+ * write accessor (state "after an EVEN step", so the next kernel is ODD; neighbour writes wrap on
+ * periodic axes and land in the ghost layer otherwise).  table_f64: FP64 method table (moment family; ignored for cumulants).
  * Computes in FP64 and rounds once to the storage type. */
 int lbm355_set_equilibrium(const lbm355_sweep_desc* desc, void* pdf, const double* density, const double* velocity,
                            const double* table_f64, int32_t prev_pattern, void* stream);
@@ -130,7 +150,8 @@ int lbm355_set_equilibrium(const lbm355_sweep_desc* desc, void* pdf, const doubl
 /* Density and velocity (u = j/rho_ref + F/(2 rho_ref)) of the populations the last kernel wrote,
  * read through its write accessor (lbmpy macroscopic_value_kernels.py:48-61 with
  * get_field_accesses :14-26): LBM355_PULL slot i at x, LBM355_AA_EVEN slot inv(i) at x,
- * LBM355_AA_ODD slot i at x + c_i (bounced into slot inv(i) of x at a NoSlip face).
+ * LBM355_AA_ODD slot i at x + c_i (bounced into slot inv(i) of x at a NoSlip face), the Eso kernels
+ * through their write accessors.
  * Either output may be NULL.  FP64 outputs, strides as for lbm355_set_equilibrium. */
 int lbm355_get_macroscopic(const lbm355_sweep_desc* desc, const void* pdf, double* density, double* velocity,
                            int32_t last_pattern, void* stream);
@@ -157,7 +178,8 @@ int lbm355_stream_copy(const void* src, void* dst, int64_t n_floats, int32_t non
 #define LBM355_BC_OUTFLOW_INIT 5 /* sets s0 = s1 = f_out(inv d) at x + c_d - n (before the first step) */
 
 typedef struct lbm355_boundary_desc {
-  int32_t stencil, pattern, store_dtype, kind; /* pattern: the kernel that ran last (PULL, AA_EVEN, AA_ODD) */
+  int32_t stencil, pattern, store_dtype, kind; /* pattern: the kernel that ran last (PULL, or an in-place
+                                                  pattern's EVEN / ODD kernel; the next one is the other) */
   int32_t nx, ny, nz, ghost_layers;            /* interior sizes (nz = 1 in 2-D) */
   int32_t periodic[3];                         /* 1: wrap this axis */
   int32_t flags;                               /* LBM355_FLAG_ZERO_CENTERED | LBM355_FLAG_COMPRESSIBLE (UBB) */

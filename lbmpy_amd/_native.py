@@ -17,6 +17,9 @@ c_int32, c_int64, c_double, c_void_p = ctypes.c_int32, ctypes.c_int64, ctypes.c_
 D2Q9, D3Q19, D3Q27 = 0, 1, 2
 FAMILY_MOMENT, FAMILY_CENTRAL, FAMILY_CUMULANT = 0, 1, 2
 PULL, AA_EVEN, AA_ODD = 0, 1, 2
+ESOTWIST_EVEN, ESOTWIST_ODD, ESOPULL_EVEN, ESOPULL_ODD, ESOPUSH_EVEN, ESOPUSH_ODD, PUSH = 3, 4, 5, 6, 7, 8, 9
+# EVEN kernel code of each in-place pattern (the ODD kernel is the next code)
+INPLACE_EVEN = {"aa": AA_EVEN, "esotwist": ESOTWIST_EVEN, "esopull": ESOPULL_EVEN, "esopush": ESOPUSH_EVEN}
 F64, F32, F16 = 0, 1, 2
 FACE_GHOST, FACE_PERIODIC, FACE_NOSLIP = 0, 1, 2
 FLAG_ZERO_CENTERED, FLAG_COMPRESSIBLE, FLAG_EQ_SHIFT, FLAG_FORCE, FLAG_SMAGORINSKY = 1, 2, 4, 8, 16

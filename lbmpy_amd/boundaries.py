@@ -157,6 +157,18 @@ class LatticeBoltzmannBoundaryHandling:
         self._groups = None
         self._faces = None
         self.version = 0           # incremented whenever the link lists are rebuilt
+        self._fuse = True
+
+    @property
+    def fuse_noslip_faces(self):
+        """Whether full NoSlip planes become fused sweep faces ('noslip') rather than boundary links."""
+        return self._fuse
+
+    @fuse_noslip_faces.setter
+    def fuse_noslip_faces(self, value):
+        if bool(value) != self._fuse:
+            self._fuse = bool(value)
+            self._dirty = True
 
     def _id_of(self, obj):
         for k, o in enumerate(self._objects):
@@ -210,7 +222,7 @@ class LatticeBoltzmannBoundaryHandling:
             lo = [slice(None)] * D
             hi = [slice(None)] * D
             lo[a], hi[a] = 0, -1
-            if g and is_noslip[tuple(lo)].all() and is_noslip[tuple(hi)].all():
+            if self._fuse and g and is_noslip[tuple(lo)].all() and is_noslip[tuple(hi)].all():
                 faces.append("noslip")
             else:
                 faces.append("ghost")

@@ -165,6 +165,7 @@ int lbm355_batch_create(const lbm355_sweep_desc* descs, const void* const* pdf_s
       return berr(LBM355_ERR_ARG, "batch members must share stencil, family, streaming pattern and dtypes");
     if (d.flags & (LBM355_FLAG_SMAGORINSKY | LBM355_FLAG_LEARNED_CLOSURE))
       return berr(LBM355_ERR_UNSUPPORTED, "subgrid models are not batched");
+    if (table_pattern(d.pattern)) return berr(LBM355_ERR_UNSUPPORTED, "batched ensembles stream with pull or AA");
     if (d.flags & (LBM355_FLAG_GALILEAN | LBM355_FLAG_FOURTH_ORDER))
       return berr(LBM355_ERR_UNSUPPORTED, "the cumulant corrections are not batched");
     if (!pdf_src[k] || !pdf_dst[k]) return berr(LBM355_ERR_ARG, "NULL member PDF pointer");

@@ -53,6 +53,102 @@ __host__ __device__ constexpr int dir_index(int cx, int cy, int cz) {
   return -1;
 }
 
+// ---------------------------------------------------------------- streaming patterns
+// lbmpy's 'lehmann' direction orderings (stencils.py:146-149, 214-218, 242-248): +-pairs at positions
+// (1, 2), (3, 4), ...; EsoPull / EsoPush treat the second direction of each pair ("even" position) and
+// the first differently.
+struct Lehmann2D {
+  static constexpr int n = 9;
+  static constexpr int c[9][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0},
+                                  {1, 1, 0}, {-1, -1, 0}, {1, -1, 0}, {-1, 1, 0}};
+};
+struct Lehmann3D {
+  static constexpr int n = 27;   // D3Q19 uses the first 19 entries
+  static constexpr int c[27][3] = {
+      {0, 0, 0},  {1, 0, 0},   {-1, 0, 0},  {0, 1, 0},  {0, -1, 0},  {0, 0, 1},   {0, 0, -1},
+      {1, 1, 0},  {-1, -1, 0}, {1, 0, 1},   {-1, 0, -1}, {0, 1, 1},  {0, -1, -1}, {1, -1, 0},
+      {-1, 1, 0}, {1, 0, -1},  {-1, 0, 1},  {0, 1, -1}, {0, -1, 1},  {1, 1, 1},   {-1, -1, -1},
+      {1, 1, -1}, {-1, -1, 1}, {1, -1, 1},  {-1, 1, -1}, {-1, 1, 1}, {1, -1, -1}};
+};
+
+// true when direction i of S (walberla order) sits at an even position of the lehmann ordering
+template <class S>
+__host__ __device__ constexpr bool lehmann_even(int i) {
+  for (int k = 0; k < (S::D == 2 ? Lehmann2D::n : S::Q); ++k) {
+    const int* l = S::D == 2 ? Lehmann2D::c[k] : Lehmann3D::c[k];
+    if (l[0] == S::c[i][0] && l[1] == S::c[i][1] && l[2] == S::c[i][2]) return k % 2 == 0;
+  }
+  return false;
+}
+
+// Where a kernel of pattern PAT reads (r, rslot) and writes (w, wslot) population i of the cell at x:
+// cell offsets r, w in {-1, 0, 1}^3 (the per-pattern table in include/lbm355.h).
+struct Access {
+  int r[3], rslot, w[3], wslot;
+};
+
+__host__ __device__ constexpr int pos_part(int v) { return v > 0 ? v : 0; }
+
+template <class S>
+__host__ __device__ constexpr Access pattern_access(int pat, int i) {
+  const int c0 = S::c[i][0], c1 = S::c[i][1], c2 = S::c[i][2];
+  const int ii = S::inv[i];
+  const bool e = lehmann_even<S>(i);
+  Access a = {{0, 0, 0}, i, {0, 0, 0}, i};
+  auto minus = [&](int(&o)[3]) { o[0] = -c0; o[1] = -c1; o[2] = -c2; };
+  auto plus = [&](int(&o)[3]) { o[0] = c0; o[1] = c1; o[2] = c2; };
+  switch (pat) {
+    case LBM355_PULL:                  // read (-c, i), write (0, i)
+      minus(a.r);
+      break;
+    case LBM355_PUSH:                  // read (0, i), write (+c, i)
+      plus(a.w);
+      break;
+    case LBM355_AA_EVEN:               // read (0, i), write (0, inv)
+      a.wslot = ii;
+      break;
+    case LBM355_AA_ODD:                // read (-c, inv), write (+c, i)
+      minus(a.r);
+      a.rslot = ii;
+      plus(a.w);
+      break;
+    case LBM355_ESOTWIST_EVEN:         // read (max(-c,0), i), write (max(c,0), inv)
+    case LBM355_ESOTWIST_ODD:          // read (max(-c,0), inv), write (max(c,0), i)
+      a.r[0] = pos_part(-c0); a.r[1] = pos_part(-c1); a.r[2] = pos_part(-c2);
+      a.w[0] = pos_part(c0); a.w[1] = pos_part(c1); a.w[2] = pos_part(c2);
+      if (pat == LBM355_ESOTWIST_EVEN) a.wslot = ii; else a.rslot = ii;
+      break;
+    case LBM355_ESOPULL_EVEN:          // E: read (-c, i), write (0, inv); O: read (0, i), write (+c, inv)
+    case LBM355_ESOPULL_ODD:           // E: read (-c, inv), write (0, i); O: read (0, inv), write (+c, i)
+      if (e) minus(a.r); else plus(a.w);
+      if (pat == LBM355_ESOPULL_EVEN) a.wslot = ii; else a.rslot = ii;
+      break;
+    case LBM355_ESOPUSH_EVEN:          // E: read (0, inv), write (+c, i); O: read (-c, inv), write (0, i)
+    case LBM355_ESOPUSH_ODD:           // E: read (0, i), write (+c, inv); O: read (-c, i), write (0, inv)
+      if (e) plus(a.w); else minus(a.r);
+      if (pat == LBM355_ESOPUSH_EVEN) a.rslot = ii; else a.wslot = ii;
+      break;
+  }
+  return a;
+}
+
+// the whole table of one pattern, for compile-time lookups in unrolled loops
+template <class S, int PAT>
+struct AccessTable {
+  Access a[S::Q];
+  __host__ __device__ constexpr AccessTable() : a() {
+    for (int i = 0; i < S::Q; ++i) a[i] = pattern_access<S>(PAT, i);
+  }
+};
+
+// the kernel that follows PAT (in-place patterns alternate EVEN and ODD)
+__host__ __device__ constexpr int next_pattern(int pat) {
+  return (pat == LBM355_PULL || pat == LBM355_PUSH) ? pat : (pat % 2 == 1 ? pat + 1 : pat - 1);
+}
+__host__ __device__ constexpr bool is_eso(int pat) { return pat >= LBM355_ESOTWIST_EVEN && pat <= LBM355_ESOPUSH_ODD; }
+// patterns swept through the access table (sweep_eso.hip): the Eso patterns and push
+__host__ __device__ constexpr bool table_pattern(int pat) { return is_eso(pat) || pat == LBM355_PUSH; }
+
 // lattice weights of the rest state (maxwellian_equilibrium.py:24-61)
 template <class S>
 __host__ __device__ constexpr double weight(int i) {

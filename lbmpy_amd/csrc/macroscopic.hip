@@ -2,11 +2,13 @@
 // reference's LatticeBoltzmannStep runs on the CPU after copying the field back (lbstep.py:311-314,
 // 444-449).
 //
-// setter: PDFs := equilibrium(rho, u - F/(2 rho_ref)) in storage form, written through the
-//         accessor of the *previous* time step (pull: slot i; AA with previous EVEN: slot inv(i)).
+// setter: PDFs := equilibrium(rho, u - F/(2 rho_ref)) in storage form, written through the write
+//         accessor of the *previous* time step (pull: slot i; an in-place pattern with previous EVEN:
+//         that kernel's write accessor, e.g. AA slot inv(i) at x, EsoTwist slot inv(i) at x + max(c_i, 0)).
 // getter: rho, u = j / rho_ref + F/(2 rho_ref) from the populations the last kernel wrote,
 //         read through that kernel's write accessor (pull: slot i at x; AA EVEN: slot inv(i) at x;
-//         AA ODD: slot i at x + c_i, bounced back into slot inv(i) at a wall).
+//         AA ODD: slot i at x + c_i, bounced back into slot inv(i) at a wall; Eso: their tables).
+// Neighbour accesses wrap on periodic axes and use the ghost layer on ghost faces.
 // Both compute in FP64 regardless of the storage type.
 #include "lbm_common.h"
 #include "collide_moment.h"
@@ -42,7 +44,8 @@ __device__ __forceinline__ double phi(int c, double u) {
 template <class S, int FAM, int PAT, class TS>
 __global__ void set_equilibrium_kernel(TS* pdf, const double* rho_in, const double* u_in, const double* tab, int nx,
                                        int ny, int nz, int g, int64_t sy, int64_t sz, int64_t sq, int64_t osy,
-                                       int64_t osz, int64_t osc, int flags, double fx, double fy, double fz) {
+                                       int64_t osz, int64_t osc, int flags, double fx, double fy, double fz,
+                                       int face_x, int face_y, int face_z) {
   constexpr int Q = S::Q, D = S::D;
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= nx) return;
@@ -69,11 +72,18 @@ __global__ void set_equilibrium_kernel(TS* pdf, const double* rho_in, const doub
       feq[i] = zc ? fi - weight<S>(i) : fi;
     }
   }
-  const int64_t c = (int64_t)(x + g) + (int64_t)(y + g) * sy + (D == 3 ? (int64_t)(z + g) * sz : 0);
+  const AxisNb ax = axis_neighbours(x, nx, g, face_x);
+  const AxisNb ay = axis_neighbours(y, ny, g, face_y);
+  AxisNb az = {0, 0, false, false};
+  if (D == 3) az = axis_neighbours(z, nz, g, face_z);
+  const int64_t X[3] = {ax.lo, x + g, ax.hi};
+  const int64_t Y[3] = {ay.lo * sy, (y + g) * sy, ay.hi * sy};
+  const int64_t Z[3] = {D == 3 ? az.lo * sz : 0, D == 3 ? (z + g) * sz : 0, D == 3 ? az.hi * sz : 0};
+  constexpr AccessTable<S, PAT> acc{};
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
-    const int slot = PAT == LBM355_PULL ? i : S::inv[i];
-    pdf[slot * sq + c] = (TS)feq[i];
+    const Access& t = acc.a[i];
+    pdf[t.wslot * sq + X[1 + t.w[0]] + Y[1 + t.w[1]] + Z[1 + t.w[2]]] = (TS)feq[i];
   }
 }
 
@@ -99,10 +109,9 @@ __global__ void get_macroscopic_kernel(const TS* pdf, double* rho_out, double* u
   for (int i = 0; i < Q; ++i) {
     const int cx = S::c[i][0], cy = S::c[i][1], cz = S::c[i][2];
     double v;
-    if (PAT == LBM355_PULL) {
-      v = (double)pdf[i * sq + center];
-    } else if (PAT == LBM355_AA_EVEN) {
-      v = (double)pdf[S::inv[i] * sq + center];
+    if (PAT != LBM355_AA_ODD) {
+      const Access t = pattern_access<S>(PAT, i);
+      v = (double)pdf[t.wslot * sq + X[1 + t.w[0]] + Y[1 + t.w[1]] + Z[1 + t.w[2]]];
     } else {
       const bool wall = (cx == -1 && ax.wall_lo) || (cx == 1 && ax.wall_hi) || (cy == -1 && ay.wall_lo) ||
                         (cy == 1 && ay.wall_hi) || (cz == -1 && az.wall_lo) || (cz == 1 && az.wall_hi);
@@ -140,11 +149,15 @@ int set_eq_launch(const lbm355_sweep_desc* d, void* pdf, const double* rho, cons
                   int prev_pattern, hipStream_t s) {
   const int bx = d->nx >= 256 ? 256 : ((d->nx + 63) / 64) * 64;
   dim3 block(bx), grid((d->nx + bx - 1) / bx, d->ny, d->nz);
-  auto k = prev_pattern == LBM355_PULL ? set_equilibrium_kernel<S, FAM, LBM355_PULL, TS>
-                                       : set_equilibrium_kernel<S, FAM, LBM355_AA_EVEN, TS>;
+  auto k = prev_pattern == LBM355_PULL            ? set_equilibrium_kernel<S, FAM, LBM355_PULL, TS>
+           : prev_pattern == LBM355_PUSH          ? set_equilibrium_kernel<S, FAM, LBM355_PUSH, TS>
+           : prev_pattern == LBM355_AA_EVEN       ? set_equilibrium_kernel<S, FAM, LBM355_AA_EVEN, TS>
+           : prev_pattern == LBM355_ESOTWIST_EVEN ? set_equilibrium_kernel<S, FAM, LBM355_ESOTWIST_EVEN, TS>
+           : prev_pattern == LBM355_ESOPULL_EVEN  ? set_equilibrium_kernel<S, FAM, LBM355_ESOPULL_EVEN, TS>
+                                                  : set_equilibrium_kernel<S, FAM, LBM355_ESOPUSH_EVEN, TS>;
   hipLaunchKernelGGL(k, grid, block, 0, s, (TS*)pdf, rho, u, tab, d->nx, d->ny, d->nz, d->ghost_layers, d->stride_y,
                      d->stride_z, d->stride_q, d->out_stride_y, d->out_stride_z, d->out_stride_c, d->flags,
-                     d->force[0], d->force[1], d->force[2]);
+                     d->force[0], d->force[1], d->force[2], d->face[0], d->face[1], d->face[2]);
   return hipGetLastError() == hipSuccess ? 0 : fail2(LBM355_ERR_LAUNCH, "setter launch failed");
 }
 
@@ -153,9 +166,19 @@ int get_launch(const lbm355_sweep_desc* d, const void* pdf, double* rho, double*
                hipStream_t s) {
   const int bx = d->nx >= 256 ? 256 : ((d->nx + 63) / 64) * 64;
   dim3 block(bx), grid((d->nx + bx - 1) / bx, d->ny, d->nz);
-  auto k = last_pattern == LBM355_PULL      ? get_macroscopic_kernel<S, LBM355_PULL, TS>
-           : last_pattern == LBM355_AA_EVEN ? get_macroscopic_kernel<S, LBM355_AA_EVEN, TS>
-                                            : get_macroscopic_kernel<S, LBM355_AA_ODD, TS>;
+  decltype(&get_macroscopic_kernel<S, LBM355_PULL, TS>) k = nullptr;
+  switch (last_pattern) {
+    case LBM355_PULL: k = get_macroscopic_kernel<S, LBM355_PULL, TS>; break;
+    case LBM355_AA_EVEN: k = get_macroscopic_kernel<S, LBM355_AA_EVEN, TS>; break;
+    case LBM355_AA_ODD: k = get_macroscopic_kernel<S, LBM355_AA_ODD, TS>; break;
+    case LBM355_ESOTWIST_EVEN: k = get_macroscopic_kernel<S, LBM355_ESOTWIST_EVEN, TS>; break;
+    case LBM355_ESOTWIST_ODD: k = get_macroscopic_kernel<S, LBM355_ESOTWIST_ODD, TS>; break;
+    case LBM355_ESOPULL_EVEN: k = get_macroscopic_kernel<S, LBM355_ESOPULL_EVEN, TS>; break;
+    case LBM355_ESOPULL_ODD: k = get_macroscopic_kernel<S, LBM355_ESOPULL_ODD, TS>; break;
+    case LBM355_ESOPUSH_EVEN: k = get_macroscopic_kernel<S, LBM355_ESOPUSH_EVEN, TS>; break;
+    case LBM355_ESOPUSH_ODD: k = get_macroscopic_kernel<S, LBM355_ESOPUSH_ODD, TS>; break;
+    default: k = get_macroscopic_kernel<S, LBM355_PUSH, TS>; break;
+  }
   hipLaunchKernelGGL(k, grid, block, 0, s, (const TS*)pdf, rho, u, d->nx, d->ny, d->nz, d->ghost_layers, d->stride_y,
                      d->stride_z, d->stride_q, d->out_stride_y, d->out_stride_z, d->out_stride_c, d->face[0],
                      d->face[1], d->face[2], d->flags, d->force[0], d->force[1], d->force[2]);
@@ -184,8 +207,13 @@ int lbm355_set_equilibrium(const lbm355_sweep_desc* d, void* pdf, const double* 
                            const double* table_f64, int32_t prev_pattern, void* stream) {
   if (!d || !pdf || !density || !velocity) return fail2(LBM355_ERR_ARG, "NULL argument");
   if (d->store_dtype < 0 || d->store_dtype > 2) return fail2(LBM355_ERR_ARG, "bad dtype");
-  if (prev_pattern != LBM355_PULL && prev_pattern != LBM355_AA_EVEN)
-    return fail2(LBM355_ERR_ARG, "setter writes through the pull or AA-EVEN accessor");
+  if (prev_pattern != LBM355_PULL && prev_pattern != LBM355_PUSH && prev_pattern != LBM355_AA_EVEN &&
+      prev_pattern != LBM355_ESOTWIST_EVEN && prev_pattern != LBM355_ESOPULL_EVEN && prev_pattern != LBM355_ESOPUSH_EVEN)
+    return fail2(LBM355_ERR_ARG, "setter writes through the pull / push accessor or an EVEN kernel's accessor");
+  for (int a = 0; a < (d->stencil == LBM355_D2Q9 ? 2 : 3); ++a)
+    if (d->face[a] < 0 || d->face[a] > 2 || (d->face[a] == LBM355_FACE_GHOST && d->ghost_layers < 1) ||
+        (table_pattern(prev_pattern) && d->face[a] == LBM355_FACE_NOSLIP))
+      return fail2(LBM355_ERR_ARG, "bad face mode (Eso patterns: periodic or ghost faces)");
   hipStream_t s = (hipStream_t)stream;
   if (d->family == LBM355_FAMILY_MOMENT) {
     if (!table_f64) return fail2(LBM355_ERR_ARG, "moment family needs the FP64 method table");
@@ -205,7 +233,10 @@ int lbm355_get_macroscopic(const lbm355_sweep_desc* d, const void* pdf, double* 
                            int32_t last_pattern, void* stream) {
   if (!d || !pdf) return fail2(LBM355_ERR_ARG, "NULL argument");
   if (d->store_dtype < 0 || d->store_dtype > 2) return fail2(LBM355_ERR_ARG, "bad dtype");
-  if (last_pattern < 0 || last_pattern > 2) return fail2(LBM355_ERR_ARG, "bad pattern");
+  if (last_pattern < LBM355_PULL || last_pattern > LBM355_PUSH) return fail2(LBM355_ERR_ARG, "bad pattern");
+  if (table_pattern(last_pattern))
+    for (int a = 0; a < (d->stencil == LBM355_D2Q9 ? 2 : 3); ++a)
+      if (d->face[a] == LBM355_FACE_NOSLIP) return fail2(LBM355_ERR_ARG, "Eso patterns have no fused NoSlip faces");
   hipStream_t s = (hipStream_t)stream;
   if (d->stencil == LBM355_D2Q9) return get_dtype<D2Q9>(d, pdf, density, velocity, last_pattern, s);
   if (d->stencil == LBM355_D3Q19) return get_dtype<D3Q19>(d, pdf, density, velocity, last_pattern, s);

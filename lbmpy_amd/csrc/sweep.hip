@@ -6,6 +6,7 @@
 //   pull    read  src[i][x - c_i],        write dst[i][x]           (fieldaccess.py:66-75)
 //   AA even read  f[i][x],                write f[inv i][x]         (fieldaccess.py:136-145)
 //   AA odd  read  f[inv i][x - c_i],      write f[i][x + c_i]       (fieldaccess.py:148-157)
+// (the in-place Eso patterns are in sweep_eso.hip)
 // Periodic axes wrap inside the index math (PeriodicTwoFieldsAccessor semantics,
 // fieldaccess.py:90-133), NoSlip faces bounce back in place of the separate boundary kernel
 // (boundaries/boundaryconditions.py:104-139): a pull read from a wall cell returns
@@ -219,21 +220,25 @@ __device__ __forceinline__ int64_t bc_out(const BoundaryArgs& b, int i, int cx, 
       cy = ty;
       cz = tz;
     }
-  } else if (PAT == LBM355_AA_EVEN) {
-    slot = S::inv[i];
+  } else {
+    // the write accessor of the kernel that ran last
+    const Access t = pattern_access<S>(PAT, i);
+    slot = t.wslot;
+    cx += t.w[0];
+    cy += t.w[1];
+    cz += t.w[2];
   }
   return slot * b.sq + wrap_axis(cx, b.n[0], b.g, b.periodic[0]) + wrap_axis(cy, b.n[1], b.g, b.periodic[1]) * b.sy +
          (S::D == 3 ? wrap_axis(cz, b.n[2], b.g, b.periodic[2]) * b.sz : 0);
 }
+// the read accessor of the kernel that runs next
 template <class S, int PAT>
 __device__ __forceinline__ int64_t bc_in(const BoundaryArgs& b, int i, int cx, int cy, int cz) {
-  int slot = i;
-  if (PAT != LBM355_AA_ODD) {
-    cx -= S::c[i][0];
-    cy -= S::c[i][1];
-    cz -= S::c[i][2];
-    if (PAT == LBM355_AA_EVEN) slot = S::inv[i];
-  }
+  const Access t = pattern_access<S>(next_pattern(PAT), i);
+  const int slot = t.rslot;
+  cx += t.r[0];
+  cy += t.r[1];
+  cz += t.r[2];
   return slot * b.sq + wrap_axis(cx, b.n[0], b.g, b.periodic[0]) + wrap_axis(cy, b.n[1], b.g, b.periodic[1]) * b.sy +
          (S::D == 3 ? wrap_axis(cz, b.n[2], b.g, b.periodic[2]) * b.sz : 0);
 }
@@ -419,7 +424,7 @@ int xshift_dirs(int stencil) { return stencil == LBM355_D2Q9 ? 6 : (stencil == L
 
 // LDS row shifting applies when a block can span a whole periodic row
 bool use_rowlds(const lbm355_sweep_desc* d, size_t* lds_bytes) {
-  if (d->pattern == LBM355_AA_EVEN || d->face[0] != LBM355_FACE_PERIODIC) return false;
+  if (d->pattern == LBM355_AA_EVEN || table_pattern(d->pattern) || d->face[0] != LBM355_FACE_PERIODIC) return false;
   if (d->nx % 64 != 0 || d->nx > 512) return false;
   const size_t bytes = (size_t)xshift_dirs(d->stencil) * d->nx * elem_size(d->store_dtype);
   if (bytes > 65536) return false;
@@ -512,7 +517,14 @@ int validate(const lbm355_sweep_desc* d) {
   if (d->compute_dtype != LBM355_F64 && d->compute_dtype != LBM355_F32)
     return fail(LBM355_ERR_ARG, "compute dtype must be F64 or F32");
   if (d->store_dtype < 0 || d->store_dtype > 2) return fail(LBM355_ERR_ARG, "bad store dtype");
-  if (d->pattern < 0 || d->pattern > 2) return fail(LBM355_ERR_ARG, "bad pattern");
+  if (d->pattern < LBM355_PULL || d->pattern > LBM355_PUSH) return fail(LBM355_ERR_ARG, "bad pattern");
+  if (table_pattern(d->pattern)) {
+    for (int a = 0; a < D; ++a)
+      if (d->face[a] == LBM355_FACE_NOSLIP)
+        return fail(LBM355_ERR_UNSUPPORTED, "Eso and push patterns take walls as boundary links, not fused NoSlip faces");
+    if (d->flags & LBM355_FLAG_LEARNED_CLOSURE)
+      return fail(LBM355_ERR_UNSUPPORTED, "the learned closure runs with pull or AA streaming");
+  }
   if (!d->tables) return fail(LBM355_ERR_ARG, "method table pointer is NULL");
   if ((d->family == LBM355_FAMILY_MOMENT || d->family == LBM355_FAMILY_CENTRAL) &&
       (d->n_active < 0 || d->n_active > Q))
@@ -522,6 +534,7 @@ int validate(const lbm355_sweep_desc* d) {
 
 template <class TC>
 int launch_sweep(const lbm355_sweep_desc* d, const void* src, void* dst, void* rho, void* u, hipStream_t s) {
+  if (table_pattern(d->pattern)) return lbm355_launch_sweep_eso(d, src, dst, rho, u, s);
   if (d->store_dtype == LBM355_F16) {
     const int rc = lbm355_launch_sweep_h2(d, src, dst, rho, u, s);
     if (rc != -1) return rc;
@@ -597,9 +610,23 @@ int launch_bc_kind(const lbm355_boundary_desc* d, T* pdf, const int32_t* links, 
 template <class S, class T>
 int launch_bc(const lbm355_boundary_desc* d, void* pdf, const int32_t* links, int64_t n, double* vel,
               const BoundaryArgs& b, hipStream_t s) {
-  if (d->pattern == LBM355_PULL) return launch_bc_kind<S, LBM355_PULL, T>(d, (T*)pdf, links, n, vel, b, s);
-  if (d->pattern == LBM355_AA_EVEN) return launch_bc_kind<S, LBM355_AA_EVEN, T>(d, (T*)pdf, links, n, vel, b, s);
-  return launch_bc_kind<S, LBM355_AA_ODD, T>(d, (T*)pdf, links, n, vel, b, s);
+#define LBM_BC(P) \
+  case P:          \
+    return launch_bc_kind<S, P, T>(d, (T*)pdf, links, n, vel, b, s);
+  switch (d->pattern) {
+    LBM_BC(LBM355_PULL)
+    LBM_BC(LBM355_AA_EVEN)
+    LBM_BC(LBM355_AA_ODD)
+    LBM_BC(LBM355_ESOTWIST_EVEN)
+    LBM_BC(LBM355_ESOTWIST_ODD)
+    LBM_BC(LBM355_ESOPULL_EVEN)
+    LBM_BC(LBM355_ESOPULL_ODD)
+    LBM_BC(LBM355_ESOPUSH_EVEN)
+    LBM_BC(LBM355_ESOPUSH_ODD)
+    LBM_BC(LBM355_PUSH)
+  }
+#undef LBM_BC
+  return fail(LBM355_ERR_ARG, "bad pattern");
 }
 
 template <class S>
@@ -640,8 +667,10 @@ int lbm355_sweep(const lbm355_sweep_desc* d, const void* pdf_src, void* pdf_dst,
   int rc = validate(d);
   if (rc) return rc;
   if (!pdf_src || !pdf_dst) return fail(LBM355_ERR_ARG, "PDF pointers must not be NULL");
-  if (d->pattern != LBM355_PULL && pdf_src != pdf_dst) return fail(LBM355_ERR_ARG, "AA needs pdf_src == pdf_dst");
-  if (d->pattern == LBM355_PULL && pdf_src == pdf_dst) return fail(LBM355_ERR_ARG, "pull needs two arrays");
+  const bool two_arrays = d->pattern == LBM355_PULL || d->pattern == LBM355_PUSH;
+  if (!two_arrays && pdf_src != pdf_dst)
+    return fail(LBM355_ERR_ARG, "in-place patterns (AA, Eso) need pdf_src == pdf_dst");
+  if (two_arrays && pdf_src == pdf_dst) return fail(LBM355_ERR_ARG, "pull and push need two arrays");
   if (d->family == LBM355_FAMILY_CUMULANT && d->stencil != LBM355_D3Q27)
     return fail(LBM355_ERR_UNSUPPORTED, "cumulant sweep is implemented for D3Q27");
   if ((d->flags & LBM355_FLAG_SMAGORINSKY) && d->family != LBM355_FAMILY_CUMULANT)
@@ -696,7 +725,9 @@ int lbm355_boundary_links(const lbm355_boundary_desc* d, void* pdf, const int32_
   if (!d) return fail(LBM355_ERR_ARG, "desc is NULL");
   if (n_links == 0) return LBM355_OK;
   if (!pdf || !links || n_links < 0) return fail(LBM355_ERR_ARG, "bad pointer or link count");
-  if (d->pattern < 0 || d->pattern > 2) return fail(LBM355_ERR_ARG, "bad pattern");
+  if (d->pattern < LBM355_PULL || d->pattern > LBM355_PUSH) return fail(LBM355_ERR_ARG, "bad pattern");
+  if (table_pattern(d->pattern) && (d->fused_noslip[0] || d->fused_noslip[1] || d->fused_noslip[2]))
+    return fail(LBM355_ERR_ARG, "fused NoSlip faces exist with pull and AA streaming only");
   if (d->store_dtype < 0 || d->store_dtype > 2) return fail(LBM355_ERR_ARG, "bad dtype");
   if (d->kind < LBM355_BC_NOSLIP || d->kind > LBM355_BC_OUTFLOW_INIT) return fail(LBM355_ERR_ARG, "bad boundary kind");
   if ((d->kind == LBM355_BC_OUTFLOW || d->kind == LBM355_BC_OUTFLOW_INIT) && !link_velocity)

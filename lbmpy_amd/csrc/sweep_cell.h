@@ -135,6 +135,29 @@ __device__ __forceinline__ void sweep_cell(const SweepArgs<TC>& a, int x, int y,
       WALLS && (ax.wall_lo || ax.wall_hi || ay.wall_lo || ay.wall_hi || az.wall_lo || az.wall_hi);
 
   TC f[Q];
+  if constexpr (table_pattern(PAT)) {
+    // Eso patterns and push: every access from the pattern's table (periodic wrap or ghost layer in X/Y/Z).
+    // In place, a cell reads exactly the locations it writes, so one thread per cell needs no synchronisation.
+    // Accesses without an x offset are row-aligned and non-temporal.
+    constexpr AccessTable<S, PAT> acc{};
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const Access& t = acc.a[i];
+      const TS* p = src + t.rslot * sq + X[1 + t.r[0]] + Y[1 + t.r[1]] + Z[1 + t.r[2]];
+      f[i] = t.r[0] == 0 ? load_nt<TS, TC>(p) : load_as<TS, TC>(p);
+    }
+    collide_cell<S, FAM, TC>(f, a, x, y, z);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const Access& t = acc.a[i];
+      TS* p = dst + t.wslot * sq + X[1 + t.w[0]] + Y[1 + t.w[1]] + Z[1 + t.w[2]];
+      if (t.w[0] == 0)
+        store_nt<TS, TC>(p, f[i]);
+      else
+        store_as<TS, TC>(p, f[i]);
+    }
+    return;
+  }
   // pull reads src[i] at x - c_i; AA odd reads slot inv(i) there.  At a wall the
   // bounce-back value sits in the cell itself: pull slot inv(i), AA odd slot i.
   auto load = [&](auto walls) {
@@ -228,6 +251,10 @@ inline SweepArgs<TC> make_args(const lbm355_sweep_desc* d, const void* src, void
 
 // checks a sweep descriptor (sizes, strides, dtypes, table pointer); defined in sweep.hip
 int lbm355_validate_desc(const lbm355_sweep_desc* d);
+
+// sweeps of the Eso patterns and push (sweep_eso.hip); the descriptor has been validated
+int lbm355_launch_sweep_eso(const lbm355_sweep_desc* d, const void* src, void* dst, void* rho, void* u,
+                            hipStream_t s);
 
 // row-staged FP16 sweep (sweep_h2.hip): launches and returns a status when it applies, else returns -1
 int lbm355_launch_sweep_h2(const lbm355_sweep_desc* d, const void* src, void* dst, void* rho, void* u,

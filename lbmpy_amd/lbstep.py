@@ -12,8 +12,10 @@ Differences, all deliberate: the target is always the GPU (this package has
 no CPU sweep); PDF fields are PyTorch tensors that stay resident in HBM; the
 macroscopic setter/getter run on the GPU (the reference compiles them for the
 CPU, lbstep.py:444-449); periodicity is built into the sweep instead of ghost
-copies; in-place AA streaming is available (``streaming_pattern='aa'``), which
-the reference's fixed time loop does not support.
+copies; besides pull, the time loop runs push and the in-place patterns 'aa',
+'esotwist', 'esopull' and 'esopush' (EVEN/ODD kernel pairs, ODD first after
+the setter, boundaries between the kernels as lbmpy's BetweenTimestepsIndexing
+addresses them), which the reference's fixed time loop does not support.
 """
 import ctypes
 import time
@@ -82,8 +84,12 @@ class LatticeBoltzmannStep:
         self._lbm_optimisation = lbm_optimisation
         self._config = config
         self._pattern = lbm_config.streaming_pattern
-        if self._pattern not in ("pull", "aa"):
-            raise NotImplementedError(f"streaming pattern {self._pattern!r}")
+        if self._pattern not in ("pull", "push") + tuple(nat.INPLACE_EVEN):
+            raise ValueError("Invalid value of parameter 'streaming_pattern'.", self._pattern)
+        self._two_arrays = self._pattern in ("pull", "push")
+        # kernel codes: (the single kernel,) for pull / push, (EVEN, ODD) for in-place patterns
+        self._codes = ((nat.PULL,) if self._pattern == "pull" else (nat.PUSH,)) if self._two_arrays else \
+            (nat.INPLACE_EVEN[self._pattern], nat.INPLACE_EVEN[self._pattern] + 1)
 
         from dataclasses import replace
         lc = replace(lbm_config, field_name=self._pdf_arr_name, temporary_field_name=self._tmp_arr_name,
@@ -91,7 +97,7 @@ class LatticeBoltzmannStep:
         cfg = replace(config, ghost_layers=1)
         if lbm_kernel is not None:
             self._kernels = [lbm_kernel]
-        elif self._pattern == "pull":
+        elif self._two_arrays:
             self._kernels = [create_lb_function(lbm_config=lc, lbm_optimisation=lbm_optimisation, config=cfg)]
         else:
             self._kernels = [create_lb_function(lbm_config=replace(lc, timestep=t), lbm_optimisation=lbm_optimisation,
@@ -101,13 +107,15 @@ class LatticeBoltzmannStep:
         q = self.method.stencil.Q
         dt = config.data_type
         self._src = create_pdf_array(domain_size, q, 1, dt)
-        self._tmp = create_pdf_array(domain_size, q, 1, dt) if self._pattern == "pull" else None
+        self._tmp = create_pdf_array(domain_size, q, 1, dt) if self._two_arrays else None
         # macroscopic fields: host numpy (interior) + FP64 device staging with the PDF ghost layout
         self._rho_host = np.ones(domain_size)
         self._u_host = np.zeros(domain_size + (dim,))
         self._rho_dev = torch.zeros([n + 2 for n in domain_size][::-1], dtype=torch.float64, device="cuda")
         self._u_dev = torch.zeros([dim] + [n + 2 for n in domain_size][::-1], dtype=torch.float64, device="cuda")
         self._boundary_handling = LatticeBoltzmannBoundaryHandling(self.method, domain_size, self.periodicity)
+        # full NoSlip planes are fused into the pull / AA sweeps; the other patterns take them as links
+        self._boundary_handling.fuse_noslip_faces = self._pattern in ("pull", "aa")
         self._aa_next_odd = True
         self._links_dev = None
         self._links_version = None
@@ -207,7 +215,7 @@ class LatticeBoltzmannStep:
         tab, _ = method_table(self.method, self.kernel_params, np.float64,
                               corrections=self._kernels[0].corrections)
         tab_dev = torch.from_numpy(tab).cuda()
-        prev = nat.PULL if self._pattern == "pull" else nat.AA_EVEN
+        prev = self._codes[0]
         lib = nat.load()
         nat.check(lib.lbm355_set_equilibrium(ctypes.byref(d), ctypes.c_void_p(self._src.data_ptr()),
                                              ctypes.c_void_p(self._rho_dev.data_ptr()),
@@ -300,27 +308,27 @@ class LatticeBoltzmannStep:
             k.set_faces(self._boundary_handling.faces)
 
     def time_step(self):
-        if self._pattern == "pull":
-            self._apply_links(nat.PULL)
+        if self._two_arrays:
+            self._apply_links(self._codes[0])
             self._kernels[0](**{self._pdf_arr_name: self._src, self._tmp_arr_name: self._tmp}, **self.kernel_params)
             self._src, self._tmp = self._tmp, self._src
         else:
             even, odd = self._kernels
             if self._aa_next_odd:
-                self._apply_links(nat.AA_EVEN)
+                self._apply_links(self._codes[0])
                 odd(**{self._pdf_arr_name: self._src}, **self.kernel_params)
             else:
-                self._apply_links(nat.AA_ODD)
+                self._apply_links(self._codes[1])
                 even(**{self._pdf_arr_name: self._src}, **self.kernel_params)
             self._aa_next_odd = not self._aa_next_odd
 
     def post_run(self):
         """Density and velocity of the latest populations into the host fields (lbstep.py:311-314)."""
         d = self._descriptor()
-        if self._pattern == "pull":
-            last = nat.PULL
+        if self._two_arrays:
+            last = self._codes[0]
         else:
-            last = nat.AA_EVEN if self._aa_next_odd else nat.AA_ODD
+            last = self._codes[0] if self._aa_next_odd else self._codes[1]
         lib = nat.load()
         nat.check(lib.lbm355_get_macroscopic(ctypes.byref(d), ctypes.c_void_p(self._src.data_ptr()),
                                              ctypes.c_void_p(self._rho_dev.data_ptr()),

@@ -77,10 +77,10 @@ class SweepKernel:
                                       "create the kernel with CreateKernelConfig(target=Target.GPU)")
         if lc.kernel_type != 'default_stream_collide':
             raise NotImplementedError(f"kernel_type {lc.kernel_type!r} is not implemented on the GPU path")
-        if lc.streaming_pattern not in ("pull", "aa"):
-            raise NotImplementedError(f"streaming pattern {lc.streaming_pattern!r} is not implemented")
-        if lc.streaming_pattern == "aa" and int(lc.timestep) not in (0, 1):
-            raise ValueError(f"Invalid timestep for streaming pattern aa: {lc.timestep}")
+        if lc.streaming_pattern not in ("pull", "push") + tuple(nat.INPLACE_EVEN):
+            raise ValueError("Invalid value of parameter 'streaming_pattern'.", lc.streaming_pattern)
+        if lc.streaming_pattern in nat.INPLACE_EVEN and int(lc.timestep) not in (0, 1):
+            raise ValueError(f"Invalid timestep for streaming pattern {lc.streaming_pattern}: {lc.timestep}")
         if lo.field_layout not in ("fzyx", "f", "reverse_numpy", "SoA"):
             raise NotImplementedError("the GPU sweep uses the fzyx (SoA) layout")
         if m.stencil.name not in STENCIL_CODES or not m.stencil.is_gpu_layout():
@@ -98,8 +98,13 @@ class SweepKernel:
         if self.compute not in (nat.F32, nat.F64):
             raise ValueError("compute type must be float32 or float64")
         self.ghost_layers = int(cfg.ghost_layers)
-        self.pattern = nat.PULL if lc.streaming_pattern == "pull" else (
-            nat.AA_EVEN if int(lc.timestep) == 0 else nat.AA_ODD)
+        if lc.streaming_pattern == "pull":
+            self.pattern = nat.PULL
+        elif lc.streaming_pattern == "push":
+            self.pattern = nat.PUSH
+        else:
+            self.pattern = nat.INPLACE_EVEN[lc.streaming_pattern] + int(lc.timestep)
+        self.two_arrays = self.pattern in (nat.PULL, nat.PUSH)
         self.faces = [nat.FACE_PERIODIC if p else nat.FACE_GHOST for p in lo.builtin_periodicity[:m.dim]]
         self.smagorinsky = None
         sgs = lc.subgrid_scale_model
@@ -142,7 +147,10 @@ class SweepKernel:
     def set_faces(self, faces):
         """Per-axis face treatment: 'periodic', 'noslip' or 'ghost'."""
         codes = {"periodic": nat.FACE_PERIODIC, "noslip": nat.FACE_NOSLIP, "ghost": nat.FACE_GHOST}
-        self.faces = [codes[f] if isinstance(f, str) else int(f) for f in faces]
+        faces = [codes[f] if isinstance(f, str) else int(f) for f in faces]
+        if self.pattern not in (nat.PULL, nat.AA_EVEN, nat.AA_ODD) and nat.FACE_NOSLIP in faces:
+            raise ValueError("Eso and push streaming take walls as boundary links, not fused NoSlip faces")
+        self.faces = faces
         return self
 
     def table(self, bindings, device):
@@ -224,9 +232,10 @@ class SweepKernel:
         lc = self.lbm_config
         src = kwargs.pop(lc.field_name)
         dst = kwargs.pop(lc.temporary_field_name, None)
-        if self.pattern == nat.PULL:
+        if self.two_arrays:
             if dst is None:
-                raise ValueError(f"pull streaming needs the temporary field '{lc.temporary_field_name}'")
+                raise ValueError(f"{lc.streaming_pattern} streaming needs the temporary field "
+                                 f"'{lc.temporary_field_name}'")
         else:
             dst = src
         bindings = {}
@@ -241,9 +250,9 @@ class SweepKernel:
                 outputs[q] = arr
         self._check_tensor(src)
         self._check_tensor(dst)
-        if self.pattern == nat.PULL and src.shape != dst.shape:
+        if self.two_arrays and src.shape != dst.shape:
             raise ValueError("src and dst must have the same shape")
-        if self.pattern == nat.PULL and src.stride() != dst.stride():
+        if self.two_arrays and src.stride() != dst.stride():
             raise ValueError("src and dst must have the same strides")
         rho_o = outputs.get("density")
         u_o = outputs.get("velocity")

@@ -535,6 +535,51 @@ def _background_monomial(e, dim):
 
 # ------------------------------------------------------------------------ streaming driver
 
+# ------------------------------------------------------------------------ streaming accessors
+
+# lbmpy's 'lehmann' orderings (stencils.py:146-149, 214-218, 242-248); D3Q19 is the first 19 of D3Q27's
+_LEHMANN_2D = ((0, 0), (1, 0), (-1, 0), (0, 1), (0, -1), (1, 1), (-1, -1), (1, -1), (-1, 1))
+_LEHMANN_3D = ((0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1),
+               (1, 1, 0), (-1, -1, 0), (1, 0, 1), (-1, 0, -1), (0, 1, 1), (0, -1, -1), (1, -1, 0), (-1, 1, 0),
+               (1, 0, -1), (-1, 0, 1), (0, 1, -1), (0, -1, 1), (1, 1, 1), (-1, -1, -1), (1, 1, -1), (-1, -1, 1),
+               (1, -1, 1), (-1, 1, -1), (-1, 1, 1), (1, -1, -1))
+
+INPLACE_PATTERNS = ("aa", "esotwist", "esopull", "esopush")
+
+
+def streaming_accessor(c, inv, pattern, timestep):
+    """Read and write accesses of one kernel (fieldaccess.py:66-87 pull / push, :136-157 AA,
+    :160-316 EsoTwist / EsoPull / EsoPush) for directions c (Q, D) with inverse indices inv:
+    (read offsets (Q, D), read slots (Q,), write offsets (Q, D), write slots (Q,)).  timestep: 0 EVEN,
+    1 ODD (ignored by pull / push)."""
+    c = np.asarray(c, dtype=int)
+    Q, D = c.shape
+    inv = np.asarray(inv)
+    zero = np.zeros_like(c)
+    idx = np.arange(Q)
+    lehmann = _LEHMANN_2D if D == 2 else _LEHMANN_3D[:Q]
+    # EsoPull / EsoPush: "even" directions sit at even positions of the lehmann ordering
+    even = np.array([lehmann.index(tuple(v)) % 2 == 0 for v in c])[:, None]
+    if pattern == "pull":
+        return -c, idx, zero, idx
+    if pattern == "push":
+        return zero, idx, c, idx
+    if pattern == "aa":
+        return (zero, idx, zero, inv) if timestep == 0 else (-c, inv, c, idx)
+    if pattern == "esotwist":
+        up, down = np.maximum(c, 0), np.maximum(-c, 0)
+        return (down, idx, up, inv) if timestep == 0 else (down, inv, up, idx)
+    if pattern == "esopull":
+        r = np.where(even, -c, zero)
+        w = np.where(even, zero, c)
+        return (r, idx, w, inv) if timestep == 0 else (r, inv, w, idx)
+    if pattern == "esopush":
+        r = np.where(even, zero, -c)
+        w = np.where(even, c, zero)
+        return (r, inv, w, idx) if timestep == 0 else (r, idx, w, inv)
+    raise ValueError(f"unknown streaming pattern {pattern!r}")
+
+
 class OracleDomain:
     """A domain with one ghost layer, stepping the way ``LatticeBoltzmannStep`` does.
 
@@ -565,8 +610,10 @@ class OracleDomain:
         self.inv = np.array(method.stencil.inverse_indices())
         gshape = tuple(n + 2 for n in reversed(self.shape))
         self.src = np.zeros((self.Q,) + gshape)
-        self.dst = np.zeros_like(self.src) if pattern == "pull" else None
-        self.aa_parity = 0            # number of executed AA steps mod 2 (0: next kernel is ODD)
+        self.dst = np.zeros_like(self.src) if pattern in ("pull", "push") else None
+        self.aa_parity = 0            # number of executed in-place steps mod 2 (0: next kernel is ODD)
+        if pattern not in ("pull", "push") + INPLACE_PATTERNS:
+            raise ValueError(f"unknown streaming pattern {pattern!r}")
         self.rho_out = None
         self.u_out = None
 
@@ -644,23 +691,30 @@ class OracleDomain:
         d = links[:, self.D]
         return [links[:, a] + self.c[d, a] for a in range(self.D)]
 
+    def _accessor(self, pattern, timestep):
+        return streaming_accessor(self.c, self.inv, pattern, timestep)
+
+    def _kernel_key(self, prev):
+        """(pattern, timestep) of a kernel; "pull", "even", "odd" are the pull and AA kernels."""
+        if prev == "pull":
+            return ("pull", 2)
+        if prev in ("even", "odd"):
+            return (self.pattern if self.pattern in INPLACE_PATTERNS else "aa", 0 if prev == "even" else 1)
+        return prev
+
     def _out_addr(self, prev, i, cell):
-        """f_out(i) at `cell` after a `prev` kernel (its write accessor, indexing.py:41-46)."""
-        if prev == "odd":
-            coords = [cell[a] + self.c[i, a] for a in range(self.D)]
-            slot = i
-        else:
-            coords = list(cell)
-            slot = self.inv[i] if prev == "even" else i
-        return (slot,) + tuple(reversed(coords))
+        """f_out(i) at `cell` after the kernel `prev`: its write accessor (indexing.py:41-46)."""
+        pat, t = self._kernel_key(prev)
+        _, _, woff, wslot = self._accessor(pat, t)
+        coords = [cell[a] + woff[i, a] for a in range(self.D)]
+        return (wslot[i],) + tuple(reversed(coords))
 
     def _in_addr(self, prev, i, cell):
         """f_in(i) at `cell`: the read accessor of the kernel that follows `prev`."""
-        if prev == "odd":
-            return (i,) + tuple(reversed(list(cell)))
-        coords = [cell[a] - self.c[i, a] for a in range(self.D)]
-        slot = self.inv[i] if prev == "even" else i
-        return (slot,) + tuple(reversed(coords))
+        pat, t = self._kernel_key(prev)
+        roff, rslot, _, _ = self._accessor(pat, t if t == 2 else 1 - t)
+        coords = [cell[a] + roff[i, a] for a in range(self.D)]
+        return (rslot[i],) + tuple(reversed(coords))
 
     def apply_links(self, arr, prev):
         """Boundary links between time steps; `prev` is the kernel that ran last ("pull", "even", "odd"):
@@ -767,7 +821,47 @@ class OracleDomain:
         self.u_out = u_out.reshape((self.D,) + shape)
         return fpost.reshape((self.Q,) + shape)
 
+    def _generic_step(self):
+        """push, EsoTwist, EsoPull, EsoPush: periodicity, boundary links of the previous kernel, then the kernel
+        through its accessor tables, periodic images of ghost writes folded back (lbstep / communication order)."""
+        if self.pattern == "push":
+            prev = next_k = ("push", 2)
+            arr_in, arr_out = self.src, self.dst
+        else:
+            t = 1 if self.aa_parity == 0 else 0
+            prev, next_k = (self.pattern, 1 - t), (self.pattern, t)
+            arr_in = arr_out = self.src
+            if any(self.faces.get(a) == "ghost" for a in range(self.D)) and not self.boundaries:
+                raise NotImplementedError("the in-place oracle needs periodic or boundary-covered faces")
+        self.sync_periodic(arr_in)
+        self.apply_links(arr_in, prev)
+        roff, rslot, woff, wslot = self._accessor(*next_k)
+        fpre = np.stack([self._neighbor_view(arr_in, rslot[i], roff[i]) for i in range(self.Q)])
+        fpost = self._collide_cells(fpre)
+        for i in range(self.Q):
+            self._neighbor_view(arr_out, wslot[i], woff[i])[...] = fpost[i]
+        self._fold_writes(arr_out, woff, wslot)
+        if self.pattern == "push":
+            self.src, self.dst = self.dst, self.src
+        else:
+            self.aa_parity ^= 1
+
+    def _fold_writes(self, arr, woff, wslot):
+        """Move values a kernel wrote into ghost cells of periodic axes to their periodic images."""
+        for i in range(self.Q):
+            for a in range(self.D):
+                if self.faces.get(a) != "periodic" or woff[i, a] == 0:
+                    continue
+                ax = self._ax(a) - 1
+                src_sl = [slice(None)] * self.D
+                dst_sl = [slice(None)] * self.D
+                src_sl[ax], dst_sl[ax] = (-1, 1) if woff[i, a] == 1 else (0, -2)
+                arr[wslot[i]][tuple(dst_sl)] = arr[wslot[i]][tuple(src_sl)]
+
     def step(self):
+        if self.pattern in ("push", "esotwist", "esopull", "esopush"):
+            self._generic_step()
+            return
         if self.pattern == "pull":
             self.sync_periodic(self.src)
             self.noslip_pull(self.src)
@@ -834,13 +928,31 @@ class OracleDomain:
     def set_pdfs_interior(self, f):
         self.interior()[...] = f
 
+    def _last_kernel(self):
+        if self.pattern in ("pull", "push"):
+            return (self.pattern, 2)
+        return (self.pattern, 0 if self.aa_parity == 0 else 1)
+
+    def set_pdfs_natural(self, f):
+        """Place post-collision populations f (Q, [nz,] ny, nx) where the kernel that ran last wrote them (pull /
+        push, or the EVEN kernel of an in-place pattern before the first step), as lbmpy's setter does with
+        ``previous_timestep`` (macroscopic_value_kernels.py:29-45)."""
+        _, _, woff, wslot = self._accessor(*self._last_kernel())
+        for i in range(self.Q):
+            self._neighbor_view(self.src, wslot[i], woff[i])[...] = f[i]
+        self._fold_writes(self.src, woff, wslot)
+
     def pdfs_interior(self):
         return self.interior().copy()
 
     def pdfs_in_natural_layout(self):
         """Post-collision populations of the last step at each cell (slot i holds f*_i(x)),
         read through the write accessor of the last kernel (macroscopic_value_kernels.py:14-26):
-        pull: dst[i][x]; AA after EVEN (or initialisation): f[inv i][x]; AA after ODD: f[i][x + c_i]."""
+        pull: dst[i][x]; AA after EVEN (or initialisation): f[inv i][x]; AA after ODD: f[i][x + c_i];
+        push and the Eso patterns through their accessor tables."""
+        if self.pattern in ("push", "esotwist", "esopull", "esopush"):
+            _, _, woff, wslot = self._accessor(*self._last_kernel())
+            return np.stack([self._neighbor_view(self.src, wslot[i], woff[i]) for i in range(self.Q)])
         if self.pattern != "aa":
             return self.pdfs_interior()
         if self.aa_parity == 0:
