@@ -350,6 +350,66 @@ def run_closure(args):
     print(json.dumps(line), flush=True)
 
 
+def run_patterns(args):
+    """The D3Q27 cumulant sweep (FP32, periodic, no ghost layers) with every streaming pattern -- pull and push
+    on two arrays, AA, EsoTwist, EsoPull and EsoPush in place -- and with the Galilean / fourth-order
+    corrections (AA): MLUPS and achieved bandwidth (2 x 27 x 4 B per cell) per kernel, from HIP events around
+    each launch on the launch stream.  The state is the rest equilibrium (timing only; parity is in the tests)."""
+    import torch
+    from lbmpy_amd import (LBMConfig, LBStencil, Stencil, Method, create_lb_function, CreateKernelConfig, Target,
+                           Timestep)
+    from lbmpy_amd.sweep import create_pdf_array
+    n = args.n
+    cells = n ** 3
+    cfg = CreateKernelConfig(target=Target.GPU, data_type="float32", ghost_layers=0)
+    base = dict(stencil=LBStencil(Stencil.D3Q27), method=Method.CUMULANT, relaxation_rate=1.99, compressible=True)
+    variants = [("pull", "pull", {}), ("push", "push", {}), ("aa", "aa", {}), ("esotwist", "esotwist", {}),
+                ("esopull", "esopull", {}), ("esopush", "esopush", {}),
+                ("aa+galilean", "aa", dict(galilean_correction=True)),
+                ("aa+fourth_order", "aa", dict(fourth_order_correction=0.01)),
+                ("aa+galilean+fourth_order", "aa", dict(galilean_correction=True, fourth_order_correction=0.01))]
+    out = {}
+    for name, pattern, extra in variants:
+        if pattern in ("pull", "push"):
+            ks = [create_lb_function(lbm_config=LBMConfig(**base, **extra, streaming_pattern=pattern), config=cfg)]
+        else:
+            ks = [create_lb_function(lbm_config=LBMConfig(**base, **extra, streaming_pattern=pattern, timestep=t),
+                                     config=cfg) for t in (Timestep.ODD, Timestep.EVEN)]
+        for k in ks:
+            k.set_faces(["periodic"] * 3)
+        a = create_pdf_array((n, n, n), 27, ghost_layers=0, dtype="float32")
+        b = create_pdf_array((n, n, n), 27, ghost_layers=0, dtype="float32") if len(ks) == 1 else a
+
+        def launch(t):
+            if len(ks) == 1:
+                ks[0](src=a if t % 2 == 0 else b, dst=b if t % 2 == 0 else a)
+            else:
+                ks[t % 2](src=a)
+
+        for t in range(args.warmup):
+            launch(t)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+        torch.cuda.synchronize()
+        ev[0].record()
+        for t in range(args.steps):
+            launch(t)
+            ev[t + 1].record()
+        torch.cuda.synchronize()
+        per = [ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)]
+        launch_ms = sum(per) / len(per)
+        gbs = 2 * 27 * 4 * cells / (launch_ms / 1e3) / 1e9
+        out[name] = {"mlups": round(cells / (launch_ms / 1e3) / 1e6, 1), "launch_ms": round(launch_ms, 4),
+                     "achieved_gbs": round(gbs, 1), "roofline_frac": round(gbs / HBM_PEAK_GBS, 4)}
+        print(f"{name}: {out[name]}", flush=True)
+        del a, b, ks
+        torch.cuda.empty_cache()
+    line = {"metric": "D3Q27 cumulant sweep MLUPS per streaming pattern and correction (FP32, 256^3 periodic)",
+            "workload": f"D3Q27 cumulant, omega=1.99, periodic {n}^3, FP32 zero-centred, rest state, "
+                        f"{args.steps} timed launches after {args.warmup} warm-up launches",
+            "unit": "MLUPS", "bytes_per_cell_step": 216, "patterns": out}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -363,9 +423,10 @@ def main():
     ap.add_argument("--forms", default="float64,float16",
                     help="other storage forms run beside the headline one ('' for none)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="tgv", choices=["tgv", "channel", "ensemble", "closure"],
+    ap.add_argument("--workload", default="tgv", choices=["tgv", "channel", "ensemble", "closure", "patterns"],
                     help="tgv: the headline D3Q27 cumulant TGV; channel: D3Q19 MRT channel in three storage forms; "
-                         "ensemble: 256 batched D3Q19 TRT channels; closure: learned closure on decaying turbulence")
+                         "ensemble: 256 batched D3Q19 TRT channels; closure: learned closure on decaying turbulence; "
+                         "patterns: the D3Q27 cumulant sweep with every streaming pattern and correction")
     ap.add_argument("--closure-coarse", type=int, default=128)
     ap.add_argument("--closure-steps", type=int, default=1200)
     ap.add_argument("--members", type=int, default=256)
@@ -392,6 +453,9 @@ def main():
         return
     if args.workload == "closure":
         run_closure(args)
+        return
+    if args.workload == "patterns":
+        run_patterns(args)
         return
 
     import torch
