@@ -21,6 +21,44 @@
 
 namespace lbm355 {
 
+#define CONST_AS __attribute__((address_space(4)))
+#define GLOBAL_AS __attribute__((address_space(1)))
+
+// a generic PDF pointer re-tagged as global memory: accesses through it become global (not flat) ones
+__device__ __forceinline__ void* global_space(void* p) { return (void*)(GLOBAL_AS char*)p; }
+
+// field by field: a struct is not copied out of the constant address space as a whole
+template <class TC>
+__device__ __forceinline__ SweepArgs<TC> load_args(const CONST_AS SweepArgs<TC>* p) {
+  SweepArgs<TC> a;
+  a.src = p->src;
+  a.dst = p->dst;
+  a.rho_out = p->rho_out;
+  a.u_out = p->u_out;
+  a.tab = p->tab;
+  a.nx = p->nx;
+  a.ny = p->ny;
+  a.nz = p->nz;
+  a.g = p->g;
+  a.sy = p->sy;
+  a.sz = p->sz;
+  a.sq = p->sq;
+  a.osy = p->osy;
+  a.osz = p->osz;
+  a.osc = p->osc;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    a.face[k] = p->face[k];
+    a.force[k] = p->force[k];
+  }
+  a.flags = p->flags;
+  a.nactive = p->nactive;
+  a.smag_cs2x18 = p->smag_cs2x18;
+  a.omega_shear = p->omega_shear;
+  a.closure = p->closure;
+  return a;
+}
+
 template <class TC>
 struct BatchEntry {
   SweepArgs<TC> a;
@@ -28,10 +66,18 @@ struct BatchEntry {
   int64_t ncells;       // interior cells
 };
 
-// occupancy target 6 waves per SIMD: the compiler otherwise keeps 113 VGPRs (4 waves) for the argument
-// loads and index decoding; 80 VGPRs hold everything without spills
+// FP32 occupancy target: 6 waves per SIMD for the moment family up to 19 directions (the compiler
+// otherwise keeps more VGPRs than the argument loads and index decoding need; 80 hold everything without
+// spills), 4 for the cumulant collision (128 VGPRs), 3 for the D3Q27 moment family; FP64 arithmetic takes
+// what it needs.  Every instance compiles without spills.
 template <class S, int FAM, int PAT, class TS, class TC>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8)))
+__host__ __device__ constexpr int batch_min_waves() {
+  return sizeof(TC) == 8 ? 1 : (FAM == LBM355_FAMILY_MOMENT ? (S::Q <= 19 ? 6 : 3) : 4);
+}
+
+template <class S, int FAM, int PAT, class TS, class TC>
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(batch_min_waves<S, FAM, PAT, TS, TC>(), 8)))
 batch_kernel(const BatchEntry<TC>* __restrict__ members, const int32_t* __restrict__ tile_member, int64_t ntiles,
              int swap, int xcd_order) {
   const int waves_per_block = blockDim.x >> 6;
@@ -40,12 +86,19 @@ batch_kernel(const BatchEntry<TC>* __restrict__ members, const int32_t* __restri
   const int64_t wave = (int64_t)b * waves_per_block + (threadIdx.x >> 6);
   const int64_t nwaves = (int64_t)gridDim.x * waves_per_block;
   const int lane = threadIdx.x & 63;
+  // The member table and the tile map stay unchanged during the launch: read through the constant
+  // address space they become wave-uniform scalar loads, also after the stores of earlier tiles (the
+  // collision reads the method tables the same way, table_ptr).  The PDF pointers are global.
+  const CONST_AS BatchEntry<TC>* mem = (const CONST_AS BatchEntry<TC>*)members;
+  const CONST_AS int32_t* map = (const CONST_AS int32_t*)tile_member;
   for (int64_t tile = wave; tile < ntiles; tile += nwaves) {
-    const int m = __builtin_amdgcn_readfirstlane(tile_member[tile]);     // tile is uniform over the wave
-    const BatchEntry<TC>* e = members + m;
+    const int m = __builtin_amdgcn_readfirstlane(map[tile]);     // tile is uniform over the wave
+    const CONST_AS BatchEntry<TC>* e = mem + m;
     const uint32_t local = (uint32_t)(tile - e->first_tile) * 64u + (uint32_t)lane;
     if (local < (uint32_t)e->ncells) {
-      SweepArgs<TC> a = e->a;
+      SweepArgs<TC> a = load_args(&e->a);
+      a.src = global_space(const_cast<void*>(a.src));
+      a.dst = global_space(a.dst);
       if (swap) {
         void* t = const_cast<void*>(a.src);
         a.src = a.dst;

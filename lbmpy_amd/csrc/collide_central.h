@@ -77,7 +77,7 @@ __device__ __forceinline__ void forward_central(T (&m)[3][3][3], const T (&u)[3]
 
 // central moments about u -> populations (the inverse of forward_central), result on the grid
 template <class S, class T>
-__device__ __forceinline__ void backward_central(T (&m)[3][3][3], const T (&u)[3], const T* __restrict__ minv,
+__device__ __forceinline__ void backward_central(T (&m)[3][3][3], const T (&u)[3], TablePtr<T> __restrict__ minv,
                                                  T (&out)[S::Q]) {
 #pragma unroll
   for (int i = 0; i < 3; ++i)
@@ -127,7 +127,7 @@ template <class S, class T>
 __device__ __forceinline__ void collide_central(T (&f)[S::Q], const SweepArgs<T>& a, T& rho_out_v, T (&u_out_v)[3]) {
   constexpr int Q = S::Q, D = S::D;
   using L = CentralTab;
-  const T* __restrict__ tab = a.tab;
+  const TablePtr<T> __restrict__ tab = table_ptr(a.tab);
 
   T m[3][3][3];
   load_grid<S, T>(f, m);
@@ -178,8 +178,8 @@ __device__ __forceinline__ void collide_central(T (&f)[S::Q], const SweepArgs<T>
 #pragma unroll
   for (int e = 0; e < 27; ++e) dkp[e] = r0 * dk[e];
   for (int k = 0; k < a.nactive; ++k) {
-    const T* prow = tab + L::PROW + 27 * k;
-    const T* pinv = tab + L::PINV + 27 * k;
+    const TablePtr<T> prow = tab + L::PROW + 27 * k;
+    const TablePtr<T> pinv = tab + L::PINV + 27 * k;
     T sk = 0;
 #pragma unroll
     for (int e = 0; e < 27; ++e) sk += prow[e] * dk[e];
@@ -204,8 +204,8 @@ __device__ __forceinline__ void collide_central(T (&f)[S::Q], const SweepArgs<T>
       load_grid<S, T>(fl, kl);
       forward_central<S, T>(kl, u);
       for (int k = 0; k < a.nactive; ++k) {
-        const T* prow = tab + L::PROW + 27 * k;
-        const T* pinv = tab + L::PINV + 27 * k;
+        const TablePtr<T> prow = tab + L::PROW + 27 * k;
+        const TablePtr<T> pinv = tab + L::PINV + 27 * k;
         T sk = 0;
 #pragma unroll
         for (int e = 0; e < 27; ++e) sk += prow[e] * kl[e / 9][(e / 3) % 3][e % 3];

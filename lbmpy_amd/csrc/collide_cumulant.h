@@ -101,7 +101,7 @@ template <class T, bool LEARNED = false, bool CORRECTED = false>
 __device__ __forceinline__ void collide_cumulant_d3q27(T (&f)[27], const SweepArgs<T>& a, T& rho_out_v,
                                                        T (&u_out_v)[3]) {
   using S = D3Q27;
-  const T* __restrict__ r = a.tab + CumulantTab::RATES;
+  const TablePtr<T> __restrict__ r = table_ptr(a.tab) + CumulantTab::RATES;
   T m[3][3][3];  // [x][y][z]: first values at c=-1,0,+1, then exponents 0,1,2
 #pragma unroll
   for (int cx = -1; cx <= 1; ++cx)
@@ -223,7 +223,7 @@ __device__ __forceinline__ void collide_cumulant_d3q27(T (&f)[27], const SweepAr
   T w5 = r[16];
   if constexpr (CORRECTED) {
     if (a.flags & LBM355_FLAG_FOURTH_ORDER) {
-      const T* __restrict__ x = a.tab + CumulantTab::EXTRA;
+      const TablePtr<T> __restrict__ x = table_ptr(a.tab) + CumulantTab::EXTRA;
       const T lim = x[CumulantTab::LIMITER];
       const T sp = fabs(mu120 + mu102), sm = fabs(mu120 - mu102), s1 = fabs(mu111);
       const T l3 = x[CumulantTab::W3] + (T(1) - x[CumulantTab::W3]) * sp / (lim + sp);
@@ -259,7 +259,7 @@ __device__ __forceinline__ void collide_cumulant_d3q27(T (&f)[27], const SweepAr
       tr += g2 * (ex + ey + ez);
     }
     if (a.flags & LBM355_FLAG_FOURTH_ORDER) {
-      const T* __restrict__ x = a.tab + CumulantTab::EXTRA;
+      const TablePtr<T> __restrict__ x = table_ptr(a.tab) + CumulantTab::EXTRA;
       const T pre = T(1) / w1 - T(0.5);
       const T pa = pre * x[CumulantTab::A], pb = pre * x[CumulantTab::B];
       p1 += T(2.0 / 3.0) * pa * r[17] * (dxu - T(2) * dyv + dzw);

@@ -37,7 +37,7 @@ __device__ __forceinline__ void collide_moment(T (&f)[S::Q], const SweepArgs<T>&
   constexpr int Q = S::Q;
   constexpr int D = S::D;
   using L = MomentTab<Q>;
-  const T* __restrict__ tab = a.tab;
+  const TablePtr<T> __restrict__ tab = table_ptr(a.tab);
 
   T s = 0, j[3] = {0, 0, 0};
 #pragma unroll
@@ -101,8 +101,8 @@ __device__ __forceinline__ void collide_moment(T (&f)[S::Q], const SweepArgs<T>&
   }
   // individually relaxed moments: rank-one projector corrections
   for (int k = 0; k < a.nactive; ++k) {
-    const T* mrow = tab + L::MROW + k * Q;
-    const T* minv = tab + L::MINV + k * Q;
+    const TablePtr<T> mrow = tab + L::MROW + k * Q;
+    const TablePtr<T> minv = tab + L::MINV + k * Q;
     T m = 0;
 #pragma unroll
     for (int i = 0; i < Q; ++i) m += mrow[i] * dl[i];

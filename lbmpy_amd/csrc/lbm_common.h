@@ -183,6 +183,17 @@ struct SweepArgs {
   const void* closure;        // learned-closure parameter block (LBM355_FLAG_LEARNED_CLOSURE)
 };
 
+// ---------------------------------------------------------------- method tables
+// The method table is read-only for a whole launch.  Read through the constant address space, its
+// wave-uniform entries become scalar loads wherever they are read, also after stores of the same
+// launch (a persistent loop's later tiles); through a generic pointer they would be per-lane loads.
+template <class T>
+using TablePtr = const __attribute__((address_space(4))) T*;
+template <class T>
+__device__ __forceinline__ TablePtr<T> table_ptr(const T* p) {
+  return (TablePtr<T>)p;
+}
+
 // ---------------------------------------------------------------- addressing
 // Per axis: the coordinate of the neighbour at offset -1/0/+1 (periodic wrap folded in)
 // and whether that neighbour is a NoSlip wall cell.
