@@ -142,7 +142,9 @@ int lbm355_reduce_conserved(const lbm355_sweep_desc* desc, const void* pdf, doub
  * equilibrium velocity is u - F/(2 rho_ref) when a force model is present.  prev_pattern:
  * LBM355_PULL writes slot i; the EVEN kernel of an in-place pattern writes through that kernel's
  * write accessor (state "after an EVEN step", so the next kernel is ODD; neighbour writes wrap on
- * periodic axes and land in the ghost layer otherwise).  table_f64: FP64 method table (moment family; ignored for cumulants).
+ * periodic axes and land in the ghost layer otherwise).  table_f64: FP64 method table (moment and
+ * central-moment families; ignored for cumulants); a central-moment equilibrium is the method's
+ * collision with every rate 1.
  * Computes in FP64 and rounds once to the storage type. */
 int lbm355_set_equilibrium(const lbm355_sweep_desc* desc, void* pdf, const double* density, const double* velocity,
                            const double* table_f64, int32_t prev_pattern, void* stream);

@@ -123,7 +123,9 @@ __device__ __forceinline__ T beta_e(const T (&u)[3], const T (&u2)[3]) {
   return beta<EX, EY, EZ, T>(u, u2);
 }
 
-template <class S, class T>
+// EQUILIBRIUM: relax every central moment with rate 1 and no force terms, which maps a state to the
+// method's equilibrium at its (rho, j) -- the macroscopic setter's use (macroscopic_value_kernels.py:29-45)
+template <class S, class T, bool EQUILIBRIUM = false>
 __device__ __forceinline__ void collide_central(T (&f)[S::Q], const SweepArgs<T>& a, T& rho_out_v, T (&u_out_v)[3]) {
   constexpr int Q = S::Q, D = S::D;
   using L = CentralTab;
@@ -173,11 +175,11 @@ __device__ __forceinline__ void collide_central(T (&f)[S::Q], const SweepArgs<T>
   LBM_DK(2, 2, 0) LBM_DK(2, 2, 1) LBM_DK(2, 2, 2)
 #undef LBM_DK
 
-  const T r0 = tab[L::R0];
+  const T r0 = EQUILIBRIUM ? T(1) : tab[L::R0];
   T dkp[27];
 #pragma unroll
   for (int e = 0; e < 27; ++e) dkp[e] = r0 * dk[e];
-  for (int k = 0; k < a.nactive; ++k) {
+  for (int k = 0; k < (EQUILIBRIUM ? 0 : a.nactive); ++k) {
     const TablePtr<T> prow = tab + L::PROW + 27 * k;
     const TablePtr<T> pinv = tab + L::PINV + 27 * k;
     T sk = 0;
@@ -189,7 +191,7 @@ __device__ __forceinline__ void collide_central(T (&f)[S::Q], const SweepArgs<T>
   }
 
   // force vector in population space (affine in u) and its Guo correction
-  const int fmode = (int)tab[L::FMODE];
+  const int fmode = EQUILIBRIUM ? 0 : (int)tab[L::FMODE];
   T fl[Q];
   if (fmode != 0) {
 #pragma unroll

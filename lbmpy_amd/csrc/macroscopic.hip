@@ -13,6 +13,7 @@
 #include "lbm_common.h"
 #include "collide_moment.h"
 #include "collide_cumulant.h"
+#include "collide_central.h"
 
 namespace lbm355 {
 
@@ -64,6 +65,26 @@ __global__ void set_equilibrium_kernel(TS* pdf, const double* rho_in, const doub
   double feq[Q];
   if (FAM == LBM355_FAMILY_MOMENT) {
     moment_equilibrium<S>(tab, rho - 1.0, comp ? rho : 1.0, u, feq);
+  } else if (FAM == LBM355_FAMILY_CENTRAL) {
+    // a state with this density and momentum (j = rho_ref u) collided with every central moment relaxed
+    // at rate 1 is the method's equilibrium; the collision works on deviations f - w
+    SweepArgs<double> ca{};
+    ca.tab = tab;
+    ca.flags = comp ? LBM355_FLAG_COMPRESSIBLE : 0;
+    const double ref_j = comp ? rho : 1.0;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      double cj = 0;
+#pragma unroll
+      for (int d = 0; d < D; ++d) cj += S::c[i][d] * ref_j * u[d];
+      feq[i] = weight<S>(i) * ((rho - 1.0) + 3.0 * cj);
+    }
+    double r_out, u_out[3];
+    collide_central<S, double, true>(feq, ca, r_out, u_out);
+    if (!(flags & LBM355_FLAG_ZERO_CENTERED)) {
+#pragma unroll
+      for (int i = 0; i < Q; ++i) feq[i] += weight<S>(i);
+    }
   } else {
     const bool zc = flags & LBM355_FLAG_ZERO_CENTERED;
 #pragma unroll
@@ -225,6 +246,14 @@ int lbm355_set_equilibrium(const lbm355_sweep_desc* d, void* pdf, const double* 
       return set_eq_dtype<D3Q27, LBM355_FAMILY_MOMENT>(d, pdf, density, velocity, table_f64, prev_pattern, s);
   } else if (d->family == LBM355_FAMILY_CUMULANT && d->stencil == LBM355_D3Q27) {
     return set_eq_dtype<D3Q27, LBM355_FAMILY_CUMULANT>(d, pdf, density, velocity, table_f64, prev_pattern, s);
+  } else if (d->family == LBM355_FAMILY_CENTRAL) {
+    if (!table_f64) return fail2(LBM355_ERR_ARG, "central-moment family needs the FP64 method table");
+    if (d->stencil == LBM355_D2Q9)
+      return set_eq_dtype<D2Q9, LBM355_FAMILY_CENTRAL>(d, pdf, density, velocity, table_f64, prev_pattern, s);
+    if (d->stencil == LBM355_D3Q19)
+      return set_eq_dtype<D3Q19, LBM355_FAMILY_CENTRAL>(d, pdf, density, velocity, table_f64, prev_pattern, s);
+    if (d->stencil == LBM355_D3Q27)
+      return set_eq_dtype<D3Q27, LBM355_FAMILY_CENTRAL>(d, pdf, density, velocity, table_f64, prev_pattern, s);
   }
   return fail2(LBM355_ERR_UNSUPPORTED, "no setter for this stencil/family");
 }
