@@ -177,3 +177,29 @@ def test_fp32_esotwist_bit_identical_to_aa():
     a = _gpu_run(kw, m, shape, "aa", f0, 8, 0, "float32")
     b = _gpu_run(kw, m, shape, "esotwist", f0, 8, 0, "float32")
     np.testing.assert_array_equal(a, b)
+
+
+ROW_CASES = [
+    ("D2Q9", Method.TRT, dict(relaxation_rates=[1.8, 1.1], force=(1e-4, 0)), (128, 6)),
+    ("D3Q19", Method.TRT, dict(relaxation_rates=[1.8, 1.2], force=(1e-5, 0, 0)), (64, 5, 4)),
+    ("D3Q27", Method.CUMULANT, dict(relaxation_rate=1.9, compressible=True), (64, 4, 5)),
+]
+
+
+@pytest.mark.parametrize("g", [1, 0])
+@pytest.mark.parametrize("pattern", PATTERNS)
+@pytest.mark.parametrize("case", ROW_CASES, ids=lambda c: c[0])
+def test_pattern_row_staged_sweep_fp64_matches_oracle(case, pattern, g):
+    """Whole periodic rows (nx % 64 == 0) take the row-staged kernel (x shifts through LDS)."""
+    st, meth, extra, shape = case
+    kw = dict(stencil=LBStencil(Stencil[st]), method=meth, **extra)
+    m = create_lb_method(LBMConfig(**kw))
+    f0 = seeded_state(m, shape, seed=23, du=0.04, noise=1e-3)
+    dom = OracleDomain(m, shape, faces={a: "periodic" for a in range(m.dim)}, pattern=pattern)
+    dom.set_pdfs_natural(f0)
+    steps = 6
+    for _ in range(steps):
+        dom.step()
+    print(f"row-staged pattern {pattern} {st} g={g}", flush=True)
+    got = _gpu_run(kw, m, shape, pattern, f0, steps, g)
+    assert np.abs(got - dom.pdfs_in_natural_layout()).max() <= FP64_TOL
