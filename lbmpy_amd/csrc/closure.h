@@ -41,11 +41,9 @@ typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 
 // FP32 -> BF16 bits, round to nearest even (finite inputs)
-__device__ __forceinline__ uint16_t f2bf(float f) {
-  uint32_t u = __float_as_uint(f);
-  u = (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
-  return (uint16_t)u;
-}
+// FP32 -> BF16 with round-to-nearest-even (one v_cvt_pk_bf16_f32 on gfx950; the same result as adding
+// 0x7FFF plus the lowest kept bit and truncating, for every finite value)
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
 
 __device__ __forceinline__ float softplus(float o) { return o > 20.f ? o : log1pf(expf(o)); }
@@ -66,6 +64,19 @@ __device__ __forceinline__ void closure_features(const float (&p)[6], float rho,
   x[7] = q[0] + q[1] + q[2];
   x[8] = qq;
   x[9] = t3;
+}
+
+// tanh in FP32 for the hidden layers: an odd polynomial for |x| < 0.5 (least-squares fit of tanh(x)/x in
+// x^2 at Chebyshev nodes, relative error below 1e-7) and 1 - 2 / (exp(2|x|) + 1) above (hardware exp and
+// reciprocal, a few ulp); about ten instructions where the library tanh takes several times as many
+__device__ __forceinline__ float tanh_fast(float x) {
+  // both forms, then a select: no per-lane branches
+  const float ax = fabsf(x);
+  const float t = x * x;
+  const float p = ((((-6.615864039e-3f * t + 2.1276234072e-2f) * t - 5.389375384e-2f) * t + 0.13332905004f) * t -
+                   0.33333324269f) * t + 0.99999999969f;
+  const float r = copysignf(1.f - 2.f * __builtin_amdgcn_rcpf(__expf(2.f * ax) + 1.f), x);
+  return ax < 0.5f ? x * p : r;
 }
 
 // A fragment (4 BF16) of a [32 x K] row-major BF16 matrix for K block kb
@@ -109,14 +120,18 @@ __device__ __forceinline__ float mlp_forward_wave(const ClosureParams* __restric
   float out = 0.f;
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
-    const uint32_t* src = (c == half) ? own : other;       // features of cell 32c + lane%32
+    // features of cell 32c + lane%32: this lane's own (c == half) or the partner lane's; selected element by
+    // element with compile-time indices so that both arrays stay in registers
+    const bool mine = (c == half);
     v16f acc;
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[v] = P->b1[c_row(v, lane)];
 #pragma unroll
     for (int kb = 0; kb < 2; ++kb) {
       // k = 8kb + 4*half + t  ->  dwords 4kb + 2*half + {0, 1}
-      const uint32_t d0 = src[4 * kb + 2 * half], d1 = src[4 * kb + 2 * half + 1];
+      const uint32_t o0 = half ? own[4 * kb + 2] : own[4 * kb], o1 = half ? own[4 * kb + 3] : own[4 * kb + 1];
+      const uint32_t p0 = half ? other[4 * kb + 2] : other[4 * kb], p1 = half ? other[4 * kb + 3] : other[4 * kb + 1];
+      const uint32_t d0 = mine ? o0 : p0, d1 = mine ? o1 : p1;
       v4s b;
       b[0] = (short)(d0 & 0xFFFF);
       b[1] = (short)(d0 >> 16);
@@ -127,7 +142,7 @@ __device__ __forceinline__ float mlp_forward_wave(const ClosureParams* __restric
     v4s hb[4];
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
-      const float h = tanhf(acc[v]);
+      const float h = tanh_fast(acc[v]);
       if (h1) h1[c][v] = h;
       hb[v >> 2][v & 3] = (short)f2bf(h);
     }
@@ -139,7 +154,7 @@ __device__ __forceinline__ float mlp_forward_wave(const ClosureParams* __restric
     float part = 0.f;
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
-      const float h = tanhf(acc2[v]);
+      const float h = tanh_fast(acc2[v]);
       if (h2) h2[c][v] = h;
       part += bf2f(P->w3[c_row(v, lane)]) * h;
     }
