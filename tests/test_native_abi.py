@@ -67,6 +67,13 @@ def test_header_constants_match_binding():
         "LBM355_FLAG_ZERO_CENTERED": n.FLAG_ZERO_CENTERED, "LBM355_FLAG_COMPRESSIBLE": n.FLAG_COMPRESSIBLE,
         "LBM355_FLAG_EQ_SHIFT": n.FLAG_EQ_SHIFT, "LBM355_FLAG_FORCE": n.FLAG_FORCE,
         "LBM355_FLAG_SMAGORINSKY": n.FLAG_SMAGORINSKY, "LBM355_FLAG_LEARNED_CLOSURE": n.FLAG_LEARNED_CLOSURE,
+        "LBM355_FLAG_GALILEAN": n.FLAG_GALILEAN, "LBM355_FLAG_FOURTH_ORDER": n.FLAG_FOURTH_ORDER,
+        "LBM355_ESOTWIST_EVEN": n.ESOTWIST_EVEN, "LBM355_ESOTWIST_ODD": n.ESOTWIST_ODD,
+        "LBM355_ESOPULL_EVEN": n.ESOPULL_EVEN, "LBM355_ESOPULL_ODD": n.ESOPULL_ODD,
+        "LBM355_ESOPUSH_EVEN": n.ESOPUSH_EVEN, "LBM355_ESOPUSH_ODD": n.ESOPUSH_ODD, "LBM355_PUSH": n.PUSH,
+        "LBM355_BC_NOSLIP": n.BC_NOSLIP, "LBM355_BC_UBB": n.BC_UBB, "LBM355_BC_FREESLIP": n.BC_FREESLIP,
+        "LBM355_BC_FIXED_DENSITY": n.BC_FIXED_DENSITY, "LBM355_BC_OUTFLOW": n.BC_OUTFLOW,
+        "LBM355_BC_OUTFLOW_INIT": n.BC_OUTFLOW_INIT,
     }
     for k, v in expect.items():
         assert macros.get(k) == v, k
@@ -92,3 +99,88 @@ def test_invalid_table_query_reports_error(lib):
     from lbmpy_amd import _native
     assert lib.lbm355_table_elements(_native.D3Q19, _native.FAMILY_CUMULANT) <= 0
     assert lib.lbm355_last_error()
+
+
+# ---- argument validation: every case below is rejected before any HIP call, so it runs without a GPU
+
+DUMMY_A, DUMMY_B = ctypes.c_void_p(0x10000), ctypes.c_void_p(0x20000)     # never dereferenced
+
+
+def _desc(stencil="D3Q27", family="cumulant", pattern="AA_EVEN", faces=("PERIODIC",) * 3, flags=0):
+    from lbmpy_amd import _native as n
+    d = n.SweepDesc()
+    d.stencil = n.STENCIL_NAMES[stencil]
+    d.family = {"moment": n.FAMILY_MOMENT, "central": n.FAMILY_CENTRAL, "cumulant": n.FAMILY_CUMULANT}[family]
+    d.pattern = getattr(n, pattern) if isinstance(pattern, str) else pattern
+    d.store_dtype = d.compute_dtype = n.F32
+    d.nx, d.ny, d.nz = 64, 8, 8
+    d.ghost_layers = 1
+    for a in range(3):
+        d.face[a] = getattr(n, "FACE_" + faces[a])
+    d.stride_y, d.stride_z = 66, 66 * 10
+    d.stride_q = 66 * 10 * 10
+    d.flags = flags
+    d.tables = 0x30000
+    return d
+
+
+def _sweep_error(lib, d, src=DUMMY_A, dst=None):
+    rc = lib.lbm355_sweep(ctypes.byref(d), src, src if dst is None else dst, None, None, None)
+    return rc, lib.lbm355_last_error().decode()
+
+
+def test_sweep_rejects_bad_patterns_and_array_counts(lib):
+    assert _sweep_error(lib, _desc(pattern=42))[0] != 0
+    rc, msg = _sweep_error(lib, _desc(pattern="ESOTWIST_EVEN", faces=("PERIODIC", "NOSLIP", "PERIODIC")))
+    assert rc != 0 and "boundary links" in msg
+    rc, msg = _sweep_error(lib, _desc(pattern="PULL"))
+    assert rc != 0 and "two arrays" in msg
+    rc, msg = _sweep_error(lib, _desc(pattern="PUSH"))
+    assert rc != 0 and "two arrays" in msg
+    rc, msg = _sweep_error(lib, _desc(pattern="ESOPULL_ODD"), DUMMY_A, DUMMY_B)
+    assert rc != 0 and "in-place" in msg
+
+
+def test_sweep_rejects_bad_correction_combinations(lib):
+    from lbmpy_amd import _native as n
+    rc, msg = _sweep_error(lib, _desc(stencil="D3Q19", family="moment", flags=n.FLAG_GALILEAN))
+    assert rc != 0 and "cumulant" in msg
+    rc, msg = _sweep_error(lib, _desc(flags=n.FLAG_FOURTH_ORDER | n.FLAG_SMAGORINSKY))
+    assert rc != 0 and "Smagorinsky" in msg
+    rc, msg = _sweep_error(lib, _desc(flags=n.FLAG_GALILEAN | n.FLAG_LEARNED_CLOSURE))
+    assert rc != 0
+    rc, msg = _sweep_error(lib, _desc(pattern="ESOPUSH_EVEN", flags=n.FLAG_LEARNED_CLOSURE))
+    assert rc != 0 and "pull or AA" in msg
+
+
+def test_boundary_links_reject_bad_patterns(lib):
+    from lbmpy_amd import _native as n
+    b = n.BoundaryDesc()
+    b.stencil, b.store_dtype, b.kind = n.D2Q9, n.F64, n.BC_NOSLIP
+    b.nx, b.ny, b.nz, b.ghost_layers = 8, 8, 1, 1
+    b.stride_y, b.stride_q = 10, 100
+    b.pattern = 42
+    assert lib.lbm355_boundary_links(ctypes.byref(b), DUMMY_A, DUMMY_B, 4, None, None) != 0
+    b.pattern = n.ESOTWIST_ODD
+    b.fused_noslip[1] = 1
+    assert lib.lbm355_boundary_links(ctypes.byref(b), DUMMY_A, DUMMY_B, 4, None, None) != 0
+    assert "fused NoSlip" in lib.lbm355_last_error().decode()
+
+
+def test_macroscopic_kernels_reject_bad_patterns(lib):
+    from lbmpy_amd import _native as n
+    d = _desc(stencil="D2Q9", family="moment", faces=("PERIODIC", "NOSLIP", "PERIODIC"))
+    d.nz, d.stride_z, d.stride_q = 1, 0, 660
+    assert lib.lbm355_set_equilibrium(ctypes.byref(d), DUMMY_A, DUMMY_B, DUMMY_B, DUMMY_B, n.AA_ODD, None) != 0
+    assert lib.lbm355_set_equilibrium(ctypes.byref(d), DUMMY_A, DUMMY_B, DUMMY_B, DUMMY_B, n.ESOPULL_EVEN, None) != 0
+    assert lib.lbm355_get_macroscopic(ctypes.byref(d), DUMMY_A, DUMMY_B, DUMMY_B, 42, None) != 0
+    assert lib.lbm355_get_macroscopic(ctypes.byref(d), DUMMY_A, DUMMY_B, DUMMY_B, n.ESOTWIST_ODD, None) != 0
+
+
+def test_batch_rejects_eso_and_corrected_members(lib):
+    from lbmpy_amd import _native as n
+    out = ctypes.c_void_p()
+    for d in (_desc(pattern="ESOTWIST_EVEN"), _desc(flags=n.FLAG_GALILEAN)):
+        descs = (n.SweepDesc * 1)(d)
+        ptrs = (ctypes.c_void_p * 1)(DUMMY_A)
+        assert lib.lbm355_batch_create(descs, ptrs, ptrs, 1, ctypes.byref(out)) != 0
