@@ -207,6 +207,24 @@ typedef struct lbm355_boundary_desc {
 int lbm355_boundary_links(const lbm355_boundary_desc* desc, void* pdf, const int32_t* links, int64_t n_links,
                           double* link_data, void* stream);
 
+/* ---- Ghost-layer periodicity -----------------------------------------------------------------------
+ * For sweeps whose faces are LBM355_FACE_GHOST (lbmpy's own kernels never wrap): the copies of lbmpy's
+ * LBMPeriodicityHandling (advanced_streaming/communication.py:21-126), each source slice of
+ * get_communication_slices (:128-183) copied onto its destination slice, as boxes.  boxes: host array of
+ * n_boxes x 5 int64 {source offset, destination offset, extent x, extent y, extent z}, offsets in elements
+ * from `pdf` (slot included), element (i, j, l) of a box at i stride_x + j stride_y + l stride_z from its
+ * corners.  Every box must lie within [0, span) elements; sources and destinations must not overlap (true
+ * of lbmpy's slices), since all boxes of a call are copied concurrently.  Any element type of the given
+ * size (LBM355_F64 / F32 / F16) is copied bit for bit. */
+int lbm355_copy_boxes(int32_t dtype, void* pdf, int64_t span, int64_t stride_x, int64_t stride_y, int64_t stride_z,
+                      const int64_t* boxes, int32_t n_boxes, void* stream);
+
+/* Access table of a streaming-pattern kernel code (LBM355_PULL ... LBM355_PUSH), as the sweep kernels are
+ * compiled with it: out[Q][8] = {read offset x, y, z, read slot, write offset x, y, z, write slot} per
+ * direction (lbmpy's accessor read() / write(), fieldaccess.py:66-316).  Host only, no GPU.  Returns Q, or
+ * minus an error code. */
+int32_t lbm355_pattern_access(int32_t stencil, int32_t pattern, int32_t* out);
+
 /* ---- Batched ensembles -------------------------------------------------------------------------
  * Many independent domains ("members") advanced by one launch per time step.  lbmpy has no batched
  * sweep: an ensemble there is a Python loop calling one compiled kernel per member and time step

@@ -58,7 +58,8 @@ EXPORTED = ("lbm355_sweep", "lbm355_table_elements", "lbm355_noslip_links", "lbm
             "lbm355_last_error", "lbm355_version", "lbm355_batch_create", "lbm355_batch_sweep",
             "lbm355_batch_info", "lbm355_batch_destroy", "lbm355_closure_params_size",
             "lbm355_closure_master_size", "lbm355_closure_features", "lbm355_closure_forward",
-            "lbm355_closure_train_step", "lbm355_closure_adam", "lbm355_boundary_links")
+            "lbm355_closure_train_step", "lbm355_closure_adam", "lbm355_boundary_links", "lbm355_copy_boxes",
+            "lbm355_pattern_access")
 
 _lib = None
 
@@ -114,6 +115,11 @@ def load(path=None):
     lib.lbm355_boundary_links.argtypes = [ctypes.POINTER(BoundaryDesc), c_void_p, c_void_p, c_int64, c_void_p,
                                           c_void_p]
     lib.lbm355_boundary_links.restype = c_int32
+    lib.lbm355_copy_boxes.argtypes = [c_int32, c_void_p, c_int64, c_int64, c_int64, c_int64, c_void_p, c_int32,
+                                      c_void_p]
+    lib.lbm355_copy_boxes.restype = c_int32
+    lib.lbm355_pattern_access.argtypes = [c_int32, c_int32, c_void_p]
+    lib.lbm355_pattern_access.restype = c_int32
     lib.lbm355_desc_size.argtypes = []
     lib.lbm355_desc_size.restype = c_int64
     if lib.lbm355_desc_size() != ctypes.sizeof(SweepDesc):
