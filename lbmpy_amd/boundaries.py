@@ -5,11 +5,16 @@ Mirrors lbmpy ``boundaries/boundaryconditions.py`` (``NoSlip`` :104-139, ``UBB``
 machinery of ``boundaries/boundaryhandling.py:15-199``:
 boundary cells are flagged on a (ghost-layer inclusive) flag field with the id of their boundary
 object; every link from a fluid cell x into a boundary cell in direction d becomes an index-list
-entry (x, y, z, d) of that object, executed by ``lbm355_boundary_links`` between sweeps.  Links
-are found across periodic axes with wrapped coordinates, and the kernel wraps them the same way.
+entry (x, y, z, d) of that object, executed by ``lbm355_boundary_links`` between sweeps.
+
+For ``LatticeBoltzmannStep`` (``LatticeBoltzmannBoundaryHandling.for_domain``) periodic axes wrap inside the
+sweeps: links are found across them with wrapped coordinates, and the kernel wraps them the same way.
 Whole NoSlip planes on the ghost layer of a non-periodic axis (the usual channel walls,
-``slice_from_direction``) are instead handled inside the sweep (``LBM355_FACE_NOSLIP``), which reads
-the bounce-back value directly and saves the separate launch; both give identical results.
+``slice_from_direction``) are handled inside the sweep there (``LBM355_FACE_NOSLIP``), which reads the
+bounce-back value directly and saves the separate launch; both give identical results.  In lbmpy's form
+(constructed from a data handling, called as ``bh(prev_timestep=...)`` in a user's loop) the kernels read
+ghost layers filled by ``LBMPeriodicityHandling``, so links are taken from the ghost-layer flags as they are
+and nothing wraps or is fused, as in lbmpy.
 """
 import numpy as np
 
@@ -142,14 +147,54 @@ def slice_from_direction(direction_name, dim, normal_offset=0, tangential_offset
 
 
 class LatticeBoltzmannBoundaryHandling:
-    """Flag field (boundary-object ids) + index lists for one domain (ghost-layer inclusive coordinates)."""
+    """Flag field (boundary-object ids) + index lists for one domain (ghost-layer inclusive coordinates), and
+    the launches of ``lbm355_boundary_links`` that apply them (boundaries/boundaryhandling.py:15-199).
 
-    def __init__(self, method, domain_size, periodicity, ghost_layers=1):
+    lbmpy's form, for loops written around ``create_lb_function`` (as lbmpy's tests are):
+    ``LatticeBoltzmannBoundaryHandling(lb_method, data_handling, pdf_field_name, streaming_pattern='pull')``,
+    ``set_boundary(...)`` and, between kernels, ``bh(prev_timestep=...)`` with the time step of the kernel that
+    ran last.  ``data_handling`` needs ``periodicity`` and ``gpu_arrays`` (pystencils' SerialDataHandling or
+    ``lbmpy_amd.advanced_streaming.GpuArrays``).  As in lbmpy, the kernels of such a loop read ghost layers and
+    periodicity comes from ghost copies (``LBMPeriodicityHandling``, called before the boundaries): links are
+    never wrapped and no NoSlip face is fused into a sweep.  ``target`` and ``openmp`` are accepted; the
+    boundaries always run on the GPU.
+
+    ``LatticeBoltzmannStep`` builds its own with ``for_domain``: periodic axes are wrapped inside the sweeps
+    and links across them are wrapped the same way."""
+
+    def __init__(self, lb_method, data_handling, pdf_field_name, streaming_pattern="pull", name="boundary_handling",
+                 flag_interface=None, target=None, openmp=False):
+        from .advanced_streaming.utility import is_inplace
+        is_inplace(streaming_pattern)                                   # validates the name
+        arr = data_handling.gpu_arrays[pdf_field_name]
+        g = 1
+        if hasattr(data_handling, "ghost_layers_of_field"):
+            g = int(data_handling.ghost_layers_of_field(pdf_field_name))
+        D = lb_method.dim
+        self._setup(lb_method, tuple(int(n) - 2 * g for n in arr.shape[:D]),
+                    tuple(bool(p) for p in data_handling.periodicity)[:D], g, builtin_periodicity=False)
+        self.name = name
+        self._dh = data_handling
+        self._pdf_name = pdf_field_name
+        self._pattern = streaming_pattern
+
+    @classmethod
+    def for_domain(cls, method, domain_size, periodicity, ghost_layers=1):
+        """The step driver's form: periodic axes wrap inside the sweeps."""
+        self = cls.__new__(cls)
+        self._setup(method, tuple(domain_size), tuple(periodicity), ghost_layers, builtin_periodicity=True)
+        self.name = "boundary_handling"
+        self._dh = self._pdf_name = None
+        self._pattern = None
+        return self
+
+    def _setup(self, method, domain_size, periodicity, ghost_layers, builtin_periodicity):
         self.method = method
         self.dim = len(domain_size)
         self.domain_size = tuple(domain_size)
         self.periodicity = tuple(periodicity)
         self.ghost_layers = ghost_layers
+        self.builtin_periodicity = builtin_periodicity
         self.shape = tuple(n + 2 * ghost_layers for n in domain_size)   # index [x, y, (z)]
         self.flags = np.zeros(self.shape, dtype=np.int16)                # 0 fluid, k: self._objects[k - 1]
         self._objects = []
@@ -157,7 +202,10 @@ class LatticeBoltzmannBoundaryHandling:
         self._groups = None
         self._faces = None
         self.version = 0           # incremented whenever the link lists are rebuilt
-        self._fuse = True
+        self._fuse = builtin_periodicity
+        self._links_dev = None
+        self._links_version = None
+        self._fd_table = None
 
     @property
     def fuse_noslip_faces(self):
@@ -166,6 +214,8 @@ class LatticeBoltzmannBoundaryHandling:
 
     @fuse_noslip_faces.setter
     def fuse_noslip_faces(self, value):
+        if value and not self.builtin_periodicity:
+            raise ValueError("fused NoSlip faces belong to the step driver's sweeps")
         if bool(value) != self._fuse:
             self._fuse = bool(value)
             self._dirty = True
@@ -199,9 +249,20 @@ class LatticeBoltzmannBoundaryHandling:
         self._dirty = True
 
     def get_mask(self, slice_obj=None, boundary_name="domain", inverse=False):
+        """Interior mask of the fluid ('domain'), of one boundary object (the object or its name), or of every
+        boundary cell (None)."""
         g = self.ghost_layers
         inner = self.flags[(slice(g, -g),) * self.dim] if g else self.flags
-        mask = inner == 0 if boundary_name == "domain" else inner != 0
+        if isinstance(boundary_name, str) and boundary_name == "domain":
+            mask = inner == 0
+        elif boundary_name is None:
+            mask = inner != 0
+        else:
+            ids = [k + 1 for k, o in enumerate(self._objects)
+                   if o is boundary_name or o == boundary_name or getattr(o, "name", None) == boundary_name]
+            if not ids:
+                raise ValueError(f"no boundary {boundary_name!r} is set")
+            mask = np.isin(inner, ids)
         if inverse:
             mask = ~mask
         return mask if slice_obj is None else mask[slice_obj]
@@ -216,6 +277,9 @@ class LatticeBoltzmannBoundaryHandling:
         is_noslip = np.isin(self.flags, noslip_ids) if noslip_ids else np.zeros(self.shape, dtype=bool)
         faces = []
         for a in range(D):
+            if not self.builtin_periodicity:
+                faces.append("ghost")
+                continue
             if self.periodicity[a]:
                 faces.append("periodic")
                 continue
@@ -291,3 +355,104 @@ class LatticeBoltzmannBoundaryHandling:
         if not self._groups:
             return np.zeros((0, 4), dtype=np.int32)
         return np.concatenate([l for _, l, _ in self._groups])
+
+    # ------------------------------------------------------------------ execution
+    def _device_groups(self, device):
+        """Device copies of each boundary object's links (rebuilt when the flags change):
+        [object, links, per-link data or None, ready]."""
+        import torch
+        groups = self.link_groups
+        if self._links_version != self.version:
+            self._links_version = self.version
+            self._links_dev = []
+            for obj, l, v in groups:
+                data = None if v is None else torch.from_numpy(np.ascontiguousarray(v)).to(device)
+                if isinstance(obj, ExtrapolationOutflow):
+                    data = torch.zeros((len(l), 2), dtype=torch.float64, device=device)  # set before first use
+                self._links_dev.append([obj, torch.from_numpy(np.ascontiguousarray(l)).to(device), data,
+                                        not isinstance(obj, ExtrapolationOutflow)])
+        return self._links_dev
+
+    def apply(self, pdf, pattern_code):
+        """Applies every boundary object's links to `pdf` (a [x, y, (z,) q] tensor with this domain's ghost
+        layers) after the kernel with code `pattern_code` (LBM355_PULL ... LBM355_PUSH), on the current
+        stream."""
+        import ctypes
+        import torch
+        groups = self._device_groups(pdf.device)
+        if not groups:
+            return
+        D = self.dim
+        m = self.method
+        if tuple(pdf.shape[:D]) != self.shape:
+            raise ValueError(f"PDF array {tuple(pdf.shape)} does not match the flag field {self.shape}")
+        dtypes = {torch.float64: nat.F64, torch.float32: nat.F32, torch.float16: nat.F16}
+        if pdf.dtype not in dtypes or not pdf.is_cuda:
+            raise TypeError("boundaries run on GPU tensors of float64, float32 or float16")
+        lib = nat.load()
+        faces = self._faces
+        for group in groups:
+            obj, links, vel, ready = group
+            d = nat.BoundaryDesc()
+            d.stencil = nat.STENCIL_NAMES[m.stencil.name]
+            d.pattern = int(pattern_code)
+            d.store_dtype = dtypes[pdf.dtype]
+            d.kind = obj.kind
+            sizes = list(self.domain_size) + [1] * (3 - D)
+            d.nx, d.ny, d.nz = sizes
+            d.ghost_layers = self.ghost_layers
+            for ax in range(3):
+                d.periodic[ax] = int(ax < D and self.builtin_periodicity and bool(self.periodicity[ax]))
+                d.fused_noslip[ax] = int(ax < D and faces[ax] == "noslip")
+            d.flags = (nat.FLAG_ZERO_CENTERED if m.zero_centered_pdfs else 0) | \
+                (nat.FLAG_COMPRESSIBLE if m.compressible else 0)
+            d.stride_y = pdf.stride(1)
+            d.stride_z = pdf.stride(2) if D == 3 else 0
+            d.stride_q = pdf.stride(D)
+            if isinstance(obj, UBB):
+                if not obj.velocity_is_callable:
+                    for ax, v in enumerate(obj._velocity):
+                        d.velocity[ax] = float(v)
+                d.density = float(obj._density) if obj._density is not None else 0.0
+            elif isinstance(obj, FreeSlip):
+                for ax, v in enumerate(obj.normal_direction):
+                    d.normal[ax] = int(v)
+            elif isinstance(obj, ExtrapolationOutflow):
+                for ax, v in enumerate(obj.normal_direction):
+                    d.normal[ax] = int(v)
+                if not ready:
+                    # the state starts from the populations present now (read through the same accessor)
+                    d.kind = nat.BC_OUTFLOW_INIT
+                    nat.check(lib.lbm355_boundary_links(ctypes.byref(d), ctypes.c_void_p(pdf.data_ptr()),
+                                                        ctypes.c_void_p(links.data_ptr()), links.shape[0],
+                                                        ctypes.c_void_p(vel.data_ptr()),
+                                                        nat.current_stream_handle()))
+                    d.kind = nat.BC_OUTFLOW
+                    group[3] = True
+            elif isinstance(obj, FixedDensity):
+                if self._fd_table is None:
+                    from .kernel_tables import fixed_density_table
+                    self._fd_table = torch.from_numpy(fixed_density_table(m)).to(pdf.device)
+                d.density = obj.density
+                d.equilibrium = self._fd_table.data_ptr()
+                fm = m.force_model
+                if fm is not None:
+                    if fm.equilibrium_velocity_shift:
+                        d.flags |= nat.FLAG_EQ_SHIFT
+                    for ax in range(D):
+                        d.force[ax] = float(fm.force[ax])
+            nat.check(lib.lbm355_boundary_links(ctypes.byref(d), ctypes.c_void_p(pdf.data_ptr()),
+                                                ctypes.c_void_p(links.data_ptr()), links.shape[0],
+                                                ctypes.c_void_p(vel.data_ptr() if vel is not None else 0),
+                                                nat.current_stream_handle()))
+
+    def __call__(self, prev_timestep=None, **kwargs):
+        """lbmpy's form: applies the boundaries to the data handling's PDF array after the kernel that ran at
+        `prev_timestep` (boundaryhandling.py:76-96)."""
+        if self._dh is None:
+            raise TypeError("this boundary handling belongs to a LatticeBoltzmannStep, which applies it itself")
+        from .advanced_streaming.utility import kernel_code
+        from .creationfunctions import Timestep
+        if prev_timestep is None:
+            prev_timestep = Timestep.BOTH
+        self.apply(self._dh.gpu_arrays[self._pdf_name], kernel_code(self._pattern, prev_timestep))

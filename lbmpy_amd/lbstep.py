@@ -25,10 +25,10 @@ import numpy as np
 import torch
 
 from . import _native as nat
-from .boundaries import LatticeBoltzmannBoundaryHandling, UBB, FreeSlip, FixedDensity, ExtrapolationOutflow
+from .boundaries import LatticeBoltzmannBoundaryHandling
 from .creationfunctions import (update_with_default_parameters, create_lb_function, Timestep, CreateKernelConfig)
 from .enums import Stencil, Target
-from .kernel_tables import method_table, fixed_density_table
+from .kernel_tables import method_table
 from .stencils import LBStencil
 from .sweep import create_pdf_array, TORCH_DTYPES
 
@@ -113,12 +113,11 @@ class LatticeBoltzmannStep:
         self._u_host = np.zeros(domain_size + (dim,))
         self._rho_dev = torch.zeros([n + 2 for n in domain_size][::-1], dtype=torch.float64, device="cuda")
         self._u_dev = torch.zeros([dim] + [n + 2 for n in domain_size][::-1], dtype=torch.float64, device="cuda")
-        self._boundary_handling = LatticeBoltzmannBoundaryHandling(self.method, domain_size, self.periodicity)
+        self._boundary_handling = LatticeBoltzmannBoundaryHandling.for_domain(self.method, domain_size,
+                                                                             self.periodicity)
         # full NoSlip planes are fused into the pull / AA sweeps; the other patterns take them as links
         self._boundary_handling.fuse_noslip_faces = self._pattern in ("pull", "aa")
         self._aa_next_odd = True
-        self._links_dev = None
-        self._links_version = None
         self.time_steps_run = 0
         self.set_pdf_fields_from_macroscopic_values()
 
@@ -224,83 +223,8 @@ class LatticeBoltzmannStep:
         self._aa_next_odd = True
         torch.cuda.synchronize()
 
-    def _link_groups(self):
-        """Device copies of each boundary object's links (rebuilt when the flags change)."""
-        groups = self._boundary_handling.link_groups
-        key = self._boundary_handling.version
-        if self._links_version != key:
-            self._links_version = key
-            self._links_dev = []
-            for obj, l, v in groups:
-                data = None if v is None else torch.from_numpy(np.ascontiguousarray(v)).cuda()
-                if isinstance(obj, ExtrapolationOutflow):
-                    data = torch.zeros((len(l), 2), dtype=torch.float64, device="cuda")    # set before first use
-                self._links_dev.append([obj, torch.from_numpy(np.ascontiguousarray(l)).cuda(), data,
-                                        not isinstance(obj, ExtrapolationOutflow)])
-        return self._links_dev
-
     def _apply_links(self, pattern):
-        groups = self._link_groups()
-        if not groups:
-            return
-        a = self._src
-        D = self.dim
-        m = self.method
-        lib = nat.load()
-        for group in groups:
-            obj, links, vel, ready = group
-            d = nat.BoundaryDesc()
-            d.stencil = nat.STENCIL_NAMES[m.stencil.name]
-            d.pattern = pattern
-            d.store_dtype = nat.DTYPE_CODES[self._config.data_type]
-            d.kind = obj.kind
-            sizes = list(self._domain_size) + [1] * (3 - D)
-            d.nx, d.ny, d.nz = sizes
-            d.ghost_layers = 1
-            faces = self._boundary_handling.faces
-            for ax in range(3):
-                d.periodic[ax] = int(ax < D and bool(self.periodicity[ax]))
-                d.fused_noslip[ax] = int(ax < D and faces[ax] == "noslip")
-            d.flags = (nat.FLAG_ZERO_CENTERED if m.zero_centered_pdfs else 0) | \
-                (nat.FLAG_COMPRESSIBLE if m.compressible else 0)
-            d.stride_y = a.stride(1)
-            d.stride_z = a.stride(2) if D == 3 else 0
-            d.stride_q = a.stride(D)
-            if isinstance(obj, UBB):
-                if not obj.velocity_is_callable:
-                    for ax, v in enumerate(obj._velocity):
-                        d.velocity[ax] = float(v)
-                d.density = float(obj._density) if obj._density is not None else 0.0
-            elif isinstance(obj, FreeSlip):
-                for ax, v in enumerate(obj.normal_direction):
-                    d.normal[ax] = int(v)
-            elif isinstance(obj, ExtrapolationOutflow):
-                for ax, v in enumerate(obj.normal_direction):
-                    d.normal[ax] = int(v)
-                if not ready:
-                    # the state starts from the populations present now (read through the same accessor)
-                    d.kind = nat.BC_OUTFLOW_INIT
-                    nat.check(lib.lbm355_boundary_links(ctypes.byref(d), ctypes.c_void_p(a.data_ptr()),
-                                                        ctypes.c_void_p(links.data_ptr()), links.shape[0],
-                                                        ctypes.c_void_p(vel.data_ptr()),
-                                                        nat.current_stream_handle()))
-                    d.kind = nat.BC_OUTFLOW
-                    group[3] = True
-            elif isinstance(obj, FixedDensity):
-                if getattr(self, "_fd_table", None) is None:
-                    self._fd_table = torch.from_numpy(fixed_density_table(m)).cuda()
-                d.density = obj.density
-                d.equilibrium = self._fd_table.data_ptr()
-                fm = m.force_model
-                if fm is not None:
-                    if fm.equilibrium_velocity_shift:
-                        d.flags |= nat.FLAG_EQ_SHIFT
-                    for ax in range(D):
-                        d.force[ax] = float(fm.force[ax])
-            nat.check(lib.lbm355_boundary_links(ctypes.byref(d), ctypes.c_void_p(a.data_ptr()),
-                                                ctypes.c_void_p(links.data_ptr()), links.shape[0],
-                                                ctypes.c_void_p(vel.data_ptr() if vel is not None else 0),
-                                                nat.current_stream_handle()))
+        self._boundary_handling.apply(self._src, pattern)
 
     # ------------------------------------------------------------------ stepping
     def pre_run(self):

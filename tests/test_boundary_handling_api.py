@@ -1,0 +1,77 @@
+"""lbmpy's form of LatticeBoltzmannBoundaryHandling (boundaries/boundaryhandling.py:15-96) without a GPU:
+constructor from a data handling and a PDF field name, links built on the ghost-layer flag field without
+wrapping (periodicity comes from ghost copies in such loops), no fused NoSlip faces, and get_mask by object
+or name.  The step driver's form (for_domain) keeps wrapped links on periodic axes."""
+import numpy as np
+import pytest
+import torch
+
+from lbmpy_amd import LBMConfig, LBStencil, Stencil, Method, create_lb_method
+from lbmpy_amd.advanced_streaming import GpuArrays
+from lbmpy_amd.boundaries import LatticeBoltzmannBoundaryHandling, NoSlip, UBB, slice_from_direction
+
+
+def _cpu_pdf(shape, Q):
+    store = torch.zeros((Q,) + tuple(reversed(shape)), dtype=torch.float64)
+    return store.permute(*range(len(shape), 0, -1), 0)
+
+
+def _method(stencil="D2Q9"):
+    return create_lb_method(LBMConfig(stencil=LBStencil(Stencil[stencil]), method=Method.SRT, relaxation_rate=1.5))
+
+
+def test_lbmpy_form_links_do_not_wrap_and_faces_are_ghost():
+    m = _method()
+    arr = _cpu_pdf((8 + 2, 6 + 2), 9)
+    dh = GpuArrays((True, False), pdfs=arr)
+    bh = LatticeBoltzmannBoundaryHandling(m, dh, "pdfs", streaming_pattern="esotwist")
+    assert bh.domain_size == (8, 6) and bh.shape == (10, 8)
+    bh.set_boundary(NoSlip(), slice_from_direction("N", 2))
+    bh.set_boundary(UBB((0.01, 0)), slice_from_direction("S", 2))
+    assert bh.faces == ["ghost", "ghost"]
+    assert not bh.fuse_noslip_faces
+    with pytest.raises(ValueError):
+        bh.fuse_noslip_faces = True
+    links = bh.links
+    # every link points into a flagged ghost cell at the same x (no wrap: x neighbours outside the interior
+    # reach ghost columns, which are flagged too, as lbmpy's flag field is)
+    c = np.array(LBStencil(Stencil.D2Q9).stencil_entries)
+    nb = links[:, :2] + c[links[:, 3]]
+    assert np.all(bh.flags[nb[:, 0], nb[:, 1]] > 0)
+    assert nb[:, 0].min() == 0 and nb[:, 0].max() == 9           # ghost columns are reached, not wrapped
+    mask_n = bh.get_mask(None, NoSlip())
+    mask_u = bh.get_mask(None, "UBB")
+    assert mask_n.shape == (8, 6) and not mask_n.any() and not mask_u.any()   # flags only on ghost rows
+    assert bh.get_mask(None, "domain").all()
+    with pytest.raises(ValueError):
+        bh.get_mask(None, "nothing")
+    with pytest.raises(TypeError):
+        LatticeBoltzmannBoundaryHandling.for_domain(m, (8, 6), (True, False))()
+
+
+def test_step_driver_form_wraps_periodic_axes():
+    m = _method()
+    bh = LatticeBoltzmannBoundaryHandling.for_domain(m, (8, 6), (True, False))
+    bh.set_boundary(NoSlip(), slice_from_direction("N", 2))
+    bh.set_boundary(NoSlip(), slice_from_direction("S", 2))
+    assert bh.faces == ["periodic", "noslip"] and bh.fuse_noslip_faces
+    assert len(bh.links) == 0                                     # fused into the sweep
+    bh.fuse_noslip_faces = False
+    links = bh.links
+    c = np.array(LBStencil(Stencil.D2Q9).stencil_entries)
+    nb = links[:, :2] + c[links[:, 3]]
+    assert nb[:, 0].min() >= 0 and nb[:, 1].min() == 0 and nb[:, 1].max() == 7
+    assert bh.faces == ["periodic", "ghost"]
+
+
+def test_obstacle_mask_by_object():
+    m = _method("D3Q19")
+    arr = _cpu_pdf((6, 6, 6), 19)
+    bh = LatticeBoltzmannBoundaryHandling(m, GpuArrays((True, True, True), pdfs=arr), "pdfs")
+    wall = NoSlip()
+    bh.set_boundary(wall, mask_callback=lambda x, y, z: (x == 1) & (y == 1) & (z == 1))
+    mask = bh.get_mask(None, wall)
+    assert mask.sum() == 1 and mask[1, 1, 1]
+    assert len(bh.links) == 18                                     # every direction but the centre
+    with pytest.raises(ValueError):
+        LatticeBoltzmannBoundaryHandling(m, GpuArrays((True,) * 3, pdfs=arr), "pdfs", streaming_pattern="twist")
