@@ -195,8 +195,9 @@ class LatticeBoltzmannBoundaryHandling:
         self.periodicity = tuple(periodicity)
         self.ghost_layers = ghost_layers
         self.builtin_periodicity = builtin_periodicity
-        self.shape = tuple(n + 2 * ghost_layers for n in domain_size)   # index [x, y, (z)]
-        self.flags = np.zeros(self.shape, dtype=np.int16)                # 0 fluid, k: self._objects[k - 1]
+        self.shape = tuple(domain_size)                                 # lbmpy's: the domain without ghost layers
+        self.flag_shape = tuple(n + 2 * ghost_layers for n in domain_size)   # index [x, y, (z)], ghosts included
+        self.flags = np.zeros(self.flag_shape, dtype=np.int16)          # 0 fluid, k: self._objects[k - 1]
         self._objects = []
         self._dirty = True
         self._groups = None
@@ -229,6 +230,10 @@ class LatticeBoltzmannBoundaryHandling:
 
     def set_boundary(self, boundary_obj, slice_obj=None, mask_callback=None, ghost_layers=True,
                      inner_ghost_layers=True, replace=True, force_flag_value=None):
+        """Flags the cells of `slice_obj` (ghost-layer inclusive indices) or, with `mask_callback`, those of them
+        where the callback, given the cells' midpoint coordinates, returns True (boundaryhandling.py:218-266).
+        Returns the boundary object's flag value (its id here, not a bit).  A cell holds one boundary: the
+        last one set wins."""
         if not isinstance(boundary_obj, (NoSlip, UBB, FreeSlip, FixedDensity, ExtrapolationOutflow)):
             raise NotImplementedError(f"boundary {boundary_obj!r} is not available on the GPU path "
                                       "(NoSlip, UBB, FreeSlip, FixedDensity and ExtrapolationOutflow are)")
@@ -239,14 +244,17 @@ class LatticeBoltzmannBoundaryHandling:
             slice_obj = (slice(None),) * self.dim
         view = self.flags[slice_obj]
         if mask_callback is not None:
+            # cell midpoints, ghost layers included: interior cell i of an axis sits at i + 0.5 (pystencils'
+            # midpoint_arrays, which lbmpy's set_boundary passes to the callback)
             g = self.ghost_layers
-            coords = np.meshgrid(*[np.arange(n) - g for n in self.shape], indexing="ij")
+            coords = np.meshgrid(*[np.arange(n) - g + 0.5 for n in self.flag_shape], indexing="ij")
             sub = tuple(c[slice_obj] for c in coords)
             mask = np.asarray(mask_callback(*sub), dtype=bool)
             view[mask] = bid
         else:
             view[...] = bid
         self._dirty = True
+        return bid
 
     def get_mask(self, slice_obj=None, boundary_name="domain", inverse=False):
         """Interior mask of the fluid ('domain'), of one boundary object (the object or its name), or of every
@@ -274,7 +282,7 @@ class LatticeBoltzmannBoundaryHandling:
         g = self.ghost_layers
         D = self.dim
         noslip_ids = [k + 1 for k, o in enumerate(self._objects) if isinstance(o, NoSlip)]
-        is_noslip = np.isin(self.flags, noslip_ids) if noslip_ids else np.zeros(self.shape, dtype=bool)
+        is_noslip = np.isin(self.flags, noslip_ids) if noslip_ids else np.zeros(self.flag_shape, dtype=bool)
         faces = []
         for a in range(D):
             if not self.builtin_periodicity:
@@ -302,7 +310,7 @@ class LatticeBoltzmannBoundaryHandling:
         st = self.method.stencil
         c = np.array(st.stencil_entries)
         inner = tuple(slice(g, g + n) for n in self.domain_size)
-        fluid = np.zeros(self.shape, dtype=bool)
+        fluid = np.zeros(self.flag_shape, dtype=bool)
         fluid[inner] = self.flags[inner] == 0
         cells = np.argwhere(fluid)                                   # ghost coordinates
         per_id = {}
@@ -384,8 +392,8 @@ class LatticeBoltzmannBoundaryHandling:
             return
         D = self.dim
         m = self.method
-        if tuple(pdf.shape[:D]) != self.shape:
-            raise ValueError(f"PDF array {tuple(pdf.shape)} does not match the flag field {self.shape}")
+        if tuple(pdf.shape[:D]) != self.flag_shape:
+            raise ValueError(f"PDF array {tuple(pdf.shape)} does not match the flag field {self.flag_shape}")
         dtypes = {torch.float64: nat.F64, torch.float32: nat.F32, torch.float16: nat.F16}
         if pdf.dtype not in dtypes or not pdf.is_cuda:
             raise TypeError("boundaries run on GPU tensors of float64, float32 or float16")

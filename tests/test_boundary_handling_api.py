@@ -8,7 +8,7 @@ import torch
 
 from lbmpy_amd import LBMConfig, LBStencil, Stencil, Method, create_lb_method
 from lbmpy_amd.advanced_streaming import GpuArrays
-from lbmpy_amd.boundaries import LatticeBoltzmannBoundaryHandling, NoSlip, UBB, slice_from_direction
+from lbmpy_amd.boundaries import LatticeBoltzmannBoundaryHandling, NoSlip, UBB, FreeSlip, slice_from_direction
 
 
 def _cpu_pdf(shape, Q):
@@ -25,7 +25,7 @@ def test_lbmpy_form_links_do_not_wrap_and_faces_are_ghost():
     arr = _cpu_pdf((8 + 2, 6 + 2), 9)
     dh = GpuArrays((True, False), pdfs=arr)
     bh = LatticeBoltzmannBoundaryHandling(m, dh, "pdfs", streaming_pattern="esotwist")
-    assert bh.domain_size == (8, 6) and bh.shape == (10, 8)
+    assert bh.domain_size == bh.shape == (8, 6) and bh.flag_shape == (10, 8)
     bh.set_boundary(NoSlip(), slice_from_direction("N", 2))
     bh.set_boundary(UBB((0.01, 0)), slice_from_direction("S", 2))
     assert bh.faces == ["ghost", "ghost"]
@@ -69,9 +69,32 @@ def test_obstacle_mask_by_object():
     arr = _cpu_pdf((6, 6, 6), 19)
     bh = LatticeBoltzmannBoundaryHandling(m, GpuArrays((True, True, True), pdfs=arr), "pdfs")
     wall = NoSlip()
-    bh.set_boundary(wall, mask_callback=lambda x, y, z: (x == 1) & (y == 1) & (z == 1))
+    bh.set_boundary(wall, mask_callback=lambda x, y, z: (x == 1.5) & (y == 1.5) & (z == 1.5))   # midpoints
     mask = bh.get_mask(None, wall)
     assert mask.sum() == 1 and mask[1, 1, 1]
     assert len(bh.links) == 18                                     # every direction but the centre
     with pytest.raises(ValueError):
         LatticeBoltzmannBoundaryHandling(m, GpuArrays((True,) * 3, pdfs=arr), "pdfs", streaming_pattern="twist")
+
+
+# (x, y, dir) of lbmpy's index list for a D2Q9 4x4 domain whose cells with midpoints farther than 2 from (2, 2)
+# are boundary cells (tests/test_boundary_handling.py:303-341, the first three columns of its expected
+# list; the remaining columns are FreeSlip's per-link normals).  Ghost-layer coordinates.
+_CONVEX_CORNER_LINKS = [
+    (2, 1, 2), (2, 1, 3), (2, 1, 7), (2, 1, 8), (3, 1, 2), (3, 1, 4), (3, 1, 7), (3, 1, 8), (1, 2, 2), (1, 2, 3),
+    (1, 2, 5), (1, 2, 7), (2, 2, 7), (3, 2, 8), (4, 2, 2), (4, 2, 4), (4, 2, 6), (4, 2, 8), (1, 3, 1), (1, 3, 3),
+    (1, 3, 5), (1, 3, 7), (2, 3, 5), (3, 3, 6), (4, 3, 1), (4, 3, 4), (4, 3, 6), (4, 3, 8), (2, 4, 1), (2, 4, 3),
+    (2, 4, 5), (2, 4, 6), (3, 4, 1), (3, 4, 4), (3, 4, 5), (3, 4, 6)]
+
+
+def test_links_match_lbmpy_index_list_convex_corner():
+    m = _method()
+    arr = _cpu_pdf((6, 6), 9)
+    bh = LatticeBoltzmannBoundaryHandling(m, GpuArrays((False, False), src=arr), "src")
+
+    def callback(x, y):
+        return (x - 2) ** 2 + (y - 2) ** 2 > 2 ** 2
+
+    bh.set_boundary(FreeSlip(LBStencil(Stencil.D2Q9), normal_direction=(0, 1)), mask_callback=callback)
+    got = sorted(tuple(int(v) for v in (l[0], l[1], l[3])) for l in bh.links)
+    assert got == sorted(_CONVEX_CORNER_LINKS)

@@ -83,8 +83,8 @@ def test_obstacle_across_periodic_face(pattern):
     step = _make((12, 10), (True, False), lc)
     for w in ("N", "S"):
         step.boundary_handling.set_boundary(NoSlip(), slice_from_direction(w, 2))
-    step.boundary_handling.set_boundary(NoSlip(), mask_callback=lambda x, y: ((x <= 1) | (x >= 11)) & (y >= 3) &
-                                        (y <= 5))
+    step.boundary_handling.set_boundary(NoSlip(), mask_callback=lambda x, y: ((x < 2) | (x > 11)) & (y > 3) &
+                                        (y < 6))
     print(f"periodic obstacle {pattern}", flush=True)
     step.run(30)
     m = step.method

@@ -44,7 +44,8 @@ def test_noslip_single_cell(stencil, pattern, prev_timestep):
     method = create_lb_method(LBMConfig(stencil=st, method=Method.SRT))
     bh = LatticeBoltzmannBoundaryHandling(method, GpuArrays((False,) * D, pdfs=pdfs), "pdfs",
                                           streaming_pattern=pattern)
-    bh.set_boundary(NoSlip(), mask_callback=lambda *xs: np.logical_or.reduce([x != 0 for x in xs]))
+    # every cell but the single interior one (midpoint 0.5 on each axis)
+    bh.set_boundary(NoSlip(), mask_callback=lambda *xs: np.logical_or.reduce([np.abs(x - 0.5) > 0.25 for x in xs]))
     assert len(bh.links) == Q - 1
     bh(prev_timestep=prev_timestep)
     out = pdfs.cpu()
