@@ -47,8 +47,11 @@ class NoSlip(LbBoundary):
 class UBB(LbBoundary):
     """Velocity bounce-back: f_inv(d)(x, t + 1) = f*_d(x, t) - 2 w_d rho_w (c_d . u_w) / c_s^2.
 
-    ``velocity``: the wall velocity, a tuple, or a callable that receives the fluid cells' interior
-    coordinates of the links (arrays x, y[, z]) and the link directions and returns an (n, dim) array.
+    ``velocity``: the wall velocity, a tuple, or a callable.  A callable of one argument is called as lbmpy calls
+    it, with the links' boundary data (``BoundaryData``: fields 'x', 'y', ('z',) 'dir', methods
+    ``link_positions`` etc.), and sets ``data['vel_0']`` ...; a callable of more arguments receives the fluid
+    cells' interior coordinates of the links (arrays x, y[, z]) and the link directions and returns an
+    (n, dim) array.
     ``density``: wall density; if None it is computed from the fluid cell (compressible methods) or 1."""
     kind = nat.BC_UBB
 
@@ -129,6 +132,65 @@ class ExtrapolationOutflow(LbBoundary):
         self.dim = dim
         self.lb_method = lb_method
         super().__init__(name or f"Outflow: {normal_direction}")
+
+
+def _takes_boundary_data(callback):
+    """lbmpy's UBB callbacks take one argument, the boundary data; this package's take coordinates and
+    directions."""
+    import inspect
+    try:
+        params = inspect.signature(callback).parameters.values()
+    except (TypeError, ValueError):
+        return False
+    positional = [p for p in params if p.kind in (p.POSITIONAL_ONLY, p.POSITIONAL_OR_KEYWORD)]
+    if any(p.kind == p.VAR_POSITIONAL for p in params):
+        return False
+    return len([p for p in positional if p.default is p.empty]) == 1
+
+
+class BoundaryData:
+    """What an lbmpy-style UBB velocity callback receives (pystencils' boundary data setter, which lbmpy's
+    UBB.additional_data_init_callback is called with): the links of one boundary object as record fields 'x',
+    'y', ('z',) 'dir' (ghost-layer coordinates of the fluid cells) and the velocity fields 'vel_0' ... to be set,
+    plus the positions of the fluid-cell midpoints, of the links' wall crossings and of the boundary cells,
+    in the midpoint coordinates the mask callbacks see."""
+
+    def __init__(self, links, stencil, ghost_layers, dim):
+        self._links = links
+        self._c = np.array(stencil.stencil_entries)
+        self._g = ghost_layers
+        self.dim = dim
+        self._vel = {f"vel_{a}": np.zeros(len(links)) for a in range(dim)}
+
+    def __len__(self):
+        return len(self._links)
+
+    def __getitem__(self, key):
+        axes = {"x": 0, "y": 1, "z": 2}
+        if key in axes and axes[key] < self.dim:
+            return self._links[:, axes[key]].copy()
+        if key == "dir":
+            return self._links[:, 3].copy()
+        if key in self._vel:
+            return self._vel[key]
+        raise KeyError(key)
+
+    def __setitem__(self, key, value):
+        if key not in self._vel:
+            raise KeyError(f"{key}: a UBB callback sets {', '.join(self._vel)}")
+        self._vel[key][...] = value
+
+    def link_offsets(self):
+        return self._c[self._links[:, 3], :self.dim]
+
+    def fluid_cell_positions(self, coord):
+        return self._links[:, coord] - self._g + 0.5
+
+    def link_positions(self, coord):
+        return self.fluid_cell_positions(coord) + 0.5 * self._c[self._links[:, 3], coord]
+
+    def boundary_cell_positions(self, coord):
+        return self.fluid_cell_positions(coord) + self._c[self._links[:, 3], coord]
 
 
 def slice_from_direction(direction_name, dim, normal_offset=0, tangential_offset=0):
@@ -339,8 +401,13 @@ class LatticeBoltzmannBoundaryHandling:
             links = np.concatenate(blocks)
             vel = None
             if isinstance(obj, UBB) and obj.velocity_is_callable:
-                coords = [links[:, a] - g for a in range(D)]
-                v = np.asarray(obj._velocity(*coords, links[:, 3]), dtype=np.float64).reshape(len(links), D)
+                if _takes_boundary_data(obj._velocity):
+                    data = BoundaryData(links, st, g, D)
+                    obj._velocity(data)
+                    v = np.stack([data[f"vel_{a}"] for a in range(D)], axis=1)
+                else:
+                    coords = [links[:, a] - g for a in range(D)]
+                    v = np.asarray(obj._velocity(*coords, links[:, 3]), dtype=np.float64).reshape(len(links), D)
                 vel = np.zeros((len(links), 3))
                 vel[:, :D] = v
             groups.append((obj, links, vel))

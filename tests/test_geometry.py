@@ -72,3 +72,31 @@ def test_velocity_fields():
     np.random.seed(0)
     s = get_shear_flow_velocity_field((9, 12), 0.04)
     assert np.all(s[:, 4:8, 0] == -0.04) and np.all(s[:, :4, 0] == 0.04) and np.all(np.abs(s[..., 1]) <= 0.0002)
+
+
+def test_lbmpy_style_ubb_callback_equals_coordinate_callback():
+    """lbmpy's callback form (boundary data with link_positions and vel_i fields, as in lbmpy's
+    add_pipe_inflow_boundary) gives the same link velocities as the coordinate form."""
+    from lbmpy_amd.boundaries import UBB
+    u_max, radius = 0.05, 5
+
+    def lbmpy_style(boundary_data):
+        boundary_data["vel_1"] = 0.0
+        y = boundary_data.link_positions(1)
+        boundary_data["vel_0"] = u_max * (1 - ((y - radius) / radius) ** 2)
+
+    c = np.array(LBStencil(Stencil.D2Q9).stencil_entries)
+
+    def coordinate_style(x, y, d):
+        v = np.zeros((len(d), 2))
+        v[:, 0] = u_max * (1 - ((y + 0.5 + 0.5 * c[d, 1] - radius) / radius) ** 2)
+        return v
+
+    vels = []
+    for cb in (lbmpy_style, coordinate_style):
+        bh = _bh((20, 10), (False, False))
+        bh.set_boundary(UBB(cb, dim=2), slice_from_direction("W", 2))
+        (_, links, vel), = [(o, l, v) for o, l, v in bh.link_groups if v is not None]
+        vels.append(vel)
+    np.testing.assert_array_equal(vels[0], vels[1])
+    assert vels[0][:, 0].max() > 0.04
