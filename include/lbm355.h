@@ -158,6 +158,15 @@ int lbm355_set_equilibrium(const lbm355_sweep_desc* desc, void* pdf, const doubl
 int lbm355_get_macroscopic(const lbm355_sweep_desc* desc, const void* pdf, double* density, double* velocity,
                            int32_t last_pattern, void* stream);
 
+/* Strain-rate tensor S_ab = -1.5 (omega_shear / rho_ref) sum_i c_ia c_ib (f_i - f_eq,i(rho, u)) of the populations
+ * the kernel `last_pattern` wrote (read as lbm355_get_macroscopic reads them; rho and the equilibrium velocity u
+ * as the collision computes them, rho_ref = rho if compressible, else 1): lbmpy's strain_rate_tensor_getter
+ * (macroscopic_value_kernels.py:67-85).  table_f64: the FP64 method table the setter takes.  strain_rate: FP64
+ * D*D components (a*D + b), element strides of desc->out_stride_y / out_stride_z, component stride
+ * out_stride_c, ghost-layer indexed like the density output. */
+int lbm355_get_strain_rate(const lbm355_sweep_desc* desc, const void* pdf, const double* table_f64, double* strain_rate,
+                           double omega_shear, int32_t last_pattern, void* stream);
+
 /* Plain FP32 copy of n_floats elements, with regular (nontemporal = 0) or non-temporal accesses as
  * the sweeps use them.  No lbmpy counterpart: a copy of known size that calibrates achievable
  * bandwidth and the FETCH_SIZE / WRITE_SIZE counters against the sweep. */

@@ -59,7 +59,7 @@ EXPORTED = ("lbm355_sweep", "lbm355_table_elements", "lbm355_noslip_links", "lbm
             "lbm355_batch_info", "lbm355_batch_destroy", "lbm355_closure_params_size",
             "lbm355_closure_master_size", "lbm355_closure_features", "lbm355_closure_forward",
             "lbm355_closure_train_step", "lbm355_closure_adam", "lbm355_boundary_links", "lbm355_copy_boxes",
-            "lbm355_pattern_access")
+            "lbm355_pattern_access", "lbm355_get_strain_rate")
 
 _lib = None
 
@@ -115,6 +115,9 @@ def load(path=None):
     lib.lbm355_boundary_links.argtypes = [ctypes.POINTER(BoundaryDesc), c_void_p, c_void_p, c_int64, c_void_p,
                                           c_void_p]
     lib.lbm355_boundary_links.restype = c_int32
+    lib.lbm355_get_strain_rate.argtypes = [ctypes.POINTER(SweepDesc), c_void_p, c_void_p, c_void_p, c_double, c_int32,
+                                           c_void_p]
+    lib.lbm355_get_strain_rate.restype = c_int32
     lib.lbm355_copy_boxes.argtypes = [c_int32, c_void_p, c_int64, c_int64, c_int64, c_int64, c_void_p, c_int32,
                                       c_void_p]
     lib.lbm355_copy_boxes.restype = c_int32

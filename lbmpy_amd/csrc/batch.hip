@@ -68,11 +68,15 @@ struct BatchEntry {
 
 // FP32 occupancy target: 6 waves per SIMD for the moment family up to 19 directions (the compiler
 // otherwise keeps more VGPRs than the argument loads and index decoding need; 80 hold everything without
-// spills), 4 for the cumulant collision (128 VGPRs), 3 for the D3Q27 moment family; FP64 arithmetic takes
-// what it needs.  Every instance compiles without spills.
+// spills), 4 for the cumulant collision (128 VGPRs), 3 for the D3Q27 moment family and for the central-moment
+// family beyond D2Q9 (its chimera transforms need up to 168 VGPRs); FP64 arithmetic takes what it needs.
+// Every instance compiles without spills.
 template <class S, int FAM, int PAT, class TS, class TC>
 __host__ __device__ constexpr int batch_min_waves() {
-  return sizeof(TC) == 8 ? 1 : (FAM == LBM355_FAMILY_MOMENT ? (S::Q <= 19 ? 6 : 3) : 4);
+  if (sizeof(TC) == 8) return 1;
+  if (FAM == LBM355_FAMILY_MOMENT) return S::Q <= 19 ? 6 : 3;
+  if (FAM == LBM355_FAMILY_CENTRAL) return S::Q <= 9 ? 4 : 3;
+  return 4;
 }
 
 template <class S, int FAM, int PAT, class TS, class TC>
@@ -146,6 +150,10 @@ void* batch_pick(const lbm355_sweep_desc* d, int pattern) {
     if (d->stencil == LBM355_D3Q27) return batch_pick_store<D3Q27, LBM355_FAMILY_MOMENT, TC>(d->store_dtype, pattern);
   } else if (d->family == LBM355_FAMILY_CUMULANT && d->stencil == LBM355_D3Q27) {
     return batch_pick_store<D3Q27, LBM355_FAMILY_CUMULANT, TC>(d->store_dtype, pattern);
+  } else if (d->family == LBM355_FAMILY_CENTRAL) {
+    if (d->stencil == LBM355_D2Q9) return batch_pick_store<D2Q9, LBM355_FAMILY_CENTRAL, TC>(d->store_dtype, pattern);
+    if (d->stencil == LBM355_D3Q19) return batch_pick_store<D3Q19, LBM355_FAMILY_CENTRAL, TC>(d->store_dtype, pattern);
+    if (d->stencil == LBM355_D3Q27) return batch_pick_store<D3Q27, LBM355_FAMILY_CENTRAL, TC>(d->store_dtype, pattern);
   }
   return nullptr;
 }

@@ -42,27 +42,13 @@ __device__ __forceinline__ double phi(int c, double u) {
   return c == 0 ? (2.0 / 3.0 - u * u) : 0.5 * (1.0 / 3.0 + u * u + c * u);
 }
 
-template <class S, int FAM, int PAT, class TS>
-__global__ void set_equilibrium_kernel(TS* pdf, const double* rho_in, const double* u_in, const double* tab, int nx,
-                                       int ny, int nz, int g, int64_t sy, int64_t sz, int64_t sq, int64_t osy,
-                                       int64_t osz, int64_t osc, int flags, double fx, double fy, double fz,
-                                       int face_x, int face_y, int face_z) {
+// Equilibrium populations in storage form (minus the weights when zero-centred) for density rho and equilibrium
+// velocity u, no force shift applied, per collision family.
+template <class S, int FAM>
+__device__ __forceinline__ void equilibrium_storage(const double* tab, double rho, const double (&u)[3], int flags,
+                                                    double (&feq)[S::Q]) {
   constexpr int Q = S::Q, D = S::D;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= nx) return;
-  const int y = blockIdx.y, z = blockIdx.z;
-  const int64_t o = (int64_t)(x + g) + (int64_t)(y + g) * osy + (D == 3 ? (int64_t)(z + g) * osz : 0);
-  const double rho = rho_in[o];
   const bool comp = flags & LBM355_FLAG_COMPRESSIBLE;
-  const double ref = comp ? rho : 1.0;
-  const double F[3] = {fx, fy, fz};
-  double u[3] = {0, 0, 0};
-#pragma unroll
-  for (int d = 0; d < D; ++d) {
-    u[d] = u_in[o + d * osc];
-    if (flags & LBM355_FLAG_FORCE) u[d] -= 0.5 * F[d] / ref;
-  }
-  double feq[Q];
   if (FAM == LBM355_FAMILY_MOMENT) {
     moment_equilibrium<S>(tab, rho - 1.0, comp ? rho : 1.0, u, feq);
   } else if (FAM == LBM355_FAMILY_CENTRAL) {
@@ -93,6 +79,30 @@ __global__ void set_equilibrium_kernel(TS* pdf, const double* rho_in, const doub
       feq[i] = zc ? fi - weight<S>(i) : fi;
     }
   }
+}
+
+template <class S, int FAM, int PAT, class TS>
+__global__ void set_equilibrium_kernel(TS* pdf, const double* rho_in, const double* u_in, const double* tab, int nx,
+                                       int ny, int nz, int g, int64_t sy, int64_t sz, int64_t sq, int64_t osy,
+                                       int64_t osz, int64_t osc, int flags, double fx, double fy, double fz,
+                                       int face_x, int face_y, int face_z) {
+  constexpr int Q = S::Q, D = S::D;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= nx) return;
+  const int y = blockIdx.y, z = blockIdx.z;
+  const int64_t o = (int64_t)(x + g) + (int64_t)(y + g) * osy + (D == 3 ? (int64_t)(z + g) * osz : 0);
+  const double rho = rho_in[o];
+  const bool comp = flags & LBM355_FLAG_COMPRESSIBLE;
+  const double ref = comp ? rho : 1.0;
+  const double F[3] = {fx, fy, fz};
+  double u[3] = {0, 0, 0};
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    u[d] = u_in[o + d * osc];
+    if (flags & LBM355_FLAG_FORCE) u[d] -= 0.5 * F[d] / ref;
+  }
+  double feq[Q];
+  equilibrium_storage<S, FAM>(tab, rho, u, flags, feq);
   const AxisNb ax = axis_neighbours(x, nx, g, face_x);
   const AxisNb ay = axis_neighbours(y, ny, g, face_y);
   AxisNb az = {0, 0, false, false};
@@ -158,6 +168,65 @@ __global__ void get_macroscopic_kernel(const TS* pdf, double* rho_out, double* u
   }
 }
 
+// Strain-rate tensor of one cell (macroscopic_value_kernels.py:67-85): the populations where the kernel `pat`
+// wrote them (as get_macroscopic_kernel reads them, pattern chosen at run time: a diagnostic, not a hot loop),
+// density and equilibrium velocity as the collision computes them, f_neq = f - f_eq(rho, u),
+// Pi_ab = sum_i c_ia c_ib f_neq,i and S_ab = -1.5 (omega / rho_ref) Pi_ab, rho_ref = rho if compressible, else 1.
+template <class S, int FAM, class TS>
+__global__ void strain_rate_kernel(const TS* pdf, const double* tab, double* out, int pat, int nx, int ny, int nz,
+                                   int g, int64_t sy, int64_t sz, int64_t sq, int64_t osy, int64_t osz, int64_t osc,
+                                   int fx_mode, int fy_mode, int fz_mode, int flags, double fx, double fy, double fz,
+                                   double omega) {
+  constexpr int Q = S::Q, D = S::D;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= nx) return;
+  const int y = blockIdx.y, z = blockIdx.z;
+  const AxisNb ax = axis_neighbours(x, nx, g, fx_mode);
+  const AxisNb ay = axis_neighbours(y, ny, g, fy_mode);
+  AxisNb az = {0, 0, false, false};
+  if (D == 3) az = axis_neighbours(z, nz, g, fz_mode);
+  const int64_t X[3] = {ax.lo, x + g, ax.hi};
+  const int64_t Y[3] = {ay.lo * sy, (y + g) * sy, ay.hi * sy};
+  const int64_t Z[3] = {D == 3 ? az.lo * sz : 0, D == 3 ? (z + g) * sz : 0, D == 3 ? az.hi * sz : 0};
+  const int64_t center = X[1] + Y[1] + Z[1];
+  double f[Q], s = 0, j[3] = {0, 0, 0};
+  for (int i = 0; i < Q; ++i) {
+    const int cx = S::c[i][0], cy = S::c[i][1], cz = S::c[i][2];
+    double v;
+    if (pat != LBM355_AA_ODD) {
+      const Access t = pattern_access<S>(pat, i);
+      v = (double)pdf[t.wslot * sq + X[1 + t.w[0]] + Y[1 + t.w[1]] + Z[1 + t.w[2]]];
+    } else {
+      const bool wall = (cx == -1 && ax.wall_lo) || (cx == 1 && ax.wall_hi) || (cy == -1 && ay.wall_lo) ||
+                        (cy == 1 && ay.wall_hi) || (cz == -1 && az.wall_lo) || (cz == 1 && az.wall_hi);
+      v = wall ? (double)pdf[S::inv[i] * sq + center] : (double)pdf[i * sq + X[1 + cx] + Y[1 + cy] + Z[1 + cz]];
+    }
+    f[i] = v;
+    s += v;
+    j[0] += cx * v;
+    j[1] += cy * v;
+    j[2] += cz * v;
+  }
+  const double rho = (flags & LBM355_FLAG_ZERO_CENTERED) ? 1.0 + s : s;
+  const double ref = (flags & LBM355_FLAG_COMPRESSIBLE) ? rho : 1.0;
+  const double F[3] = {fx, fy, fz};
+  const bool shift = (flags & LBM355_FLAG_FORCE) && (flags & LBM355_FLAG_EQ_SHIFT);
+  double u[3] = {0, 0, 0};
+  for (int d = 0; d < D; ++d) u[d] = j[d] / ref + (shift ? 0.5 * F[d] / ref : 0.0);
+  double feq[Q];
+  equilibrium_storage<S, FAM>(tab, rho, u, flags, feq);
+  double pi[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (int i = 0; i < Q; ++i) {
+    const double n = f[i] - feq[i];
+    for (int a = 0; a < D; ++a)
+      for (int b = 0; b < D; ++b) pi[a][b] += S::c[i][a] * S::c[i][b] * n;
+  }
+  const double k = -1.5 * omega / ref;
+  const int64_t o = (int64_t)(x + g) + (int64_t)(y + g) * osy + (D == 3 ? (int64_t)(z + g) * osz : 0);
+  for (int a = 0; a < D; ++a)
+    for (int b = 0; b < D; ++b) out[o + (a * D + b) * osc] = k * pi[a][b];
+}
+
 }  // namespace lbm355
 
 using namespace lbm355;
@@ -220,6 +289,22 @@ int get_dtype(const lbm355_sweep_desc* d, const void* pdf, double* rho, double* 
   if (d->store_dtype == LBM355_F32) return get_launch<S, float>(d, pdf, rho, u, last, s);
   return get_launch<S, _Float16>(d, pdf, rho, u, last, s);
 }
+
+template <class S, int FAM>
+int strain_launch(const lbm355_sweep_desc* d, const void* pdf, const double* tab, double* out, double omega,
+                  int last, hipStream_t s) {
+  const int bx = d->nx >= 256 ? 256 : ((d->nx + 63) / 64) * 64;
+  dim3 block(bx), grid((d->nx + bx - 1) / bx, d->ny, d->nz);
+  auto args = [&](auto kernel, auto* p) {
+    hipLaunchKernelGGL(kernel, grid, block, 0, s, p, tab, out, last, d->nx, d->ny, d->nz, d->ghost_layers,
+                       d->stride_y, d->stride_z, d->stride_q, d->out_stride_y, d->out_stride_z, d->out_stride_c,
+                       d->face[0], d->face[1], d->face[2], d->flags, d->force[0], d->force[1], d->force[2], omega);
+  };
+  if (d->store_dtype == LBM355_F64) args(strain_rate_kernel<S, FAM, double>, (const double*)pdf);
+  else if (d->store_dtype == LBM355_F32) args(strain_rate_kernel<S, FAM, float>, (const float*)pdf);
+  else args(strain_rate_kernel<S, FAM, _Float16>, (const _Float16*)pdf);
+  return hipGetLastError() == hipSuccess ? 0 : fail2(LBM355_ERR_LAUNCH, "strain-rate launch failed");
+}
 }  // namespace
 
 extern "C" {
@@ -271,6 +356,36 @@ int lbm355_get_macroscopic(const lbm355_sweep_desc* d, const void* pdf, double* 
   if (d->stencil == LBM355_D3Q19) return get_dtype<D3Q19>(d, pdf, density, velocity, last_pattern, s);
   if (d->stencil == LBM355_D3Q27) return get_dtype<D3Q27>(d, pdf, density, velocity, last_pattern, s);
   return fail2(LBM355_ERR_ARG, "unknown stencil");
+}
+
+int lbm355_get_strain_rate(const lbm355_sweep_desc* d, const void* pdf, const double* table_f64, double* strain_rate,
+                           double omega_shear, int32_t last_pattern, void* stream) {
+  if (!d || !pdf || !strain_rate || !table_f64) return fail2(LBM355_ERR_ARG, "NULL argument");
+  if (d->store_dtype < 0 || d->store_dtype > 2) return fail2(LBM355_ERR_ARG, "bad dtype");
+  if (last_pattern < LBM355_PULL || last_pattern > LBM355_PUSH) return fail2(LBM355_ERR_ARG, "bad pattern");
+  for (int a = 0; a < (d->stencil == LBM355_D2Q9 ? 2 : 3); ++a)
+    if (d->face[a] < 0 || d->face[a] > 2 || (d->face[a] == LBM355_FACE_GHOST && d->ghost_layers < 1) ||
+        (table_pattern(last_pattern) && d->face[a] == LBM355_FACE_NOSLIP))
+      return fail2(LBM355_ERR_ARG, "bad face mode (Eso patterns: periodic or ghost faces)");
+  hipStream_t s = (hipStream_t)stream;
+  if (d->family == LBM355_FAMILY_MOMENT) {
+    if (d->stencil == LBM355_D2Q9)
+      return strain_launch<D2Q9, LBM355_FAMILY_MOMENT>(d, pdf, table_f64, strain_rate, omega_shear, last_pattern, s);
+    if (d->stencil == LBM355_D3Q19)
+      return strain_launch<D3Q19, LBM355_FAMILY_MOMENT>(d, pdf, table_f64, strain_rate, omega_shear, last_pattern, s);
+    if (d->stencil == LBM355_D3Q27)
+      return strain_launch<D3Q27, LBM355_FAMILY_MOMENT>(d, pdf, table_f64, strain_rate, omega_shear, last_pattern, s);
+  } else if (d->family == LBM355_FAMILY_CUMULANT && d->stencil == LBM355_D3Q27) {
+    return strain_launch<D3Q27, LBM355_FAMILY_CUMULANT>(d, pdf, table_f64, strain_rate, omega_shear, last_pattern, s);
+  } else if (d->family == LBM355_FAMILY_CENTRAL) {
+    if (d->stencil == LBM355_D2Q9)
+      return strain_launch<D2Q9, LBM355_FAMILY_CENTRAL>(d, pdf, table_f64, strain_rate, omega_shear, last_pattern, s);
+    if (d->stencil == LBM355_D3Q19)
+      return strain_launch<D3Q19, LBM355_FAMILY_CENTRAL>(d, pdf, table_f64, strain_rate, omega_shear, last_pattern, s);
+    if (d->stencil == LBM355_D3Q27)
+      return strain_launch<D3Q27, LBM355_FAMILY_CENTRAL>(d, pdf, table_f64, strain_rate, omega_shear, last_pattern, s);
+  }
+  return fail2(LBM355_ERR_UNSUPPORTED, "no strain-rate getter for this stencil/family");
 }
 
 int64_t lbm355_desc_size(void) { return (int64_t)sizeof(lbm355_sweep_desc); }

@@ -151,6 +151,15 @@ def lattice_weights(stencil):
     return [table[stencil.Q][sum(abs(c) for c in d)] for d in stencil]
 
 
+def get_weights(stencil, c_s_sq=Fraction(1, 3)):
+    """lbmpy's name for the lattice weights (maxwellian_equilibrium.py:24-32); exact fractions.  They are the
+    weights of the discrete equilibrium only for c_s^2 = 1/3, and a warning says so for any other value."""
+    if c_s_sq != Fraction(1, 3) and str(c_s_sq) not in ("c_s**2", "1/3"):
+        import warnings
+        warnings.warn("Weights of discrete equilibrium are only valid if c_s^2 = 1/3")
+    return lattice_weights(stencil)
+
+
 class DiscreteHydrodynamicMaxwellian:
     """Textbook polynomial equilibrium w_i rho (1 + c.u/cs2 + (c.u)^2/(2 cs2^2) - u.u/(2 cs2))."""
 

@@ -26,7 +26,8 @@ import torch
 from . import _native as nat
 from .creationfunctions import Timestep
 from .enums import Target
-from .kernel_tables import method_table, family_of, STENCIL_CODES
+from .kernel_tables import method_table, family_of, STENCIL_CODES, bind_rate
+from .relaxationrates import get_shear_relaxation_rate
 
 
 def _pattern_code(streaming_pattern, previous_timestep):
@@ -80,6 +81,16 @@ class _Context:
             u = torch.zeros((self.D,) + shape, dtype=torch.float64, device=pdfs.device)
             u = u.permute(*reversed(range(1, self.D + 1)), 0)
             f = self._staging[key] = (rho, u)
+        return f
+
+    def strain_field(self, pdfs):
+        """FP64 staging strain-rate tensor [x, y, (z), D*D], x contiguous, pdf-shaped."""
+        key = ("strain", tuple(pdfs.shape[:-1]), pdfs.device)
+        f = self._staging.get(key)
+        if f is None:
+            shape = tuple(reversed(pdfs.shape[:-1]))
+            f = torch.zeros((self.D * self.D,) + shape, dtype=torch.float64, device=pdfs.device)
+            f = self._staging[key] = f.permute(*reversed(range(1, self.D + 1)), 0)
         return f
 
     def descriptor(self, pdfs, rho, u):
@@ -169,6 +180,25 @@ def _get(ctx, pdfs, density, velocity, code):
         velocity[inner] = u[inner].to(velocity.dtype)
 
 
+def _strain(ctx, pdfs, out, code, bindings):
+    ctx.check_pdfs(pdfs)
+    if out is None:
+        raise ValueError("give the strain-rate output tensor")
+    rho, _ = ctx.fields(pdfs)
+    S = ctx.strain_field(pdfs)
+    d = ctx.descriptor(pdfs, rho, S)
+    rate = get_shear_relaxation_rate(ctx.method)
+    names = {str(k): float(v) for k, v in bindings.items()}
+    omega = float(bind_rate(rate, names))
+    lib = nat.load()
+    nat.check(lib.lbm355_get_strain_rate(ctypes.byref(d), ctypes.c_void_p(pdfs.data_ptr()),
+                                         ctypes.c_void_p(ctx.table(pdfs.device).data_ptr()),
+                                         ctypes.c_void_p(S.data_ptr()), omega, code, nat.current_stream_handle()))
+    g = ctx.g
+    inner = tuple(slice(g, n - g) for n in pdfs.shape[:-1])
+    out[inner] = S[inner].to(out.dtype)
+
+
 def compile_macroscopic_values_setter(lb_method, quantities_to_set, pdf_arr=None, ghost_layers=1,
                                       iteration_slice=None, field_layout='numpy', target=Target.GPU,
                                       streaming_pattern='pull', previous_timestep=Timestep.BOTH, periodicity=None):
@@ -223,20 +253,27 @@ class MacroscopicKernel:
     call runs it with the fields given at creation, each replaceable by keyword (``pdfs``, ``density``,
     ``velocity``)."""
 
-    def __init__(self, kind, ctx, code, pdfs, density, velocity):
+    def __init__(self, kind, ctx, code, pdfs, density, velocity, strain_rate_tensor=None):
         self.kind = kind
         self._ctx = ctx
         self._code = code
-        self._fields = {"pdfs": pdfs, "density": density, "velocity": velocity}
+        self._fields = {"pdfs": pdfs, "density": density, "velocity": velocity,
+                        "strain_rate_tensor": strain_rate_tensor}
 
     def compile(self):
         return self
 
     def __call__(self, **kwargs):
-        f = dict(self._fields, **kwargs)
+        fields = {k: v for k, v in kwargs.items() if k in self._fields}
+        bindings = {k: v for k, v in kwargs.items() if k not in self._fields}
+        f = dict(self._fields, **fields)
         if f["pdfs"] is None:
             raise ValueError("no PDF tensor given")
-        if self.kind == "setter":
+        if self.kind == "strain":
+            _strain(self._ctx, f["pdfs"], f["strain_rate_tensor"], self._code, bindings)
+        elif bindings:
+            raise TypeError(f"unexpected arguments {sorted(bindings)}")
+        elif self.kind == "setter":
             _set(self._ctx, f["pdfs"], f["density"], f["velocity"], self._code)
         else:
             _get(self._ctx, f["pdfs"], f["density"], f["velocity"], self._code)
@@ -265,3 +302,19 @@ def macroscopic_values_getter(lb_method, density, velocity, pdfs, streaming_patt
     ctx = _Context(lb_method, ghost_layers, periodicity, Target.GPU)
     return MacroscopicKernel("getter", ctx, _pattern_code(streaming_pattern, previous_timestep), pdfs, density,
                              velocity)
+
+
+def strain_rate_tensor_getter(lb_method, strain_rate_tensor, pdfs, streaming_pattern='pull',
+                              previous_timestep=Timestep.BOTH, use_pre_collision_pdfs=False, ghost_layers=1,
+                              periodicity=None):
+    """The strain-rate tensor S = -1.5 (omega_s / rho_ref) Pi_neq of the populations the kernel that ran at
+    ``previous_timestep`` wrote, into ``strain_rate_tensor`` [x, y, (z), D*D] (component a*D + b), over the
+    interior (macroscopic_value_kernels.py:67-85).  Pi_neq is the second moment of f - f_eq(rho, u) with rho
+    and the equilibrium velocity u computed from the same populations, as the collision computes them;
+    rho_ref is rho for compressible methods and 1 otherwise; omega_s is the method's shear rate, bound by
+    keyword at the call when it is a symbol.  Where lbmpy returns assignments, this returns a GPU kernel whose
+    ``compile()`` is itself."""
+    _check_args(None, use_pre_collision_pdfs)
+    ctx = _Context(lb_method, ghost_layers, periodicity, Target.GPU)
+    return MacroscopicKernel("strain", ctx, _pattern_code(streaming_pattern, previous_timestep), pdfs, None, None,
+                             strain_rate_tensor)

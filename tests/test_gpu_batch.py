@@ -1,6 +1,7 @@
 """Batched ensemble sweep: every member of a batch equals its single-domain run bit for bit, for direct
 launches, autotuned launch shapes and a replayed HIP graph; pull and AA; D3Q19 TRT channels (NoSlip y,
-periodic x/z) of uneven sizes with per-member force and relaxation rate."""
+periodic x/z) of uneven sizes with per-member force and relaxation rate; and every collision family and
+storage type (D3Q27 cumulant, D3Q19 central moments, FP64, FP16 storage)."""
 import numpy as np
 import pytest
 
@@ -117,4 +118,61 @@ def test_batch_graph_replay_and_autotune():
     torch.cuda.synchronize()
     assert b.steps_done == 30
     assert all(torch.equal(a, r) for a, r in zip(b.current_arrays(), ref30))
+    b.close()
+
+
+FAMILY_CASES = [
+    # (stencil, method, extra LBMConfig keywords, faces, storage dtype, pattern)
+    ("D3Q27", "CUMULANT", dict(compressible=True), ["periodic"] * 3, "float32", "aa"),
+    ("D3Q27", "CUMULANT", dict(compressible=True, force=(2e-6, 0.0, 0.0)), ["periodic", "noslip", "periodic"],
+     "float32", "pull"),
+    ("D3Q19", "CENTRAL_MOMENT", dict(compressible=True), ["periodic", "noslip", "periodic"], "float32", "pull"),
+    ("D3Q19", "TRT", dict(force=(1e-6, 0.0, 0.0)), ["periodic", "noslip", "periodic"], "float64", "pull"),
+    ("D3Q19", "TRT", dict(force=(1e-6, 0.0, 0.0)), ["periodic", "noslip", "periodic"], "float16", "aa"),
+]
+
+
+@pytest.mark.parametrize("case", FAMILY_CASES, ids=lambda c: f"{c[0]}-{c[1]}-{c[4]}-{c[5]}")
+def test_batch_families_and_storage_equal_single_runs(case):
+    """Every collision family and storage type through the batched sweep: each member (own size and rate)
+    equals its single-domain run bit for bit after 8 steps."""
+    import torch
+    from lbmpy_amd import LBMConfig, LBStencil, Stencil, Method, create_lb_function, CreateKernelConfig, Target
+    from lbmpy_amd import Timestep
+    from lbmpy_amd.batch import BatchedSweep
+    from lbmpy_amd.sweep import create_pdf_array
+    stencil, meth, extra, faces, dtype, pattern = case
+    print(f"batch family {case[:2]} {dtype} {pattern}", flush=True)
+    rng = np.random.default_rng(11)
+    st = LBStencil(Stencil[stencil])
+    cfg = CreateKernelConfig(target=Target.GPU, data_type=dtype, ghost_layers=0)
+    members = []
+    for _ in range(6):
+        shape = tuple(int(v) for v in rng.choice((8, 16, 24, 40), 3))
+        kw = dict(stencil=st, method=Method[meth], relaxation_rate=float(rng.uniform(1.5, 1.95)),
+                  streaming_pattern=pattern, **extra)
+        if pattern == "aa":
+            ks = [create_lb_function(lbm_config=LBMConfig(**kw, timestep=t), config=cfg)
+                  for t in (Timestep.ODD, Timestep.EVEN)]
+        else:
+            ks = [create_lb_function(lbm_config=LBMConfig(**kw), config=cfg)]
+        for k in ks:
+            k.set_faces(faces)
+        members.append((shape, ks))
+    g = torch.Generator(device="cuda").manual_seed(12)
+    init = []
+    for shape, _ in members:
+        a = create_pdf_array(shape, st.Q, ghost_layers=0, dtype=dtype)
+        a.copy_((1e-3 * torch.randn(a.shape, generator=g, device="cuda", dtype=torch.float32)).to(a.dtype))
+        init.append(a)
+    steps = 8
+    ref = _single(members, init, steps, pattern)
+    src = [a.clone() for a in init]
+    dst = [torch.empty_like(a) for a in init] if pattern == "pull" else None
+    b = BatchedSweep([ks[0] for _, ks in members], src, dst)
+    b.run(steps)
+    torch.cuda.synchronize()
+    for k, (got, want) in enumerate(zip(b.current_arrays(), ref)):
+        assert torch.equal(got, want), f"member {k} differs"
+    assert any(not torch.equal(r, i) for r, i in zip(ref, init))
     b.close()

@@ -1,0 +1,98 @@
+"""strain_rate_tensor_getter (macroscopic_value_kernels.py:67-85) on the GPU, lbm355_get_strain_rate:
+
+* S = -1.5 (omega_s / rho_ref) Pi(f - f_eq(rho, u)) equals the FP64 oracle's (its pinned equilibrium at the
+  density and equilibrium velocity of its collision) to 1e-12, for the moment, central-moment and cumulant
+  families, with and without force, zero-centred and absolute storage, through the pull, AA (both parities)
+  and EsoTwist accessors;
+* an equilibrium state has zero strain rate (closed form), for every family;
+* the output has D*D components, as lbmpy's test asserts (tests/test_macroscopic_value_kernels.py:91-104), and
+  is symmetric.
+"""
+import numpy as np
+import pytest
+import torch
+
+from lbmpy_amd import LBMConfig, LBStencil, Stencil, Method, ForceModel, create_lb_method, Timestep
+from lbmpy_amd.macroscopic_value_kernels import strain_rate_tensor_getter, macroscopic_values_setter
+from lbmpy_amd.relaxationrates import get_shear_relaxation_rate
+from lbmpy_amd.sweep import create_pdf_array
+from oracle.lbm_oracle import OracleCollision, equilibrium_pdfs
+from tests.lbm_helpers import seeded_state, storage_view
+from tests.test_gpu_streaming_patterns import _place
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    ("D2Q9", dict(method=Method.SRT, relaxation_rate=1.6, compressible=True, force=(2e-4, -1e-4)), (7, 6)),
+    ("D3Q19", dict(method=Method.TRT, relaxation_rates=[1.8, 1.3], compressible=False), (5, 4, 6)),
+    ("D3Q19", dict(method=Method.MRT, relaxation_rates=[1.7, 1.1, 1.2, 1.3], compressible=True,
+                   force_model=ForceModel.LUO, force=(1e-4, 0, 2e-5)), (4, 5, 3)),
+    ("D3Q19", dict(method=Method.CENTRAL_MOMENT, relaxation_rate=1.7, compressible=True, zero_centered=False),
+     (4, 3, 5)),
+    ("D3Q27", dict(method=Method.CUMULANT, relaxation_rate=1.9, compressible=True, force=(1e-4, 0, 0)), (4, 4, 3)),
+]
+KERNELS = [("pull", 0), ("aa", 0), ("aa", 1), ("esotwist", 1)]
+
+
+def _oracle_strain(m, f):
+    """S [D*D, cells] from natural populations f (Q, cells)."""
+    coll = OracleCollision(m, {s: 1.0 for s in m.relaxation_rate_symbols()})
+    rho, _, u_eq, _ = coll.macroscopic(f)
+    ref = rho if m.compressible else np.ones_like(rho)
+    u_in = u_eq + (coll.F[:, None] / (2.0 * ref[None, :]) if m.force_model is not None else 0.0)
+    feq = equilibrium_pdfs(m, rho, u_in)
+    c = np.array(m.stencil.stencil_entries, dtype=np.float64)
+    fneq = f - feq
+    D = m.dim
+    pi = np.stack([(c[:, a] * c[:, b]) @ fneq for a in range(D) for b in range(D)])
+    omega = float(get_shear_relaxation_rate(m))
+    return -1.5 * omega / ref * pi
+
+
+@pytest.mark.parametrize("kernel", KERNELS, ids=lambda k: f"{k[0]}-{k[1]}")
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c[0]}-{c[1]['method'].name}")
+def test_strain_rate_matches_oracle(case, kernel):
+    st, kw, shape = case
+    m = create_lb_method(LBMConfig(stencil=LBStencil(Stencil[st]), **kw))
+    D, Q = m.dim, m.stencil.Q
+    pattern, t = kernel
+    f0 = seeded_state(m, shape, seed=4, du=0.04, noise=2e-3)           # (Q, [z,] y, x)
+    pdfs = create_pdf_array(shape, Q, 1, "float64")
+    storage_view(pdfs, D)[...] = torch.from_numpy(_place(f0, m, shape, (pattern, t), 1, (True,) * D)).cuda()
+    out = torch.full(tuple(pdfs.shape[:-1]) + (D * D,), float("nan"), dtype=torch.float64, device="cuda")
+    ts = Timestep.BOTH if pattern == "pull" else (Timestep.EVEN if t == 0 else Timestep.ODD)
+    k = strain_rate_tensor_getter(m, out, pdfs, streaming_pattern=pattern, previous_timestep=ts,
+                                  periodicity=(True,) * D).compile()
+    k()
+    inner = tuple(slice(1, -1) for _ in range(D))
+    got = out[inner].cpu().numpy()                                     # [x, y, (z), D*D]
+    got = np.moveaxis(got, -1, 0).reshape(D * D, -1, order="F")        # cells in x-fastest order
+    want = _oracle_strain(m, f0.reshape(Q, -1))
+    assert np.abs(got - want).max() <= 1e-12
+    assert np.abs(want).max() > 1e-5
+    S = got.reshape(D, D, -1)
+    np.testing.assert_allclose(S, S.transpose(1, 0, 2), atol=1e-15)
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c[0]}-{c[1]['method'].name}")
+def test_equilibrium_has_no_strain(case):
+    """Without a force: with one, the collision's equilibrium velocity (Guo / central-moment models) is shifted
+    by F / 2 rho from the one the setter removes, so an initialised state carries an O(F) strain rate."""
+    st, kw, shape = case
+    kw = {k: v for k, v in kw.items() if k not in ("force", "force_model")}
+    m = create_lb_method(LBMConfig(stencil=LBStencil(Stencil[st]), **kw))
+    D, Q = m.dim, m.stencil.Q
+    pdfs = create_pdf_array(shape, Q, 1, "float64")
+    rng = np.random.default_rng(2)
+    rho = torch.from_numpy(1.0 + 0.02 * rng.standard_normal(shape)).cuda()
+    u = torch.from_numpy(0.03 * rng.standard_normal(shape + (D,))).cuda()
+    g_rho = torch.ones(tuple(pdfs.shape[:-1]), dtype=torch.float64, device="cuda")
+    g_u = torch.zeros(tuple(pdfs.shape[:-1]) + (D,), dtype=torch.float64, device="cuda")
+    inner = tuple(slice(1, -1) for _ in range(D))
+    g_rho[inner] = rho
+    g_u[inner] = u
+    macroscopic_values_setter(m, g_rho, g_u, pdfs, periodicity=(True,) * D).compile()()
+    out = torch.zeros(tuple(pdfs.shape[:-1]) + (D * D,), dtype=torch.float64, device="cuda")
+    strain_rate_tensor_getter(m, out, pdfs, periodicity=(True,) * D).compile()()
+    assert out.shape[-1] == D * D
+    assert float(out[inner].abs().max()) <= 1e-14
