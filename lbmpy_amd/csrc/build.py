@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 LIBDIR = os.path.join(PKG, "_lib")
 LIBNAME = "liblbm355.so"
-SOURCES = ["sweep.hip", "sweep_eso.hip", "sweep_h2.hip", "macroscopic.hip", "batch.hip", "closure.hip", "periodic.hip"]
+SOURCES = ["sweep.hip", "sweep_eso.hip", "macroscopic.hip", "batch.hip", "closure.hip", "periodic.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

@@ -539,10 +539,6 @@ int validate(const lbm355_sweep_desc* d) {
 template <class TC>
 int launch_sweep(const lbm355_sweep_desc* d, const void* src, void* dst, void* rho, void* u, hipStream_t s) {
   if (table_pattern(d->pattern)) return lbm355_launch_sweep_eso(d, src, dst, rho, u, s);
-  if (d->store_dtype == LBM355_F16) {
-    const int rc = lbm355_launch_sweep_h2(d, src, dst, rho, u, s);
-    if (rc != -1) return rc;
-  }
   bool walls = false;
   for (int a = 0; a < stencil_d(d->stencil); ++a) walls |= d->face[a] == LBM355_FACE_NOSLIP;
   size_t lds = 0;

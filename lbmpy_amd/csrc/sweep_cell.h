@@ -255,7 +255,3 @@ int lbm355_validate_desc(const lbm355_sweep_desc* d);
 // sweeps of the Eso patterns and push (sweep_eso.hip); the descriptor has been validated
 int lbm355_launch_sweep_eso(const lbm355_sweep_desc* d, const void* src, void* dst, void* rho, void* u,
                             hipStream_t s);
-
-// row-staged FP16 sweep (sweep_h2.hip): launches and returns a status when it applies, else returns -1
-int lbm355_launch_sweep_h2(const lbm355_sweep_desc* d, const void* src, void* dst, void* rho, void* u,
-                           hipStream_t s);

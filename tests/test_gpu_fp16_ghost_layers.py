@@ -1,7 +1,8 @@
-"""The row-staged FP16 sweep (sweep_h2.hip: half2 row transfers through LDS; used without ghost layers when
-x is periodic and nx % 128 == 0, moment family) gives bit-identical populations to the per-element FP16 sweep, which runs
-the same problem when the field has a ghost layer.  Both load the same FP16 values, run the same per-cell
-collision and round the same results to FP16, so the comparison is exact."""
+"""FP16 storage with FP32 arithmetic: the sweep without ghost layers (periodic wrap and fused walls in the index
+math) gives bit-identical populations to the sweep with a ghost layer on the same problem.  Both load the same
+FP16 values, run the same per-cell collision and round the same results to FP16, so the comparison is exact.
+(These cases once compared a row-staged half2 FP16 kernel with the per-element one; the per-element kernel
+without the LDS row shift proved faster and is the only FP16 sweep now.)"""
 import numpy as np
 import pytest
 
@@ -24,13 +25,13 @@ CASES = [
 
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c[0]}-{c[1].name}-{c[4]}")
-def test_row_staged_fp16_equals_per_element_fp16(case):
+def test_fp16_sweep_without_ghost_layers_equals_with(case):
     st, meth, extra, faces, pattern, shape = case
     kw = dict(stencil=LBStencil(Stencil[st]), method=meth, streaming_pattern=pattern, zero_centered=True, **extra)
     m = create_lb_method(LBMConfig(**kw))
     f0 = seeded_state(m, shape, seed=31, du=0.03, noise=1e-3)
     fdict = {a: faces[a] for a in range(m.dim)}
-    print(f"fp16 rows {st} {meth.name} {pattern} {shape}", flush=True)
+    print(f"fp16 ghost layers {st} {meth.name} {pattern} {shape}", flush=True)
     runs = []
     for g in (0, 1):
         ks = gpu_kernels(kw, data_type="float16", compute_type="float32", ghost_layers=g)
