@@ -96,3 +96,22 @@ def test_equilibrium_has_no_strain(case):
     strain_rate_tensor_getter(m, out, pdfs, periodicity=(True,) * D).compile()()
     assert out.shape[-1] == D * D
     assert float(out[inner].abs().max()) <= 1e-14
+
+
+def test_symbolic_shear_rate_is_bound_at_the_call():
+    import sympy as sp
+    shape = (6, 5, 4)
+    kw = dict(method=Method.TRT, compressible=True)
+    m_num = create_lb_method(LBMConfig(stencil=LBStencil(Stencil.D3Q19), relaxation_rate=1.7, **kw))
+    m_sym = create_lb_method(LBMConfig(stencil=LBStencil(Stencil.D3Q19), relaxation_rate=sp.Symbol("omega"), **kw))
+    f0 = seeded_state(m_num, shape, seed=9, du=0.04, noise=2e-3)
+    outs = []
+    for m, kwargs in ((m_num, {}), (m_sym, {"omega": 1.7})):
+        pdfs = create_pdf_array(shape, 19, 1, "float64")
+        storage_view(pdfs, 3)[...] = torch.from_numpy(_place(f0, m, shape, ("pull", 0), 1, (True,) * 3)).cuda()
+        out = torch.zeros(tuple(pdfs.shape[:-1]) + (9,), dtype=torch.float64, device="cuda")
+        strain_rate_tensor_getter(m, out, pdfs, periodicity=(True,) * 3).compile()(**kwargs)
+        outs.append(out)
+    assert torch.equal(outs[0], outs[1]) and float(outs[0].abs().max()) > 1e-5
+    with pytest.raises(ValueError):
+        strain_rate_tensor_getter(m_sym, outs[1], pdfs, periodicity=(True,) * 3).compile()()
