@@ -73,21 +73,56 @@ class UBB(LbBoundary):
 
 
 class FreeSlip(LbBoundary):
-    """Specular reflection at a straight wall with the given normal (pointing from the wall into the fluid):
-    f_inv(d)(x_b) = f*_{inv(mirror(d))}(x + c_d + n).  Per-cell normals (normal_direction=None) are not
-    available on the GPU path."""
-    kind = nat.BC_FREESLIP
+    """Specular reflection (boundaryconditions.py:360-487).  With ``normal_direction`` (a straight wall; the
+    normal points from the wall into the fluid): f_inv(d)(x_b) = f*_{inv(mirror(d))}(x + c_d + n).  Without it,
+    every link gets its own normal and reference direction from the boundary cells around it, as lbmpy's
+    init_callback computes them (``freeslip_link_data``), and f_inv(d)(x_b) = f*_r(x + c_d + n)."""
 
     def __init__(self, stencil, normal_direction=None, name=None):
-        if normal_direction is None:
-            raise NotImplementedError("FreeSlip needs normal_direction on the GPU path (straight walls)")
-        normal_direction = tuple(int(n) for n in normal_direction)
-        if len(normal_direction) - normal_direction.count(0) != 1 or any(n not in (-1, 0, 1) for n in normal_direction):
-            raise ValueError("the normal direction must have exactly one entry -1 or 1, e.g. (0, 1, 0)")
         self.stencil = stencil
+        if normal_direction is not None:
+            normal_direction = tuple(int(n) for n in normal_direction)
+            if len(normal_direction) - normal_direction.count(0) != 1 or \
+                    any(n not in (-1, 0, 1) for n in normal_direction):
+                raise ValueError("the normal direction must have exactly one entry -1 or 1, e.g. (0, 1, 0)")
         self.normal_direction = normal_direction
-        self.dim = len(normal_direction)
+        self.dim = len(stencil[0])
         super().__init__(name)
+
+    @property
+    def kind(self):
+        return nat.BC_FREESLIP if self.normal_direction is not None else nat.BC_FREESLIP_LINKS
+
+
+def freeslip_link_data(links, stencil, dim):
+    """Per link (x, y, (z,) d) of one FreeSlip object without a given normal: (n_x, n_y, n_z, r) with n the normal
+    from the wall to the fluid and r the index of the reference direction (boundaryconditions.py:402-440).  A link's
+    wall axes are those along which the fluid cell's neighbour, offset by that component of c_d alone, is one of
+    the object's boundary cells; n is minus c_d on those axes, and r is c_d mirrored on them and inverted.  When no
+    axis qualifies (a convex corner), n = -c_d and r = d, plain bounce-back."""
+    c = np.array(stencil.stencil_entries, dtype=int)
+    index = {tuple(v): i for i, v in enumerate(c.tolist())}
+    cells = links[:, :dim].astype(int)
+    dirs = links[:, 3].astype(int)
+    boundary = set(map(tuple, (cells + c[dirs, :dim]).tolist()))
+    out = np.zeros((len(links), 4))
+    for k in range(len(links)):
+        cd = c[dirs[k], :dim]
+        normal = np.zeros(dim, dtype=int)
+        ref = cd.copy()
+        for i in range(dim):
+            sub = np.zeros(dim, dtype=int)
+            sub[i] = cd[i]
+            if tuple((cells[k] + sub).tolist()) in boundary:
+                normal[i] = cd[i]
+                ref[i] = -ref[i]
+        if not normal.any():
+            normal = cd.copy()
+        else:
+            ref = -ref
+        out[k, :dim] = -normal
+        out[k, 3] = index[tuple(ref.tolist())]
+    return out
 
 
 class FixedDensity(LbBoundary):
@@ -400,6 +435,8 @@ class LatticeBoltzmannBoundaryHandling:
             obj = self._objects[bid - 1]
             links = np.concatenate(blocks)
             vel = None
+            if isinstance(obj, FreeSlip) and obj.normal_direction is None:
+                vel = freeslip_link_data(links, st, D)
             if isinstance(obj, UBB) and obj.velocity_is_callable:
                 if _takes_boundary_data(obj._velocity):
                     data = BoundaryData(links, st, g, D)
@@ -489,7 +526,7 @@ class LatticeBoltzmannBoundaryHandling:
                     for ax, v in enumerate(obj._velocity):
                         d.velocity[ax] = float(v)
                 d.density = float(obj._density) if obj._density is not None else 0.0
-            elif isinstance(obj, FreeSlip):
+            elif isinstance(obj, FreeSlip) and obj.normal_direction is not None:
                 for ax, v in enumerate(obj.normal_direction):
                     d.normal[ax] = int(v)
             elif isinstance(obj, ExtrapolationOutflow):

@@ -277,6 +277,15 @@ __global__ void boundary_links_kernel(T* pdf, const int32_t* links, int64_t n, d
     return;
   }
   const int64_t target = bc_in<S, PAT>(b, id, x, y, z);
+  if (KIND == LBM355_BC_FREESLIP_LINKS) {
+    // per-link normal n (wall to fluid, entries -1/0/1) and reference direction r; x + c_d + n and the accessor
+    // offset of r stay within one cell of x (n cancels c_d on wall axes, r on the others)
+    const double* q = link_vel + 4 * t;
+    const int r = (int)q[3];
+    if (r < 0 || r >= S::Q) return;
+    pdf[target] = pdf[bc_out<S, PAT>(b, r, x + cdx + (int)q[0], y + cdy + (int)q[1], z + cdz + (int)q[2])];
+    return;
+  }
   if (KIND == LBM355_BC_NOSLIP) {
     pdf[target] = pdf[bc_out<S, PAT>(b, d, x, y, z)];
   } else if (KIND == LBM355_BC_UBB) {
@@ -599,6 +608,9 @@ int launch_bc_kind(const lbm355_boundary_desc* d, T* pdf, const int32_t* links, 
   else if (d->kind == LBM355_BC_OUTFLOW_INIT)
     hipLaunchKernelGGL((boundary_links_kernel<S, PAT, T, LBM355_BC_OUTFLOW_INIT>), dim3(grid), dim3(256), 0, s, pdf,
                        links, n, vel, b);
+  else if (d->kind == LBM355_BC_FREESLIP_LINKS)
+    hipLaunchKernelGGL((boundary_links_kernel<S, PAT, T, LBM355_BC_FREESLIP_LINKS>), dim3(grid), dim3(256), 0, s, pdf,
+                       links, n, vel, b);
   else
     hipLaunchKernelGGL((boundary_links_kernel<S, PAT, T, LBM355_BC_FREESLIP>), dim3(grid), dim3(256), 0, s, pdf, links,
                        n, vel, b);
@@ -729,7 +741,9 @@ int lbm355_boundary_links(const lbm355_boundary_desc* d, void* pdf, const int32_
   if (table_pattern(d->pattern) && (d->fused_noslip[0] || d->fused_noslip[1] || d->fused_noslip[2]))
     return fail(LBM355_ERR_ARG, "fused NoSlip faces exist with pull and AA streaming only");
   if (d->store_dtype < 0 || d->store_dtype > 2) return fail(LBM355_ERR_ARG, "bad dtype");
-  if (d->kind < LBM355_BC_NOSLIP || d->kind > LBM355_BC_OUTFLOW_INIT) return fail(LBM355_ERR_ARG, "bad boundary kind");
+  if (d->kind < LBM355_BC_NOSLIP || d->kind > LBM355_BC_FREESLIP_LINKS) return fail(LBM355_ERR_ARG, "bad boundary kind");
+  if (d->kind == LBM355_BC_FREESLIP_LINKS && !link_velocity)
+    return fail(LBM355_ERR_ARG, "FreeSlip with per-link normals needs its link data (normal, reference direction)");
   if ((d->kind == LBM355_BC_OUTFLOW || d->kind == LBM355_BC_OUTFLOW_INIT) && !link_velocity)
     return fail(LBM355_ERR_ARG, "the outflow boundary needs its per-link state array");
   if (d->kind == LBM355_BC_FIXED_DENSITY && (!d->equilibrium || !(d->density > 0)))

@@ -580,6 +580,32 @@ def streaming_accessor(c, inv, pattern, timestep):
     raise ValueError(f"unknown streaming pattern {pattern!r}")
 
 
+def freeslip_normals(c, cells, dirs):
+    """FreeSlip without a given normal (boundaryconditions.py:402-440): for the links (cells[k], dirs[k]) of one
+    boundary object, the normal from the wall into the fluid and the reference direction index.  An axis a is a
+    wall axis of a link when the fluid cell shifted by c_d's a-component alone is one of the object's boundary
+    cells (cells + c_d); the normal is -c_d on wall axes and 0 elsewhere, the reference direction keeps c_d on
+    wall axes and reverses it elsewhere.  With no wall axis (a convex corner) the normal is -c_d and the reference
+    direction d itself."""
+    c = np.asarray(c, dtype=int)
+    D = c.shape[1]
+    lookup = {tuple(v): i for i, v in enumerate(c.tolist())}
+    boundary = {tuple(v) for v in (cells + c[dirs]).tolist()}
+    normals = np.zeros((len(dirs), D), dtype=int)
+    refs = np.zeros(len(dirs), dtype=int)
+    for k in range(len(dirs)):
+        cd = c[dirs[k]]
+        wall = np.array([tuple((cells[k] + np.where(np.arange(D) == a, cd, 0)).tolist()) in boundary
+                         for a in range(D)])
+        if wall.any():
+            normals[k] = np.where(wall, -cd, 0)
+            refs[k] = lookup[tuple(np.where(wall, cd, -cd).tolist())]
+        else:
+            normals[k] = -cd
+            refs[k] = dirs[k]
+    return normals, refs
+
+
 class OracleDomain:
     """A domain with one ghost layer, stepping the way ``LatticeBoltzmannStep`` does.
 
@@ -746,6 +772,12 @@ class OracleDomain:
                 else:
                     rho = 1.0
                 arr[target] = arr[self._out_addr(prev, d, cell)] - 6.0 * w * cu * rho
+            elif spec["kind"] == "freeslip" and spec.get("normal") is None:
+                # a normal and a reference direction per link (boundaryconditions.py:402-440)
+                cells = np.stack(cell, axis=1)
+                nrm, ref = freeslip_normals(self.c[:, :D], cells, d)
+                src_cell = [cell[a] + self.c[d, a] + nrm[:, a] for a in range(D)]
+                arr[target] = arr[self._out_addr(prev, ref, src_cell)]
             elif spec["kind"] == "freeslip":
                 nrm = np.asarray(spec["normal"], dtype=int)
                 ax = int(np.nonzero(nrm)[0][0])

@@ -98,3 +98,34 @@ def test_links_match_lbmpy_index_list_convex_corner():
     bh.set_boundary(FreeSlip(LBStencil(Stencil.D2Q9), normal_direction=(0, 1)), mask_callback=callback)
     got = sorted(tuple(int(v) for v in (l[0], l[1], l[3])) for l in bh.links)
     assert got == sorted(_CONVEX_CORNER_LINKS)
+
+
+# lbmpy's full expected index list for the convex-corner case (tests/test_boundary_handling.py:326-341): x, y,
+# dir, the wall-to-fluid normal (x, y) and the reference direction of FreeSlip without a given normal
+_CONVEX_CORNER_FREESLIP = [
+    (2, 1, 2, 0, 1, 2), (2, 1, 3, 1, 0, 3), (2, 1, 7, 1, 1, 7), (2, 1, 8, 0, 1, 7), (3, 1, 2, 0, 1, 2),
+    (3, 1, 4, -1, 0, 4), (3, 1, 7, 0, 1, 8), (3, 1, 8, -1, 1, 8), (1, 2, 2, 0, 1, 2), (1, 2, 3, 1, 0, 3),
+    (1, 2, 5, 1, 0, 7), (1, 2, 7, 1, 1, 7), (2, 2, 7, 1, 1, 7), (3, 2, 8, -1, 1, 8), (4, 2, 2, 0, 1, 2),
+    (4, 2, 4, -1, 0, 4), (4, 2, 6, -1, 0, 8), (4, 2, 8, -1, 1, 8), (1, 3, 1, 0, -1, 1), (1, 3, 3, 1, 0, 3),
+    (1, 3, 5, 1, -1, 5), (1, 3, 7, 1, 0, 5), (2, 3, 5, 1, -1, 5), (3, 3, 6, -1, -1, 6), (4, 3, 1, 0, -1, 1),
+    (4, 3, 4, -1, 0, 4), (4, 3, 6, -1, -1, 6), (4, 3, 8, -1, 0, 6), (2, 4, 1, 0, -1, 1), (2, 4, 3, 1, 0, 3),
+    (2, 4, 5, 1, -1, 5), (2, 4, 6, 0, -1, 5), (3, 4, 1, 0, -1, 1), (3, 4, 4, -1, 0, 4), (3, 4, 5, 0, -1, 6),
+    (3, 4, 6, -1, -1, 6)
+]
+
+
+def test_freeslip_link_normals_match_lbmpy_index_list():
+    from lbmpy_amd.boundaries import freeslip_link_data
+    from oracle.lbm_oracle import freeslip_normals
+    m = _method()
+    st = LBStencil(Stencil.D2Q9)
+    bh = LatticeBoltzmannBoundaryHandling(m, GpuArrays((False, False), src=_cpu_pdf((6, 6), 9)), "src")
+    bh.set_boundary(FreeSlip(st), mask_callback=lambda x, y: (x - 2) ** 2 + (y - 2) ** 2 > 2 ** 2)
+    (obj, links, data), = bh.link_groups
+    got = sorted((int(l[0]), int(l[1]), int(l[3]), int(v[0]), int(v[1]), int(v[3])) for l, v in zip(links, data))
+    assert got == sorted(_CONVEX_CORNER_FREESLIP)
+    nrm, ref = freeslip_normals(np.array(st.stencil_entries), links[:, :2].astype(int), links[:, 3].astype(int))
+    got_oracle = sorted((int(l[0]), int(l[1]), int(l[3]), int(n[0]), int(n[1]), int(r))
+                        for l, n, r in zip(links, nrm, ref))
+    assert got_oracle == sorted(_CONVEX_CORNER_FREESLIP)
+    assert obj.kind == 6 and np.array_equal(freeslip_link_data(links, st, 2), data)
