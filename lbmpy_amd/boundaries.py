@@ -22,12 +22,28 @@ from . import _native as nat
 
 
 class LbBoundary:
+    """Base of the boundary objects (boundaryconditions.py:24-99): a name (the class name unless given),
+    hashed by name, equal to another object of the same type with the same attributes."""
     inner_or_boundary = True
     kind = None
 
     def __init__(self, name=None, calculate_force_on_boundary=False):
-        self.name = name or type(self).__name__
+        self._name = name
         self.calculate_force_on_boundary = calculate_force_on_boundary
+
+    @property
+    def name(self):
+        return self._name if self._name else type(self).__name__
+
+    @name.setter
+    def name(self, value):
+        self._name = value
+
+    def __hash__(self):
+        return hash(self.name)
+
+    def __eq__(self, other):
+        return isinstance(other, type(self)) and self.__dict__ == other.__dict__
 
     def __repr__(self):
         return self.name
@@ -36,12 +52,6 @@ class LbBoundary:
 class NoSlip(LbBoundary):
     """Half-way bounce-back: f_inv(i)(x_b, t + 1) = f*_i(x_b, t)."""
     kind = nat.BC_NOSLIP
-
-    def __hash__(self):
-        return hash(type(self))
-
-    def __eq__(self, other):
-        return type(other) == type(self)
 
 
 class UBB(LbBoundary):

@@ -129,3 +129,37 @@ def test_freeslip_link_normals_match_lbmpy_index_list():
                         for l, n, r in zip(links, nrm, ref))
     assert got_oracle == sorted(_CONVEX_CORNER_FREESLIP)
     assert obj.kind == 6 and np.array_equal(freeslip_link_data(links, st, 2), data)
+
+
+def test_boundary_object_names_and_equality():
+    """lbmpy's test_boundary_utility_functions (tests/test_boundary_handling.py:380-420) for the boundaries of
+    this package."""
+    from lbmpy_amd.boundaries import ExtrapolationOutflow, FixedDensity
+    st = LBStencil(Stencil.D2Q9)
+    method = create_lb_method(LBMConfig(stencil=st))
+    noslip = NoSlip("noslip")
+    assert noslip == NoSlip("noslip")
+    assert not noslip == NoSlip("test")
+    assert not noslip == UBB((0, 0), name="ubb")
+    assert noslip.name == "noslip"
+    noslip.name = "test name setter"
+    assert noslip.name == "test name setter"
+    ubb = UBB((0, 0), name="ubb")
+    assert ubb == UBB((0, 0), name="ubb")
+    assert not noslip == UBB((0, 0), name="test")
+    assert not ubb == NoSlip("noslip")
+    outflow = ExtrapolationOutflow(normal_direction=st[4], lb_method=method, name="outflow")
+    assert outflow == ExtrapolationOutflow(normal_direction=st[4], lb_method=method, name="outflow")
+    assert not outflow == ExtrapolationOutflow(normal_direction=st[4], lb_method=method, name="test")
+    density = FixedDensity(density=1.0, name="fixedDensity")
+    assert density == FixedDensity(density=1.0, name="fixedDensity")
+    assert not density == FixedDensity(density=1.0, name="test")
+    assert not density == UBB((0, 0), name="ubb")
+    assert NoSlip().name == "NoSlip" and hash(NoSlip()) == hash("NoSlip")
+    # one NoSlip object, however many times it is created, makes one link group
+    m = _method()
+    bh = LatticeBoltzmannBoundaryHandling.for_domain(m, (6, 5), (True, False))
+    bh.fuse_noslip_faces = False
+    bh.set_boundary(NoSlip(), slice_from_direction("N", 2))
+    bh.set_boundary(NoSlip(), slice_from_direction("S", 2))
+    assert len(bh.link_groups) == 1
