@@ -139,10 +139,10 @@ int lbm355_reduce_conserved(const lbm355_sweep_desc* desc, const void* pdf, doub
  * pdf_initialization_assignments; run by LatticeBoltzmannStep.set_pdf_fields_from_macroscopic_values,
  * lbstep.py:268-269).  density / velocity: FP64 device arrays laid out with desc->out_stride_*
  * (velocity component stride out_stride_c) and the same ghost-layer offset as the PDFs.  The
- * equilibrium velocity is u - F/(2 rho_ref) when a force model is present.  prev_pattern:
- * LBM355_PULL writes slot i; the EVEN kernel of an in-place pattern writes through that kernel's
- * write accessor (state "after an EVEN step", so the next kernel is ODD; neighbour writes wrap on
- * periodic axes and land in the ghost layer otherwise).  table_f64: FP64 method table (moment and
+ * equilibrium velocity is u - F/(2 rho_ref) when a force model is present.  prev_pattern: the kernel
+ * the state is to look as if it ran last (any code; LBM355_PULL writes slot i, the others through their
+ * write accessors; neighbour writes wrap on periodic axes and land in the ghost layer otherwise).  After
+ * an AA odd or an Eso/push kernel no face may be a fused NoSlip face.  table_f64: FP64 method table (moment and
  * central-moment families; ignored for cumulants); a central-moment equilibrium is the method's
  * collision with every rate 1.
  * Computes in FP64 and rounds once to the storage type. */

@@ -239,12 +239,19 @@ int set_eq_launch(const lbm355_sweep_desc* d, void* pdf, const double* rho, cons
                   int prev_pattern, hipStream_t s) {
   const int bx = d->nx >= 256 ? 256 : ((d->nx + 63) / 64) * 64;
   dim3 block(bx), grid((d->nx + bx - 1) / bx, d->ny, d->nz);
-  auto k = prev_pattern == LBM355_PULL            ? set_equilibrium_kernel<S, FAM, LBM355_PULL, TS>
-           : prev_pattern == LBM355_PUSH          ? set_equilibrium_kernel<S, FAM, LBM355_PUSH, TS>
-           : prev_pattern == LBM355_AA_EVEN       ? set_equilibrium_kernel<S, FAM, LBM355_AA_EVEN, TS>
-           : prev_pattern == LBM355_ESOTWIST_EVEN ? set_equilibrium_kernel<S, FAM, LBM355_ESOTWIST_EVEN, TS>
-           : prev_pattern == LBM355_ESOPULL_EVEN  ? set_equilibrium_kernel<S, FAM, LBM355_ESOPULL_EVEN, TS>
-                                                  : set_equilibrium_kernel<S, FAM, LBM355_ESOPUSH_EVEN, TS>;
+  decltype(&set_equilibrium_kernel<S, FAM, LBM355_PULL, TS>) k = nullptr;
+  switch (prev_pattern) {
+    case LBM355_PULL: k = set_equilibrium_kernel<S, FAM, LBM355_PULL, TS>; break;
+    case LBM355_AA_EVEN: k = set_equilibrium_kernel<S, FAM, LBM355_AA_EVEN, TS>; break;
+    case LBM355_AA_ODD: k = set_equilibrium_kernel<S, FAM, LBM355_AA_ODD, TS>; break;
+    case LBM355_ESOTWIST_EVEN: k = set_equilibrium_kernel<S, FAM, LBM355_ESOTWIST_EVEN, TS>; break;
+    case LBM355_ESOTWIST_ODD: k = set_equilibrium_kernel<S, FAM, LBM355_ESOTWIST_ODD, TS>; break;
+    case LBM355_ESOPULL_EVEN: k = set_equilibrium_kernel<S, FAM, LBM355_ESOPULL_EVEN, TS>; break;
+    case LBM355_ESOPULL_ODD: k = set_equilibrium_kernel<S, FAM, LBM355_ESOPULL_ODD, TS>; break;
+    case LBM355_ESOPUSH_EVEN: k = set_equilibrium_kernel<S, FAM, LBM355_ESOPUSH_EVEN, TS>; break;
+    case LBM355_ESOPUSH_ODD: k = set_equilibrium_kernel<S, FAM, LBM355_ESOPUSH_ODD, TS>; break;
+    default: k = set_equilibrium_kernel<S, FAM, LBM355_PUSH, TS>; break;
+  }
   hipLaunchKernelGGL(k, grid, block, 0, s, (TS*)pdf, rho, u, tab, d->nx, d->ny, d->nz, d->ghost_layers, d->stride_y,
                      d->stride_z, d->stride_q, d->out_stride_y, d->out_stride_z, d->out_stride_c, d->flags,
                      d->force[0], d->force[1], d->force[2], d->face[0], d->face[1], d->face[2]);
@@ -313,13 +320,12 @@ int lbm355_set_equilibrium(const lbm355_sweep_desc* d, void* pdf, const double* 
                            const double* table_f64, int32_t prev_pattern, void* stream) {
   if (!d || !pdf || !density || !velocity) return fail2(LBM355_ERR_ARG, "NULL argument");
   if (d->store_dtype < 0 || d->store_dtype > 2) return fail2(LBM355_ERR_ARG, "bad dtype");
-  if (prev_pattern != LBM355_PULL && prev_pattern != LBM355_PUSH && prev_pattern != LBM355_AA_EVEN &&
-      prev_pattern != LBM355_ESOTWIST_EVEN && prev_pattern != LBM355_ESOPULL_EVEN && prev_pattern != LBM355_ESOPUSH_EVEN)
-    return fail2(LBM355_ERR_ARG, "setter writes through the pull / push accessor or an EVEN kernel's accessor");
+  if (prev_pattern < LBM355_PULL || prev_pattern > LBM355_PUSH) return fail2(LBM355_ERR_ARG, "bad pattern");
   for (int a = 0; a < (d->stencil == LBM355_D2Q9 ? 2 : 3); ++a)
     if (d->face[a] < 0 || d->face[a] > 2 || (d->face[a] == LBM355_FACE_GHOST && d->ghost_layers < 1) ||
-        (table_pattern(prev_pattern) && d->face[a] == LBM355_FACE_NOSLIP))
-      return fail2(LBM355_ERR_ARG, "bad face mode (Eso patterns: periodic or ghost faces)");
+        ((table_pattern(prev_pattern) || prev_pattern == LBM355_AA_ODD) && d->face[a] == LBM355_FACE_NOSLIP))
+      return fail2(LBM355_ERR_ARG, "bad face mode (fused NoSlip faces: the setter writes as after pull, push or an "
+                                   "AA even kernel)");
   hipStream_t s = (hipStream_t)stream;
   if (d->family == LBM355_FAMILY_MOMENT) {
     if (!table_f64) return fail2(LBM355_ERR_ARG, "moment family needs the FP64 method table");

@@ -164,7 +164,7 @@ def _set(ctx, pdfs, density, velocity, code):
                                          nat.current_stream_handle()))
 
 
-def _get(ctx, pdfs, density, velocity, code):
+def _get(ctx, pdfs, density, velocity, code, density_deviation=None):
     ctx.check_pdfs(pdfs)
     rho, u = ctx.fields(pdfs)
     d = ctx.descriptor(pdfs, rho, u)
@@ -176,6 +176,8 @@ def _get(ctx, pdfs, density, velocity, code):
     inner = tuple(slice(g, s - g) for s in pdfs.shape[:-1])
     if density is not None:
         density[inner] = rho[inner].to(density.dtype)
+    if density_deviation is not None:
+        density_deviation[inner] = (rho[inner] - 1.0).to(density_deviation.dtype)
     if velocity is not None:
         velocity[inner] = u[inner].to(velocity.dtype)
 
@@ -225,14 +227,15 @@ def compile_macroscopic_values_setter(lb_method, quantities_to_set, pdf_arr=None
 def compile_macroscopic_values_getter(lb_method, output_quantities, pdf_arr=None, ghost_layers=1,
                                       iteration_slice=None, field_layout='numpy', target=Target.GPU,
                                       streaming_pattern='pull', previous_timestep=Timestep.BOTH, periodicity=None):
-    """A function ``getter(pdfs, density=..., velocity=...)`` that fills the requested output tensors over the
-    interior (lbmpy macroscopic_value_kernels.py:88-155)."""
+    """A function ``getter(pdfs, density=..., density_deviation=..., velocity=...)`` that fills the requested
+    output tensors over the interior (lbmpy macroscopic_value_kernels.py:88-155)."""
     _check_args(iteration_slice, False)
     if not isinstance(output_quantities, (list, tuple)):
         output_quantities = [output_quantities]
-    unknown = [q for q in output_quantities if q not in ("density", "velocity")]
+    unknown = [q for q in output_quantities if q not in ("density", "density_deviation", "velocity")]
     if unknown:
-        raise ValueError(f"No such conserved quantity: {unknown}, conserved quantities are ['density', 'velocity']")
+        raise ValueError(f"No such conserved quantity: {unknown}, conserved quantities are "
+                         "['density', 'density_deviation', 'velocity']")
     ctx = _Context(lb_method, ghost_layers, periodicity, target)
     code = _pattern_code(streaming_pattern, previous_timestep)
 
@@ -242,7 +245,7 @@ def compile_macroscopic_values_getter(lb_method, output_quantities, pdf_arr=None
         if not set(output_quantities).issubset(kwargs.keys()):
             raise ValueError(f"You have to specify the output field for each of the following quantities: "
                              f"{output_quantities}")
-        _get(ctx, pdfs, kwargs.get("density"), kwargs.get("velocity"), code)
+        _get(ctx, pdfs, kwargs.get("density"), kwargs.get("velocity"), code, kwargs.get("density_deviation"))
 
     return getter
 

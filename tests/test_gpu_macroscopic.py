@@ -147,3 +147,58 @@ def test_lbstep_central_moment_channel():
         dom.step()
     assert np.abs(got - dom.pdfs_in_natural_layout()).max() <= 1e-12
     assert np.abs(step.velocity[:, :, :]).max() > 1e-6
+
+
+_SETGET_METHODS = {}
+
+
+def _setget_method(stencil, force_model, compressible, zero_centered):
+    from lbmpy_amd import ForceModel  # noqa: F401
+    key = (stencil, force_model, compressible, zero_centered)
+    if key not in _SETGET_METHODS:
+        st = LBStencil(stencil)
+        _SETGET_METHODS[key] = create_lb_method(LBMConfig(
+            stencil=st, force_model=force_model, method=Method.TRT, compressible=compressible,
+            zero_centered=zero_centered, force=(0.1, 0.12, -0.17)[:st.D]))
+    return _SETGET_METHODS[key]
+
+
+@pytest.mark.parametrize("prev_timestep", ["EVEN", "ODD"])
+@pytest.mark.parametrize("streaming_pattern", ["push", "pull", "aa", "esotwist", "esopull", "esopush"])
+@pytest.mark.parametrize("zero_centered", [True, False])
+@pytest.mark.parametrize("compressible", [True, False])
+@pytest.mark.parametrize("force_model", ["GUO", "LUO", None])
+@pytest.mark.parametrize("stencil", ["D2Q9", "D3Q19"])
+def test_set_get_density_velocity_with_fields(stencil, force_model, compressible, zero_centered, streaming_pattern,
+                                              prev_timestep):
+    """lbmpy's tests/test_macroscopic_value_kernels.py:13-57: random density and velocity fields set through the
+    accessor of the kernel that ran last (either parity) come back from the getter, with density_deviation."""
+    from lbmpy_amd import ForceModel, Timestep
+    from lbmpy_amd.macroscopic_value_kernels import compile_macroscopic_values_getter, \
+        compile_macroscopic_values_setter
+    m = _setget_method(Stencil[stencil], None if force_model is None else ForceModel[force_model], compressible,
+                       zero_centered)
+    D = m.dim
+    g = 1
+    inner_size = (3, 7, 4)[:D]
+    total = tuple(s + 2 * g for s in inner_size)
+    rng = np.random.default_rng(7)
+    inner = (slice(g, -g),) * D
+    rho_in = torch.zeros(total, dtype=torch.float64, device="cuda")
+    u_in = torch.zeros(total + (D,), dtype=torch.float64, device="cuda")
+    rho_in[inner] = torch.from_numpy(1 + 0.2 * (rng.random(inner_size) - 0.5)).cuda()
+    u_in[inner] = torch.from_numpy(0.1 * (rng.random(inner_size + (D,)) - 0.5)).cuda()
+    pdf_arr = create_pdf_array(inner_size, m.stencil.Q, g, "float64")
+    ts = Timestep.EVEN if prev_timestep == "EVEN" else Timestep.ODD
+    setter = compile_macroscopic_values_setter(m, {"density": rho_in, "velocity": u_in}, pdf_arr=pdf_arr,
+                                               ghost_layers=g, streaming_pattern=streaming_pattern,
+                                               previous_timestep=ts)
+    setter(pdf_arr)
+    getter = compile_macroscopic_values_getter(m, ["density", "density_deviation", "velocity"], pdf_arr=pdf_arr,
+                                               ghost_layers=g, streaming_pattern=streaming_pattern,
+                                               previous_timestep=ts)
+    rho_out, dev_out, u_out = torch.zeros_like(rho_in), torch.zeros_like(rho_in), torch.zeros_like(u_in)
+    getter(pdfs=pdf_arr, density=rho_out, density_deviation=dev_out, velocity=u_out)
+    np.testing.assert_allclose(rho_out.cpu().numpy(), rho_in.cpu().numpy(), rtol=0, atol=1e-12)
+    np.testing.assert_allclose(dev_out[inner].cpu().numpy(), rho_in[inner].cpu().numpy() - 1.0, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(u_out.cpu().numpy(), u_in.cpu().numpy(), rtol=0, atol=1e-12)
