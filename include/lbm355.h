@@ -187,6 +187,8 @@ int lbm355_stream_copy(const void* src, void* dst, int64_t n_floats, int32_t non
                                 f_in(inv d) = c s0 + (1 - c) s1, c = 1/sqrt(3); then s0 = f_out(inv d) at
                                 x + c_d - n and s1 = the value just written */
 #define LBM355_BC_OUTFLOW_INIT 5 /* sets s0 = s1 = f_out(inv d) at x + c_d - n (before the first step) */
+#define LBM355_BC_SIMPLE_OUTFLOW 7 /* f_in(inv d) = f_out(inv d) at x + c_d - n (boundaryconditions.py:844-891,
+                                      Geier et al. 2015 eq. F.1), n = desc->normal pointing out of the fluid */
 #define LBM355_BC_FREESLIP_LINKS 6 /* FreeSlip with a normal per link (lbmpy's FreeSlip without normal_direction,
                                       boundaryconditions.py:402-440): link_data n x 4 doubles (wall-to-fluid normal
                                       x, y, z, reference direction r); f_in(inv d) = f_out(r) at x + c_d + n */

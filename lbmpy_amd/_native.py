@@ -30,7 +30,7 @@ STENCIL_NAMES = {"D2Q9": D2Q9, "D3Q19": D3Q19, "D3Q27": D3Q27}
 
 
 BC_NOSLIP, BC_UBB, BC_FREESLIP, BC_FIXED_DENSITY, BC_OUTFLOW, BC_OUTFLOW_INIT = 0, 1, 2, 3, 4, 5
-BC_FREESLIP_LINKS = 6
+BC_FREESLIP_LINKS, BC_SIMPLE_OUTFLOW = 6, 7
 
 
 class BoundaryDesc(ctypes.Structure):

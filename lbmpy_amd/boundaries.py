@@ -238,6 +238,26 @@ class BoundaryData:
         return self.fluid_cell_positions(coord) + self._c[self._links[:, 3], coord]
 
 
+class SimpleExtrapolationOutflow(LbBoundary):
+    """Outflow by copying along the normal (boundaryconditions.py:844-891, Geier et al. 2015 eq. F.1):
+    f_inv(d)(x_b) = f*_inv(d)(x + c_d - n), with ``normal_direction`` pointing out of the fluid (a vector or a
+    compass string such as 'E')."""
+    kind = nat.BC_SIMPLE_OUTFLOW
+
+    def __init__(self, normal_direction, stencil, name=None):
+        dim = len(stencil[0])
+        if isinstance(normal_direction, str):
+            v = [0, 0, 0]
+            for ch in normal_direction:
+                v = [a + b for a, b in zip(v, _COMPASS[ch])]
+            normal_direction = tuple(v[:dim])
+        self.normal_direction = tuple(int(n) for n in normal_direction)
+        if any(n not in (-1, 0, 1) for n in self.normal_direction):
+            raise ValueError("Only -1, 0 and 1 allowed for defining the normal direction")
+        self.dim = dim
+        super().__init__(name or f"Simple Outflow: {self.normal_direction}")
+
+
 def slice_from_direction(direction_name, dim, normal_offset=0, tangential_offset=0):
     """Slice of the ghost layer in a compass direction ('N', 'S', 'E', 'W', 'T', 'B'),
     as pystencils' ``slice_from_direction`` returns it for ghost-layer inclusive arrays."""
@@ -341,9 +361,10 @@ class LatticeBoltzmannBoundaryHandling:
         where the callback, given the cells' midpoint coordinates, returns True (boundaryhandling.py:218-266).
         Returns the boundary object's flag value (its id here, not a bit).  A cell holds one boundary: the
         last one set wins."""
-        if not isinstance(boundary_obj, (NoSlip, UBB, FreeSlip, FixedDensity, ExtrapolationOutflow)):
-            raise NotImplementedError(f"boundary {boundary_obj!r} is not available on the GPU path "
-                                      "(NoSlip, UBB, FreeSlip, FixedDensity and ExtrapolationOutflow are)")
+        if not isinstance(boundary_obj, (NoSlip, UBB, FreeSlip, FixedDensity, ExtrapolationOutflow,
+                                         SimpleExtrapolationOutflow)):
+            raise NotImplementedError(f"boundary {boundary_obj!r} is not available on the GPU path (NoSlip, UBB, "
+                                      "FreeSlip, FixedDensity, ExtrapolationOutflow and SimpleExtrapolationOutflow are)")
         if getattr(boundary_obj, "dim", self.dim) != self.dim:
             raise ValueError(f"boundary dimension {boundary_obj.dim} does not match the domain ({self.dim})")
         bid = self._id_of(boundary_obj)
@@ -513,8 +534,14 @@ class LatticeBoltzmannBoundaryHandling:
             raise TypeError("boundaries run on GPU tensors of float64, float32 or float16")
         lib = nat.load()
         faces = self._faces
+        c = np.array(m.stencil.stencil_entries)
         for group in groups:
             obj, links, vel, ready = group
+            if isinstance(obj, (ExtrapolationOutflow, SimpleExtrapolationOutflow)):
+                dirs = links[:, 3].cpu().numpy() if hasattr(links, "cpu") else links[:, 3]
+                if np.any(c[dirs] @ np.array(obj.normal_direction[:D]) <= 0):
+                    raise ValueError(f"{obj.name}: the normal direction {obj.normal_direction} must point out of the "
+                                     "fluid, into the outflow cells")
             d = nat.BoundaryDesc()
             d.stencil = nat.STENCIL_NAMES[m.stencil.name]
             d.pattern = int(pattern_code)
@@ -537,6 +564,9 @@ class LatticeBoltzmannBoundaryHandling:
                         d.velocity[ax] = float(v)
                 d.density = float(obj._density) if obj._density is not None else 0.0
             elif isinstance(obj, FreeSlip) and obj.normal_direction is not None:
+                for ax, v in enumerate(obj.normal_direction):
+                    d.normal[ax] = int(v)
+            elif isinstance(obj, SimpleExtrapolationOutflow):
                 for ax, v in enumerate(obj.normal_direction):
                     d.normal[ax] = int(v)
             elif isinstance(obj, ExtrapolationOutflow):

@@ -231,6 +231,26 @@ __device__ __forceinline__ int64_t bc_out(const BoundaryArgs& b, int i, int cx, 
   return slot * b.sq + wrap_axis(cx, b.n[0], b.g, b.periodic[0]) + wrap_axis(cy, b.n[1], b.g, b.periodic[1]) * b.sy +
          (S::D == 3 ? wrap_axis(cz, b.n[2], b.g, b.periodic[2]) * b.sz : 0);
 }
+// whether bc_out(b, i, cx, cy, cz) addresses a cell of the array (ghost layers included): the outflow kinds read
+// one cell back along a normal the caller supplies, so a normal pointing into the fluid must not read outside
+template <class S, int PAT>
+__device__ __forceinline__ bool bc_out_in_array(const BoundaryArgs& b, int i, int cx, int cy, int cz) {
+  if (PAT == LBM355_AA_ODD) {
+    cx += S::c[i][0];
+    cy += S::c[i][1];
+    cz += S::c[i][2];
+  } else {
+    const Access t = pattern_access<S>(PAT, i);
+    cx += t.w[0];
+    cy += t.w[1];
+    cz += t.w[2];
+  }
+  const int c[3] = {cx, cy, cz};
+  for (int a = 0; a < S::D; ++a)
+    if (!b.periodic[a] && (c[a] < 0 || c[a] >= b.n[a] + 2 * b.g)) return false;
+  return true;
+}
+
 // the read accessor of the kernel that runs next
 template <class S, int PAT>
 __device__ __forceinline__ int64_t bc_in(const BoundaryArgs& b, int i, int cx, int cy, int cz) {
@@ -260,6 +280,15 @@ __global__ void boundary_links_kernel(T* pdf, const int32_t* links, int64_t n, d
       cdy = S::c[i][1];
       cdz = S::c[i][2];
     }
+  if (KIND == LBM355_BC_SIMPLE_OUTFLOW || KIND == LBM355_BC_OUTFLOW_INIT || KIND == LBM355_BC_OUTFLOW) {
+    if (!bc_out_in_array<S, PAT>(b, id, x + cdx - b.normal[0], y + cdy - b.normal[1], z + cdz - b.normal[2])) return;
+  }
+  if (KIND == LBM355_BC_SIMPLE_OUTFLOW) {
+    // copy from the fluid cell one step back along the normal from the boundary cell: x + c_d - n
+    pdf[bc_in<S, PAT>(b, id, x, y, z)] =
+        pdf[bc_out<S, PAT>(b, id, x + cdx - b.normal[0], y + cdy - b.normal[1], z + cdz - b.normal[2])];
+    return;
+  }
   if (KIND == LBM355_BC_OUTFLOW_INIT || KIND == LBM355_BC_OUTFLOW) {
     // the fluid cell one step back along the normal from the boundary cell: x + c_d - n
     const T v = pdf[bc_out<S, PAT>(b, id, x + cdx - b.normal[0], y + cdy - b.normal[1], z + cdz - b.normal[2])];
@@ -608,6 +637,9 @@ int launch_bc_kind(const lbm355_boundary_desc* d, T* pdf, const int32_t* links, 
   else if (d->kind == LBM355_BC_OUTFLOW_INIT)
     hipLaunchKernelGGL((boundary_links_kernel<S, PAT, T, LBM355_BC_OUTFLOW_INIT>), dim3(grid), dim3(256), 0, s, pdf,
                        links, n, vel, b);
+  else if (d->kind == LBM355_BC_SIMPLE_OUTFLOW)
+    hipLaunchKernelGGL((boundary_links_kernel<S, PAT, T, LBM355_BC_SIMPLE_OUTFLOW>), dim3(grid), dim3(256), 0, s, pdf,
+                       links, n, vel, b);
   else if (d->kind == LBM355_BC_FREESLIP_LINKS)
     hipLaunchKernelGGL((boundary_links_kernel<S, PAT, T, LBM355_BC_FREESLIP_LINKS>), dim3(grid), dim3(256), 0, s, pdf,
                        links, n, vel, b);
@@ -741,7 +773,7 @@ int lbm355_boundary_links(const lbm355_boundary_desc* d, void* pdf, const int32_
   if (table_pattern(d->pattern) && (d->fused_noslip[0] || d->fused_noslip[1] || d->fused_noslip[2]))
     return fail(LBM355_ERR_ARG, "fused NoSlip faces exist with pull and AA streaming only");
   if (d->store_dtype < 0 || d->store_dtype > 2) return fail(LBM355_ERR_ARG, "bad dtype");
-  if (d->kind < LBM355_BC_NOSLIP || d->kind > LBM355_BC_FREESLIP_LINKS) return fail(LBM355_ERR_ARG, "bad boundary kind");
+  if (d->kind < LBM355_BC_NOSLIP || d->kind > LBM355_BC_SIMPLE_OUTFLOW) return fail(LBM355_ERR_ARG, "bad boundary kind");
   if (d->kind == LBM355_BC_FREESLIP_LINKS && !link_velocity)
     return fail(LBM355_ERR_ARG, "FreeSlip with per-link normals needs its link data (normal, reference direction)");
   if ((d->kind == LBM355_BC_OUTFLOW || d->kind == LBM355_BC_OUTFLOW_INIT) && !link_velocity)

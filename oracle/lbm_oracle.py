@@ -786,6 +786,11 @@ class OracleDomain:
                 mi = np.array([self._dir_index(tuple(v)) for v in mirrored])
                 src_cell = [cell[a] + self.c[d, a] + nrm[a] for a in range(D)]
                 arr[target] = arr[self._out_addr(prev, self.inv[mi], src_cell)]
+            elif spec["kind"] == "simple_outflow":
+                # copy along the normal (boundaryconditions.py:844-891): f_in(inv d) = f_out(inv d) at x + c_d - n
+                nrm = np.asarray(spec["normal"], dtype=int)
+                src = [cell[a] + self.c[d, a] - nrm[a] for a in range(D)]
+                arr[target] = arr[self._out_addr(prev, self.inv[d], src)]
             elif spec["kind"] == "outflow":
                 # extrapolation outflow (boundaryconditions.py:894-1042) with per-link state (s0, s1)
                 nrm = np.asarray(spec["normal"], dtype=int)

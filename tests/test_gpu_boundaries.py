@@ -260,3 +260,43 @@ def test_freeslip_obstacle_with_per_cell_normals_matches_oracle(stencil, pattern
     ref = dom.pdfs_in_natural_layout()[:, fluid]
     assert np.abs(got - ref).max() <= 1e-12
     assert np.abs(ref - f0[:, fluid]).max() > 1e-6
+
+
+@pytest.mark.parametrize("pattern", ["pull", "aa"])
+def test_simple_extrapolation_outflow_channel(pattern):
+    """Channel with a UBB inlet (W), NoSlip walls (N, S), a NoSlip block and SimpleExtrapolationOutflow (E)
+    (boundaryconditions.py:844-891), against the oracle after 60 steps; a normal pointing into the fluid is
+    rejected before any launch."""
+    from lbmpy_amd.boundaries import SimpleExtrapolationOutflow
+    st = LBStencil(Stencil.D2Q9)
+    lc = LBMConfig(stencil=st, method=Method.TRT, relaxation_rates=[1.7, 1.2], compressible=True,
+                   streaming_pattern=pattern)
+    step = _make((24, 10), (False, False), lc)
+    step.boundary_handling.set_boundary(UBB((0.03, 0.0)), slice_from_direction("W", 2))
+    step.boundary_handling.set_boundary(SimpleExtrapolationOutflow("E", st), slice_from_direction("E", 2))
+    for w in ("N", "S"):
+        step.boundary_handling.set_boundary(NoSlip(), slice_from_direction(w, 2))
+    step.boundary_handling.set_boundary(NoSlip(), mask_callback=lambda x, y: (x > 8) & (x < 11) & (y > 3) & (y < 6))
+    print(f"simple outflow {pattern}", flush=True)
+    step.run(60)
+    got = _natural(step)
+    bh = step.boundary_handling
+    specs = []
+    for k, o in enumerate(bh._objects):
+        spec = {"kind": "simple_outflow", "normal": o.normal_direction} \
+            if isinstance(o, SimpleExtrapolationOutflow) else _spec(o)
+        specs.append((bh.flags == k + 1, spec))
+    m = step.method
+    dom = OracleDomain(m, (24, 10), faces={0: "ghost", 1: "ghost"}, pattern=pattern, boundaries=specs)
+    f0 = equilibrium_pdfs(m, np.ones((10, 24)), np.zeros((2, 10, 24)))
+    dom.set_pdfs_interior(f0[dom.inv] if pattern == "aa" else f0)
+    for _ in range(60):
+        dom.step()
+    fluid = ~bh.get_mask(None, None).T
+    ref = dom.pdfs_in_natural_layout()
+    assert np.abs(got[:, fluid] - ref[:, fluid]).max() <= 1e-12
+    assert np.abs(got - f0).max() > 1e-4
+    bad = _make((24, 10), (False, False), lc)
+    bad.boundary_handling.set_boundary(SimpleExtrapolationOutflow((-1, 0), st), slice_from_direction("E", 2))
+    with pytest.raises(ValueError):
+        bad.run(1)

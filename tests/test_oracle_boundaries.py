@@ -180,3 +180,30 @@ def test_outflow_passes_uniform_flow(pattern):
     for _ in range(30):
         dom.step()
     assert np.abs(dom.pdfs_in_natural_layout() - f0).max() <= 1e-15
+
+
+def _uniform_flow_with(pattern, outlet_spec, steps=30):
+    U = 0.03
+    m = create_lb_method(LBMConfig(stencil=LBStencil(Stencil.D2Q9), method=Method.TRT, relaxation_rates=[1.6, 1.2],
+                                   streaming_pattern=pattern))
+    nx, ny = 12, 6
+    gs = (nx + 2, ny + 2)
+    bnd = [(_plane(gs, 0, 0), {"kind": "ubb", "velocity": (U, 0.0), "density": None}), (_plane(gs, 0, -1), outlet_spec)]
+    from oracle.lbm_oracle import equilibrium_pdfs
+    u = np.zeros((2, ny, nx))
+    u[0] = U
+    f0 = equilibrium_pdfs(m, np.ones((ny, nx)), u)
+    dom = OracleDomain(m, (nx, ny), faces={0: "ghost", 1: "periodic"}, pattern=pattern, boundaries=bnd)
+    dom.set_pdfs_interior(f0[dom.inv] if pattern == "aa" else f0)
+    for _ in range(steps):
+        dom.step()
+    return np.abs(dom.pdfs_in_natural_layout() - f0).max()
+
+
+@pytest.mark.parametrize("pattern", ["pull", "aa"])
+def test_simple_outflow_passes_uniform_flow(pattern):
+    """SimpleExtrapolationOutflow (boundaryconditions.py:844-891) copies along the normal, so the uniform
+    equilibrium flow through a UBB inlet stays exact; copying from the outflow cell itself instead of the fluid
+    cell upstream (a planted mistake: normal 0) breaks it."""
+    assert _uniform_flow_with(pattern, {"kind": "simple_outflow", "normal": (1, 0)}) <= 1e-15
+    assert _uniform_flow_with(pattern, {"kind": "simple_outflow", "normal": (0, 0)}, steps=4) > 1e-6
