@@ -300,3 +300,37 @@ def test_simple_extrapolation_outflow_channel(pattern):
     bad.boundary_handling.set_boundary(SimpleExtrapolationOutflow((-1, 0), st), slice_from_direction("E", 2))
     with pytest.raises(ValueError):
         bad.run(1)
+
+
+@pytest.mark.parametrize("pattern", ["pull", "aa"])
+def test_ubb_adapt_velocity_to_force(pattern):
+    """UBB(adapt_velocity_to_force=True) (boundaryconditions.py:808-812) drives the wall at u_w - F/2, the
+    equilibrium input velocity at density 1: the same run as a UBB at that velocity, for a constant wall velocity
+    and for a callback, and without a force it changes nothing."""
+    st = LBStencil(Stencil.D2Q9)
+    F = (2e-4, -1e-4)
+    runs = {}
+    for key, ubb in [("adapt", UBB((0.03, 0.0), adapt_velocity_to_force=True)),
+                     ("shifted", UBB((0.03 - F[0] / 2, -F[1] / 2))),
+                     ("callback", UBB(lambda x, y, d: np.tile([0.03, 0.0], (len(d), 1)), dim=2,
+                                      adapt_velocity_to_force=True))]:
+        lc = LBMConfig(stencil=st, method=Method.TRT, relaxation_rates=[1.7, 1.2], compressible=True, force=F,
+                       streaming_pattern=pattern)
+        step = _make((16, 10), (True, False), lc)
+        step.boundary_handling.set_boundary(ubb, slice_from_direction("N", 2))
+        step.boundary_handling.set_boundary(NoSlip(), slice_from_direction("S", 2))
+        step.run(20)
+        runs[key] = step.pdf_array.cpu().numpy()
+    np.testing.assert_array_equal(runs["adapt"], runs["shifted"])
+    np.testing.assert_array_equal(runs["callback"], runs["shifted"])
+    lc = LBMConfig(stencil=st, method=Method.TRT, relaxation_rates=[1.7, 1.2], compressible=True,
+                   streaming_pattern=pattern)
+    outs = []
+    for adapt in (True, False):
+        step = _make((16, 10), (True, False), lc)
+        step.boundary_handling.set_boundary(UBB((0.03, 0.0), adapt_velocity_to_force=adapt),
+                                            slice_from_direction("N", 2))
+        step.boundary_handling.set_boundary(NoSlip(), slice_from_direction("S", 2))
+        step.run(10)
+        outs.append(step.pdf_array.cpu().numpy())
+    np.testing.assert_array_equal(outs[0], outs[1])
