@@ -118,6 +118,19 @@ def macroscopic_from_aa_storage(storage, zero_centered=True):
     return rho, u
 
 
+def aa_schedule(warmup, steps, total_steps):
+    """Global step indices of the warm-up, timed and untimed tail launches of an in-place (AA) run.
+
+    One counter runs through all three stretches, so launch t uses kernel t % 2 (0 = ODD, the first AA step,
+    1 = EVEN) whatever the split, and the tail runs up to the first even total >= max(total_steps,
+    warmup + steps): the AA state is read in the layout an EVEN kernel leaves (slot inv(i) of every cell)."""
+    if warmup < 0 or steps < 1 or total_steps < 0:
+        raise ValueError("need warmup >= 0, steps >= 1 and total_steps >= 0")
+    total = max(total_steps, warmup + steps)
+    total += total % 2
+    return range(0, warmup), range(warmup, warmup + steps), range(warmup + steps, total)
+
+
 def run_form(dtype, n, steps, warmup, total_steps, u0=0.05):
     """Initialise the TGV, run `warmup` untimed steps, `steps` timed steps (one event pair per launch on
     the launch stream), then untimed steps up to `total_steps`; returns timings and (rho, u)."""
@@ -147,29 +160,33 @@ def run_form(dtype, n, steps, warmup, total_steps, u0=0.05):
     torch.cuda.synchronize()
 
     kernels = (odd, even)                                 # the first AA step is ODD
-    for t in range(warmup):
+    warm, timed, tail = aa_schedule(warmup, steps, total_steps)
+    for t in warm:
         kernels[t % 2](src=pdf)
     torch.cuda.synchronize()
 
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
     torch.cuda.synchronize()
     ev[0].record()
-    for t in range(steps):
+    for i, t in enumerate(timed):
         kernels[t % 2](src=pdf)
-        ev[t + 1].record()
+        ev[i + 1].record()
     torch.cuda.synchronize()
     per = [ev[i].elapsed_time(ev[i + 1]) for i in range(steps)]
     ms = ev[0].elapsed_time(ev[-1])
+    odd_ms = [p for p, t in zip(per, timed) if t % 2 == 0]
+    even_ms = [p for p, t in zip(per, timed) if t % 2 == 1]
 
-    for t in range(warmup + steps, total_steps):
+    for t in tail:
         kernels[t % 2](src=pdf)
     torch.cuda.synchronize()
     rho_f, u_f = macroscopic_from_aa_storage(storage)
     finite = bool(torch.isfinite(u_f).all().item())
     del pdf, storage
     torch.cuda.empty_cache()
-    return dict(ms=ms, odd_ms=sum(per[0::2]) / len(per[0::2]), even_ms=sum(per[1::2]) / len(per[1::2]),
-                launch_ms=sum(per) / len(per), omega=omega, rho=rho_f, u=u_f, finite=finite)
+    avg = (lambda v: sum(v) / len(v) if v else None)
+    return dict(ms=ms, odd_ms=avg(odd_ms), even_ms=avg(even_ms), launch_ms=avg(per), omega=omega, rho=rho_f,
+                u=u_f, finite=finite, total_steps=tail.stop)
 
 
 def poiseuille_startup(y, t, force, nu, h, terms=4000):
@@ -386,14 +403,15 @@ def run_patterns(args):
             else:
                 ks[t % 2](src=a)
 
-        for t in range(args.warmup):
+        warm, timed, _ = aa_schedule(args.warmup, args.steps, 0)
+        for t in warm:
             launch(t)
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
         torch.cuda.synchronize()
         ev[0].record()
-        for t in range(args.steps):
+        for i, t in enumerate(timed):
             launch(t)
-            ev[t + 1].record()
+            ev[i + 1].record()
         torch.cuda.synchronize()
         per = [ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)]
         launch_ms = sum(per) / len(per)
@@ -441,8 +459,8 @@ def main():
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (pass --gpus 1)", file=sys.stderr)
         sys.exit(2)
-    if args.steps % 2 or args.warmup % 2 or args.total_steps % 2:
-        print("AA streaming needs even step counts", file=sys.stderr)
+    if args.steps < 1 or args.warmup < 0:
+        print("need --steps >= 1 and --warmup >= 0", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "channel":
@@ -461,7 +479,7 @@ def main():
     import torch
     n = args.n
     cells = n ** 3
-    total = max(args.total_steps, args.warmup + args.steps)
+    total = aa_schedule(args.warmup, args.steps, args.total_steps)[2].stop
     forms = [args.dtype] + [f for f in args.forms.split(",") if f and f != args.dtype]
     if "float64" in forms:
         forms = ["float64"] + [f for f in forms if f != "float64"]
@@ -474,7 +492,8 @@ def main():
         b = 2 * 27 * size[dt]
         res = {"mlups": round(cells * args.steps / (r["ms"] / 1e3) / 1e6, 1),
                "ms_per_step": round(r["ms"] / args.steps, 4),
-               "kernel_ms": {"aa_odd": round(r["odd_ms"], 4), "aa_even": round(r["even_ms"], 4)},
+               "kernel_ms": {"aa_" + k: (None if r[k + "_ms"] is None else round(r[k + "_ms"], 4))
+                             for k in ("odd", "even")},
                "achieved_gbs": round(b * cells / (r["launch_ms"] / 1e3) / 1e9, 1), "bytes_per_cell_step": b,
                "finite": r["finite"]}
         res["roofline_frac"] = round(res["achieved_gbs"] / HBM_PEAK_GBS, 4)

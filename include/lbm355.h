@@ -172,6 +172,14 @@ int lbm355_get_strain_rate(const lbm355_sweep_desc* desc, const void* pdf, const
  * bandwidth and the FETCH_SIZE / WRITE_SIZE counters against the sweep. */
 int lbm355_stream_copy(const void* src, void* dst, int64_t n_floats, int32_t nontemporal, void* stream);
 
+/* Copy of n_planes (19 or 27) planes of plane_elems elements each, plane q at q * plane_stride elements, with
+ * one thread per element index loading that element of every plane and then storing them all: the sweep's
+ * HBM access pattern (Q aligned streams read and Q written) without the collision.  src == dst is allowed
+ * (in place, as an AA-even sweep accesses memory).  No lbmpy counterpart: it measures the ceiling of the
+ * fzyx layout that the sweeps are held against. */
+int lbm355_plane_copy(int32_t dtype, int32_t n_planes, const void* src, void* dst, int64_t plane_elems,
+                      int64_t plane_stride, int32_t nontemporal, void* stream);
+
 /* ---- Boundary links (general) --------------------------------------------------------------------
  * One boundary condition applied on an index list of links between two sweeps, addressed the way
  * lbmpy's BetweenTimestepsIndexing does (advanced_streaming/indexing.py:12-181): f_out(i) through
@@ -302,6 +310,10 @@ const char* lbm355_last_error(void);
 
 /* ABI version. */
 int32_t lbm355_version(void);
+
+/* SHA-256 (hex) of the library's sources (lbmpy_amd/csrc/*.hip, *.h and this header) at build time, so a
+ * loader can refuse a library built from other sources.  "unknown" when built without it. */
+const char* lbm355_source_hash(void);
 
 #ifdef __cplusplus
 }

@@ -10,7 +10,7 @@ import os
 
 import torch  # noqa: F401  (must precede the library load, see module docstring)
 
-from .csrc.build import library_path
+from .csrc.build import library_path, source_hash
 
 c_int32, c_int64, c_double, c_void_p = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
 
@@ -60,7 +60,8 @@ EXPORTED = ("lbm355_sweep", "lbm355_table_elements", "lbm355_noslip_links", "lbm
             "lbm355_batch_info", "lbm355_batch_destroy", "lbm355_closure_params_size",
             "lbm355_closure_master_size", "lbm355_closure_features", "lbm355_closure_forward",
             "lbm355_closure_train_step", "lbm355_closure_adam", "lbm355_boundary_links", "lbm355_copy_boxes",
-            "lbm355_pattern_access", "lbm355_get_strain_rate")
+            "lbm355_pattern_access", "lbm355_get_strain_rate", "lbm355_plane_copy",
+            "lbm355_source_hash")
 
 _lib = None
 
@@ -92,6 +93,8 @@ def load(path=None):
     lib.lbm355_get_macroscopic.restype = c_int32
     lib.lbm355_stream_copy.argtypes = [c_void_p, c_void_p, c_int64, c_int32, c_void_p]
     lib.lbm355_stream_copy.restype = c_int32
+    lib.lbm355_plane_copy.argtypes = [c_int32, c_int32, c_void_p, c_void_p, c_int64, c_int64, c_int32, c_void_p]
+    lib.lbm355_plane_copy.restype = c_int32
     lib.lbm355_batch_create.argtypes = [ctypes.POINTER(SweepDesc), ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
                                         c_int32, ctypes.POINTER(c_void_p)]
     lib.lbm355_batch_create.restype = c_int32
@@ -132,6 +135,14 @@ def load(path=None):
     lib.lbm355_last_error.restype = ctypes.c_char_p
     lib.lbm355_version.argtypes = []
     lib.lbm355_version.restype = c_int32
+    lib.lbm355_source_hash.argtypes = []
+    lib.lbm355_source_hash.restype = ctypes.c_char_p
+    if path == library_path():
+        built = lib.lbm355_source_hash().decode()
+        want = source_hash()
+        if built != want:
+            raise ImportError(f"{path} was built from other sources (library {built[:12]}, tree {want[:12]}): "
+                              "rebuild it with `python -m lbmpy_amd.csrc.build` (or __graft_entry__.build())")
     if path == library_path():
         _lib = lib
     return lib

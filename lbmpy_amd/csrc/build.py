@@ -4,6 +4,7 @@ Each ``.hip`` translation unit is compiled to an object in parallel, then linked
 ``lbmpy_amd/_lib/liblbm355.so``.  Run ``python -m lbmpy_amd.csrc.build`` or call
 :func:`build`.
 """
+import hashlib
 import os
 import subprocess
 import sys
@@ -13,7 +14,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 LIBDIR = os.path.join(PKG, "_lib")
 LIBNAME = "liblbm355.so"
-SOURCES = ["sweep.hip", "sweep_eso.hip", "macroscopic.hip", "batch.hip", "closure.hip", "periodic.hip"]
+SOURCES = ["sweep.hip", "sweep_moment_f64.hip", "sweep_moment_f32.hip", "sweep_central_f64.hip",
+           "sweep_central_f32.hip", "sweep_cumulant_f64.hip", "sweep_cumulant_f32.hip",
+           "sweep_cumulant_corrected_f64.hip", "sweep_cumulant_corrected_f32.hip", "sweep_cumulant_learned_f32.hip",
+           "sweep_eso.hip", "macroscopic.hip", "batch.hip", "closure.hip", "periodic.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -30,16 +34,35 @@ def library_path():
     return os.path.join(LIBDIR, LIBNAME)
 
 
+def source_files():
+    """The files the library is built from: every .hip and .h under csrc/ and the C-ABI header."""
+    names = sorted(n for n in os.listdir(HERE) if n.endswith((".hip", ".h")))
+    return [os.path.join(HERE, n) for n in names] + [os.path.join(PKG, os.pardir, "include", "lbm355.h")]
+
+
+def source_hash():
+    """SHA-256 over the names and contents of source_files(), as compiled into lbm355_source_hash()."""
+    h = hashlib.sha256()
+    for p in source_files():
+        h.update(os.path.basename(p).encode() + b"\0")
+        with open(p, "rb") as fh:
+            h.update(fh.read())
+        h.update(b"\0")
+    return h.hexdigest()
+
+
 def build(verbose=False, jobs=None):
     os.makedirs(LIBDIR, exist_ok=True)
     objdir = os.path.join(LIBDIR, "obj")
     os.makedirs(objdir, exist_ok=True)
     objs = []
     cmds = []
+    digest = source_hash()
     for src in SOURCES:
         obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
-        cmds.append([HIPCC, *HIP_FLAGS, "-I", os.path.join(PKG, os.pardir, "include"), "-c",
+        cmds.append([HIPCC, *HIP_FLAGS, f'-DLBM355_SOURCE_HASH="{digest}"',
+                     "-I", os.path.join(PKG, os.pardir, "include"), "-c",
                      os.path.join(HERE, src), "-o", obj])
 
     def run(cmd):
@@ -50,7 +73,7 @@ def build(verbose=False, jobs=None):
             raise RuntimeError(f"hipcc failed ({' '.join(cmd)}):\n{r.stdout}\n{r.stderr}")
         return r
 
-    with ThreadPoolExecutor(max_workers=jobs or min(8, len(cmds))) as ex:
+    with ThreadPoolExecutor(max_workers=jobs or min(os.cpu_count() or 8, 16, len(cmds))) as ex:
         list(ex.map(run, cmds))
     tmp = library_path() + ".tmp"
     run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])

@@ -410,7 +410,60 @@ __global__ void __launch_bounds__(256) stream_copy_kernel(const float* __restric
   else
     dst[t] = src[t];
 }
+
+// Q planes per thread, the sweep's access pattern without the collision: thread t loads element t of every
+// plane, then stores them all (in place when src == dst, as an AA-even sweep does).
+template <int P, typename T, int NT>
+__global__ void __launch_bounds__(256) plane_copy_kernel(const T* src, T* dst, int64_t n,
+                                                         int64_t stride) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  T v[P];
+#pragma unroll
+  for (int q = 0; q < P; ++q) v[q] = NT ? __builtin_nontemporal_load(src + q * stride + t) : src[q * stride + t];
+#pragma unroll
+  for (int q = 0; q < P; ++q) {
+    if (NT)
+      __builtin_nontemporal_store(v[q], dst + q * stride + t);
+    else
+      dst[q * stride + t] = v[q];
+  }
+}
+
+template <int P, typename T>
+int plane_copy_launch(const void* src, void* dst, int64_t n, int64_t stride, int32_t nt, hipStream_t s) {
+  const int64_t blocks = (n + 255) / 256;
+  if (nt)
+    hipLaunchKernelGGL((plane_copy_kernel<P, T, 1>), dim3(blocks), dim3(256), 0, s, (const T*)src, (T*)dst, n, stride);
+  else
+    hipLaunchKernelGGL((plane_copy_kernel<P, T, 0>), dim3(blocks), dim3(256), 0, s, (const T*)src, (T*)dst, n, stride);
+  return hipGetLastError() == hipSuccess ? LBM355_OK : lbm355_set_error(LBM355_ERR_LAUNCH, "plane copy launch failed");
+}
 }  // namespace
+
+extern "C" int lbm355_plane_copy(int32_t dtype, int32_t n_planes, const void* src, void* dst, int64_t plane_elems,
+                                 int64_t plane_stride, int32_t nontemporal, void* stream) {
+  if (!src || !dst || plane_elems < 0 || plane_stride < plane_elems)
+    return lbm355_set_error(LBM355_ERR_ARG, "bad plane-copy arguments (plane_stride must be >= plane_elems)");
+  if (n_planes != 19 && n_planes != 27)
+    return lbm355_set_error(LBM355_ERR_UNSUPPORTED, "plane copy: n_planes must be 19 or 27");
+  if (plane_elems == 0) return LBM355_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const int nt = nontemporal ? 1 : 0;
+  switch (dtype) {
+    case LBM355_F64:
+      return n_planes == 19 ? plane_copy_launch<19, double>(src, dst, plane_elems, plane_stride, nt, s)
+                            : plane_copy_launch<27, double>(src, dst, plane_elems, plane_stride, nt, s);
+    case LBM355_F32:
+      return n_planes == 19 ? plane_copy_launch<19, float>(src, dst, plane_elems, plane_stride, nt, s)
+                            : plane_copy_launch<27, float>(src, dst, plane_elems, plane_stride, nt, s);
+    case LBM355_F16:
+      return n_planes == 19 ? plane_copy_launch<19, _Float16>(src, dst, plane_elems, plane_stride, nt, s)
+                            : plane_copy_launch<27, _Float16>(src, dst, plane_elems, plane_stride, nt, s);
+    default:
+      return lbm355_set_error(LBM355_ERR_ARG, "plane copy: unknown dtype");
+  }
+}
 
 extern "C" int lbm355_stream_copy(const void* src, void* dst, int64_t n_floats, int32_t nontemporal, void* stream) {
   if (!src || !dst || n_floats < 0) return lbm355_set_error(LBM355_ERR_ARG, "bad copy arguments");
