@@ -100,6 +100,49 @@ def profiled_traffic(n, dtype):
     return int(m.group(1)) if m else None
 
 
+def calibration_copies(n, dtype, elem_bytes, reps=5):
+    """Achievable bandwidth of two copies as large as the headline PDF field (27 n^3 elements of `dtype`),
+    timed with HIP events on the launch stream after the timed region:
+    - flat: one FP32 stream, regular and non-temporal accesses (lbm355_stream_copy); the same copy scales
+      the FETCH_SIZE / WRITE_SIZE counters;
+    - plane27: the sweep's own access pattern without the collision, 27 aligned planes read and 27 written per
+      element index (lbm355_plane_copy), out of place and in place (as an AA sweep accesses memory).
+    GB/s count read + written bytes."""
+    import ctypes
+    import torch
+    from lbmpy_amd import _native as nat
+    lib = nat.load()
+    code = {"float64": 0, "float32": 1, "float16": 2}[dtype]
+    nbytes = 27 * n ** 3 * elem_bytes
+    a_buf = torch.zeros(nbytes // 4, dtype=torch.float32, device="cuda")
+    b_buf = torch.empty_like(a_buf)
+    nfl = a_buf.numel()
+    plane = n ** 3
+    stream = nat.current_stream_handle()
+    pa, pb = ctypes.c_void_p(a_buf.data_ptr()), ctypes.c_void_p(b_buf.data_ptr())
+    runs = {
+        "flat_regular": lambda: lib.lbm355_stream_copy(pa, pb, nfl, 0, stream),
+        "flat_nontemporal": lambda: lib.lbm355_stream_copy(pa, pb, nfl, 1, stream),
+        "plane27": lambda: lib.lbm355_plane_copy(code, 27, pa, pb, plane, plane, 0, stream),
+        "plane27_nontemporal": lambda: lib.lbm355_plane_copy(code, 27, pa, pb, plane, plane, 1, stream),
+        "plane27_in_place": lambda: lib.lbm355_plane_copy(code, 27, pa, pa, plane, plane, 0, stream),
+    }
+    out = {}
+    for name, fn in runs.items():
+        nat.check(fn())
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            nat.check(fn())
+        e1.record()
+        torch.cuda.synchronize()
+        out[name] = round(2 * nbytes * reps / (e0.elapsed_time(e1) / 1e3) / 1e9, 1)
+    print(f"calibration copies: {nbytes} bytes read and written per dispatch: {out}", flush=True)
+    del a_buf, b_buf
+    torch.cuda.empty_cache()
+    return out
+
+
 def macroscopic_from_aa_storage(storage, zero_centered=True):
     """(rho, u) in FP64 from an AA field after an even number of steps (slot j holds f_inv(j), so the
     velocity sums carry -c_j), evaluated plane by plane to bound the temporary memory."""
@@ -452,9 +495,10 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--channel-size", type=int, nargs=3, default=[512, 256, 256])
     ap.add_argument("--channel-steps", type=int, default=20000)
-    ap.add_argument("--calibrate", action="store_true",
-                    help="after the timed region, copy the PDF field (a plain copy of known size, used to scale "
-                         "the FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--no-calibrate", action="store_true",
+                    help="skip the copy measurements after the timed region (a flat copy of the PDF field's size, "
+                         "which also scales the FETCH_SIZE / WRITE_SIZE counters, and the 27-plane copy of the "
+                         "sweep's access pattern)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (pass --gpus 1)", file=sys.stderr)
@@ -511,33 +555,7 @@ def main():
         torch.cuda.empty_cache()
 
     head = results[args.dtype]
-    copy_gbs = None
-    if args.calibrate:
-        # plain copies of a field of the headline size with regular and non-temporal accesses: the counter
-        # scale for FETCH_SIZE / WRITE_SIZE and the achievable copy bandwidth
-        import ctypes
-        from lbmpy_amd import _native as nat
-        lib = nat.load()
-        nbytes = 27 * cells * size[args.dtype]
-        a_buf = torch.zeros(nbytes // 4, dtype=torch.float32, device="cuda")
-        b_buf = torch.empty_like(a_buf)
-        nfl = a_buf.numel()
-        copy_gbs = {}
-        for nt in (0, 1):
-            nat.check(lib.lbm355_stream_copy(ctypes.c_void_p(a_buf.data_ptr()), ctypes.c_void_p(b_buf.data_ptr()),
-                                             nfl, nt, nat.current_stream_handle()))
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record()
-            for _ in range(5):
-                nat.check(lib.lbm355_stream_copy(ctypes.c_void_p(a_buf.data_ptr()),
-                                                 ctypes.c_void_p(b_buf.data_ptr()), nfl, nt,
-                                                 nat.current_stream_handle()))
-            e1.record()
-            torch.cuda.synchronize()
-            copy_gbs["nontemporal" if nt else "regular"] = round(2 * nfl * 4 * 5 / (e0.elapsed_time(e1) / 1e3) / 1e9, 1)
-        print(f"calibration copy: {nfl * 4} bytes read and written per dispatch", flush=True)
-        del a_buf, b_buf
-
+    copy_gbs = None if args.no_calibrate else calibration_copies(n, args.dtype, size[args.dtype])
     cpu = None if args.no_cpu else cpu_baseline()
     b = head["bytes_per_cell_step"]
     achieved = b * cells / (head["_launch_ms"] / 1e3) / 1e9
@@ -556,7 +574,8 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "lbm355::sweep_kernel<D3Q27, cumulant, AA odd/even> (both parities, same bytes)",
                      "launch_ms": round(head["_launch_ms"], 4), "kernel_ms": head["kernel_ms"],
-                     "algorithmic_bytes_per_launch": b * cells, "copy_gbs": copy_gbs},
+                     "algorithmic_bytes_per_launch": b * cells, "copy_gbs": copy_gbs,
+                     "frac_of_plane27_copy": (round(achieved / copy_gbs["plane27"], 4) if copy_gbs else None)},
         "forms": forms_out,
         "cpu_baseline": cpu,
         "finite": all(v["finite"] for v in results.values()),

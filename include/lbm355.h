@@ -82,6 +82,10 @@ extern "C" {
 #define LBM355_FLAG_GALILEAN 64     /* D3Q27 cumulant: Galilean correction (galilean_correction.py:21-91) */
 #define LBM355_FLAG_FOURTH_ORDER 128 /* D3Q27 cumulant: fourth-order correction with limited third-order rates
                                         (fourth_order_correction.py:21-237); parameters in the table */
+#define LBM355_FLAG_ADAPT_VELOCITY 256 /* boundary links, UBB: drive the wall at its equilibrium input velocity
+                                          u_w - F / (2 rho_ref), rho_ref the link's density (the UBB's own, or
+                                          the fluid cell's for compressible methods) or 1 for incompressible ones
+                                          (UBB adapt_velocity_to_force, boundaryconditions.py:808-812) */
 
 /* error codes */
 #define LBM355_OK 0
@@ -107,6 +111,17 @@ typedef struct lbm355_sweep_desc {
   double omega_shear;    /* molecular shear relaxation rate (Smagorinsky, learned closure) */
   const void* tables;    /* device pointer: method table, compute_dtype elements */
   const void* closure;   /* device pointer: closure parameter block (LBM355_FLAG_LEARNED_CLOSURE), else NULL */
+  /* Optional per-cell fields beside the PDFs, each NULL or compute_dtype with the strides of the density /
+   * velocity outputs (x stride 1, out_stride_y / out_stride_z, components out_stride_c apart) and the PDF
+   * array's ghost layers: */
+  void* omega_out;          /* Smagorinsky: the cell's shear rate omega_sgs (lbmpy omega_output_field,
+                               turbulence_models.py:80-81) */
+  void* eddy_viscosity_out; /* Smagorinsky: Cs^2 * 3/2 * omega_sgs * Pi, the eddy viscosity (lbmpy
+                               eddy_viscosity_field, turbulence_models.py:74-78) */
+  const void* density_in;   /* moment family: density read instead of the PDFs' sum (lbmpy density_input,
+                               creationfunctions.py:662-673); needs velocity_in */
+  const void* velocity_in;  /* moment family: equilibrium velocity read instead of computed (lbmpy velocity_input,
+                               creationfunctions.py:675-677); D components */
 } lbm355_sweep_desc;
 
 /* One stream-collide time step over the interior.

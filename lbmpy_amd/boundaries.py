@@ -68,8 +68,9 @@ class UBB(LbBoundary):
     def __init__(self, velocity, density=None, adapt_velocity_to_force=False, dim=None, name=None,
                  data_type='double'):
         # with a force model, adapt_velocity_to_force sets the wall to the equilibrium input velocity of the
-        # conserved-quantity computation at its default density 1: u_w - F / 2 (boundaryconditions.py:808-812,
-        # conservedquantitycomputation.py:251-266)
+        # conserved-quantity computation, u_w - F / (2 rho_ref) (boundaryconditions.py:808-812,
+        # conservedquantitycomputation.py:251-266): rho_ref is the link's density for compressible methods (the
+        # UBB's own, or the fluid cell's from its populations, boundaryconditions.py:829-840) and 1 otherwise
         self.adapt_velocity_to_force = bool(adapt_velocity_to_force)
         if callable(velocity) and not dim:
             raise ValueError("When using a velocity callback the dimension has to be specified with the dim parameter")
@@ -179,16 +180,6 @@ class ExtrapolationOutflow(LbBoundary):
         self.dim = dim
         self.lb_method = lb_method
         super().__init__(name or f"Outflow: {normal_direction}")
-
-
-def _force_shift(ubb, method):
-    """F / 2 when the UBB adapts its velocity to the method's force, else 0 (per axis, 3 entries)."""
-    out = np.zeros(3)
-    fm = method.force_model
-    if getattr(ubb, "adapt_velocity_to_force", False) and fm is not None:
-        for a in range(method.dim):
-            out[a] = 0.5 * float(fm.force[a])
-    return out
 
 
 def _takes_boundary_data(callback):
@@ -489,7 +480,7 @@ class LatticeBoltzmannBoundaryHandling:
                     coords = [links[:, a] - g for a in range(D)]
                     v = np.asarray(obj._velocity(*coords, links[:, 3]), dtype=np.float64).reshape(len(links), D)
                 vel = np.zeros((len(links), 3))
-                vel[:, :D] = v - _force_shift(obj, self.method)[None, :D]
+                vel[:, :D] = v
             groups.append((obj, links, vel))
         return groups
 
@@ -572,10 +563,15 @@ class LatticeBoltzmannBoundaryHandling:
             d.stride_q = pdf.stride(D)
             if isinstance(obj, UBB):
                 if not obj.velocity_is_callable:
-                    shift = _force_shift(obj, m)
                     for ax, v in enumerate(obj._velocity):
-                        d.velocity[ax] = float(v) - float(shift[ax])
+                        d.velocity[ax] = float(v)
                 d.density = float(obj._density) if obj._density is not None else 0.0
+                fm = m.force_model
+                if obj.adapt_velocity_to_force and fm is not None:
+                    # the kernel drives the wall at u_w - F / (2 rho_ref) with the link's density
+                    d.flags |= nat.FLAG_ADAPT_VELOCITY
+                    for ax in range(D):
+                        d.force[ax] = float(fm.force[ax])
             elif isinstance(obj, FreeSlip) and obj.normal_direction is not None:
                 for ax, v in enumerate(obj.normal_direction):
                     d.normal[ax] = int(v)

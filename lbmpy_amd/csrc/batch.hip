@@ -56,6 +56,12 @@ __device__ __forceinline__ SweepArgs<TC> load_args(const CONST_AS SweepArgs<TC>*
   a.smag_cs2x18 = p->smag_cs2x18;
   a.omega_shear = p->omega_shear;
   a.closure = p->closure;
+  // batch_create rejects members with per-cell auxiliary fields
+  a.omega_out = nullptr;
+  a.nut_out = nullptr;
+  a.rho_in = nullptr;
+  a.u_in = nullptr;
+  a.nut_scale = TC(0);
   return a;
 }
 
@@ -229,6 +235,8 @@ int lbm355_batch_create(const lbm355_sweep_desc* descs, const void* const* pdf_s
     if (table_pattern(d.pattern)) return berr(LBM355_ERR_UNSUPPORTED, "batched ensembles stream with pull or AA");
     if (d.flags & (LBM355_FLAG_GALILEAN | LBM355_FLAG_FOURTH_ORDER))
       return berr(LBM355_ERR_UNSUPPORTED, "the cumulant corrections are not batched");
+    if (d.omega_out || d.eddy_viscosity_out || d.density_in || d.velocity_in)
+      return berr(LBM355_ERR_UNSUPPORTED, "per-cell subgrid outputs and density / velocity inputs are not batched");
     if (!pdf_src[k] || !pdf_dst[k]) return berr(LBM355_ERR_ARG, "NULL member PDF pointer");
     if (kind0 == LBM355_PULL && pdf_src[k] == pdf_dst[k]) return berr(LBM355_ERR_ARG, "pull needs two arrays");
     if (kind0 != LBM355_PULL && pdf_src[k] != pdf_dst[k]) return berr(LBM355_ERR_ARG, "AA needs src == dst");

@@ -99,7 +99,7 @@ struct CumulantTab {
 // built from c_ab / rho = gamma_ab + mu'_ab, take the deviations mu'_ab directly.
 template <class T, bool LEARNED = false, bool CORRECTED = false>
 __device__ __forceinline__ void collide_cumulant_d3q27(T (&f)[27], const SweepArgs<T>& a, T& rho_out_v,
-                                                       T (&u_out_v)[3]) {
+                                                       T (&u_out_v)[3], CellExtras<T>& ex) {
   using S = D3Q27;
   const TablePtr<T> __restrict__ r = table_ptr(a.tab) + CumulantTab::RATES;
   T m[3][3][3];  // [x][y][z]: first values at c=-1,0,+1, then exponents 0,1,2
@@ -183,6 +183,8 @@ __device__ __forceinline__ void collide_cumulant_d3q27(T (&f)[27], const SweepAr
       const T pi = sqrt(T(2) * (pxx * pxx + pyy * pyy + pzz * pzz + T(2) * (pxy * pxy + pxz * pxz + pyz * pyz))) * irho;
       const T tau0 = T(1) / a.omega_shear;
       w = T(2) / (tau0 + sqrt(a.smag_cs2x18 * pi + tau0 * tau0));
+      ex.omega_sgs = w;
+      ex.pi = pi;
     }
     if constexpr (LEARNED && sizeof(T) == 4) {
       // wave-collective: every lane of the wavefront reaches this point (the host requires nx % 64 == 0)

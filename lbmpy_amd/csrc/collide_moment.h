@@ -32,8 +32,10 @@ struct MomentTab {
                        SIZE = 16 * Q + 2 + 2 * Q * Q;
 };
 
+// o: the cell's offset in the optional per-cell fields (density / velocity inputs).
 template <class S, class T>
-__device__ __forceinline__ void collide_moment(T (&f)[S::Q], const SweepArgs<T>& a, T& rho_out_v, T (&u_out_v)[3]) {
+__device__ __forceinline__ void collide_moment(T (&f)[S::Q], const SweepArgs<T>& a, int64_t o, T& rho_out_v,
+                                               T (&u_out_v)[3]) {
   constexpr int Q = S::Q;
   constexpr int D = S::D;
   using L = MomentTab<Q>;
@@ -49,10 +51,9 @@ __device__ __forceinline__ void collide_moment(T (&f)[S::Q], const SweepArgs<T>&
   }
   const bool zc = a.flags & LBM355_FLAG_ZERO_CENTERED;
   const bool comp = a.flags & LBM355_FLAG_COMPRESSIBLE;
-  const T rho = zc ? T(1) + s : s;
-  const T drho = zc ? s : s - T(1);
+  T rho = zc ? T(1) + s : s;
+  T drho = zc ? s : s - T(1);
   const T inv_ref = comp ? T(1) / rho : T(1);
-  const T lam = comp ? rho : T(1);
   T u[3] = {0, 0, 0};
 #pragma unroll
   for (int d = 0; d < D; ++d) {
@@ -62,6 +63,17 @@ __device__ __forceinline__ void collide_moment(T (&f)[S::Q], const SweepArgs<T>&
     u_out_v[d] = (a.flags & LBM355_FLAG_FORCE) ? jr + shift : jr;
   }
   rho_out_v = rho;
+  // density / velocity inputs (creationfunctions.py:662-677): the equilibrium and the force term take the given
+  // values in place of the computed ones (the outputs above stay the PDFs' own, as lbmpy's output equations are)
+  if (a.u_in) {
+    if (a.rho_in) {
+      rho = a.rho_in[o];
+      drho = rho - T(1);
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) u[d] = a.u_in[o + d * a.osc];
+  }
+  const T lam = comp ? rho : T(1);
 
   // f_eq - f
   T dl[Q];

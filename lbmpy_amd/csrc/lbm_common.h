@@ -181,6 +181,19 @@ struct SweepArgs {
   TC smag_cs2x18;             // 18 * Cs^2 (Smagorinsky)
   TC omega_shear;             // shear rate replaced by the subgrid model
   const void* closure;        // learned-closure parameter block (LBM355_FLAG_LEARNED_CLOSURE)
+  // optional per-cell fields with the output strides (null when absent)
+  TC* omega_out;              // Smagorinsky shear rate
+  TC* nut_out;                // Smagorinsky eddy viscosity, nut_scale * omega_sgs * Pi
+  const TC* rho_in;           // moment family: density input
+  const TC* u_in;             // moment family: equilibrium-velocity input
+  TC nut_scale;               // 3/2 Cs^2
+};
+
+// What a collision hands back to the cell beyond the populations: the Smagorinsky shear rate and Pi
+// (sqrt(2 Pi_neq:Pi_neq) / rho), written to the optional per-cell outputs.
+template <class TC>
+struct CellExtras {
+  TC omega_sgs, pi;
 };
 
 // ---------------------------------------------------------------- method tables

@@ -52,7 +52,7 @@ __global__ void noslip_links_kernel(T* pdf, int64_t sy, int64_t sz, int64_t sq, 
 struct BoundaryArgs {
   int64_t sy, sz, sq;
   int n[3], g, periodic[3], fused[3];
-  int zc, comp, eq_shift;
+  int zc, comp, eq_shift, adapt;
   int normal[3], mirror_axis;
   double velocity[3];
   double density;
@@ -196,6 +196,11 @@ __global__ void boundary_links_kernel(T* pdf, const int32_t* links, int64_t n, d
 #pragma unroll
       for (int i = 0; i < S::Q; ++i) s += (TC)pdf[bc_out<S, PAT>(b, i, x, y, z)];
       rho = b.zc ? TC(1) + s : s;
+    }
+    if (b.adapt) {
+      const TC ref = b.comp ? rho : TC(1);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) u[a] -= TC(0.5) * (TC)b.force[a] / ref;
     }
     const TC cu = (TC)cdx * u[0] + (TC)cdy * u[1] + (TC)cdz * u[2];
     TC w = 0;
@@ -557,6 +562,23 @@ int lbm355_sweep(const lbm355_sweep_desc* d, const void* pdf_src, void* pdf_dst,
     if (!d->closure) return fail(LBM355_ERR_ARG, "closure parameter pointer is NULL");
     if (d->nx % 64) return fail(LBM355_ERR_UNSUPPORTED, "the learned closure needs nx % 64 == 0 (whole wavefronts)");
   }
+  if ((d->omega_out || d->eddy_viscosity_out) && !(d->flags & LBM355_FLAG_SMAGORINSKY))
+    return fail(LBM355_ERR_ARG, "the shear-rate and eddy-viscosity outputs belong to the Smagorinsky model");
+  if (d->density_in && !d->velocity_in)
+    return fail(LBM355_ERR_ARG, "a density input needs a velocity input as well");
+  if (d->velocity_in && d->family != LBM355_FAMILY_MOMENT)
+    return fail(LBM355_ERR_UNSUPPORTED, "density / velocity inputs are implemented for the moment family");
+  if (density_out || velocity_out || d->omega_out || d->eddy_viscosity_out || d->velocity_in) {
+    // the per-cell fields span the PDF array's spatial extent (ghost layers included) with x stride 1
+    const int D = stencil_d(d->stencil);
+    const int64_t g2 = 2 * d->ghost_layers;
+    if (d->out_stride_y < d->nx + g2) return fail(LBM355_ERR_ARG, "out_stride_y smaller than a row");
+    if (D == 3 && d->out_stride_z < d->out_stride_y * (d->ny + g2))
+      return fail(LBM355_ERR_ARG, "out_stride_z smaller than a plane");
+    const int64_t volume = D == 3 ? d->out_stride_z * (d->nz + g2) : d->out_stride_y * (d->ny + g2);
+    if ((velocity_out || d->velocity_in) && d->out_stride_c < volume)
+      return fail(LBM355_ERR_ARG, "out_stride_c smaller than one component");
+  }
   hipStream_t s = (hipStream_t)stream;
   if (d->compute_dtype == LBM355_F64) return launch_sweep<double>(d, pdf_src, pdf_dst, density_out, velocity_out, s);
   return launch_sweep<float>(d, pdf_src, pdf_dst, density_out, velocity_out, s);
@@ -625,6 +647,7 @@ int lbm355_boundary_links(const lbm355_boundary_desc* d, void* pdf, const int32_
   b.comp = (d->flags & LBM355_FLAG_COMPRESSIBLE) ? 1 : 0;
   b.density = d->density;
   b.eq_shift = (d->flags & LBM355_FLAG_EQ_SHIFT) ? 1 : 0;
+  b.adapt = (d->flags & LBM355_FLAG_ADAPT_VELOCITY) ? 1 : 0;
   for (int a = 0; a < 3; ++a) b.force[a] = (a < D) ? d->force[a] : 0.0;
   b.eq = d->equilibrium;
   b.mirror_axis = 0;
@@ -655,7 +678,7 @@ int lbm355_reduce_conserved(const lbm355_sweep_desc* d, const void* pdf, double*
 
 const char* lbm355_last_error(void) { return g_err.c_str(); }
 
-int32_t lbm355_version(void) { return 3; }
+int32_t lbm355_version(void) { return 4; }
 
 #ifndef LBM355_SOURCE_HASH
 #define LBM355_SOURCE_HASH "unknown"

@@ -18,8 +18,9 @@ struct Collider;
 template <class S, class TC>
 struct Collider<S, LBM355_FAMILY_MOMENT, TC> {
   static constexpr bool deviation_form = false;
-  static __device__ __forceinline__ void run(TC (&f)[S::Q], const SweepArgs<TC>& a, TC& rho, TC (&u)[3]) {
-    collide_moment<S, TC>(f, a, rho, u);
+  static __device__ __forceinline__ void run(TC (&f)[S::Q], const SweepArgs<TC>& a, int64_t o, TC& rho, TC (&u)[3],
+                                             CellExtras<TC>&) {
+    collide_moment<S, TC>(f, a, o, rho, u);
   }
 };
 
@@ -29,8 +30,9 @@ constexpr int FAMILY_CUMULANT_LEARNED = 0x102;
 template <class TC>
 struct Collider<D3Q27, FAMILY_CUMULANT_LEARNED, TC> {
   static constexpr bool deviation_form = true;
-  static __device__ __forceinline__ void run(TC (&f)[27], const SweepArgs<TC>& a, TC& rho, TC (&u)[3]) {
-    collide_cumulant_d3q27<TC, true>(f, a, rho, u);
+  static __device__ __forceinline__ void run(TC (&f)[27], const SweepArgs<TC>& a, int64_t, TC& rho, TC (&u)[3],
+                                             CellExtras<TC>& ex) {
+    collide_cumulant_d3q27<TC, true>(f, a, rho, u, ex);
   }
 };
 
@@ -40,16 +42,18 @@ constexpr int FAMILY_CUMULANT_CORRECTED = 0x103;
 template <class TC>
 struct Collider<D3Q27, FAMILY_CUMULANT_CORRECTED, TC> {
   static constexpr bool deviation_form = true;
-  static __device__ __forceinline__ void run(TC (&f)[27], const SweepArgs<TC>& a, TC& rho, TC (&u)[3]) {
-    collide_cumulant_d3q27<TC, false, true>(f, a, rho, u);
+  static __device__ __forceinline__ void run(TC (&f)[27], const SweepArgs<TC>& a, int64_t, TC& rho, TC (&u)[3],
+                                             CellExtras<TC>& ex) {
+    collide_cumulant_d3q27<TC, false, true>(f, a, rho, u, ex);
   }
 };
 
 template <class TC>
 struct Collider<D3Q27, LBM355_FAMILY_CUMULANT, TC> {
   static constexpr bool deviation_form = true;
-  static __device__ __forceinline__ void run(TC (&f)[27], const SweepArgs<TC>& a, TC& rho, TC (&u)[3]) {
-    collide_cumulant_d3q27<TC>(f, a, rho, u);
+  static __device__ __forceinline__ void run(TC (&f)[27], const SweepArgs<TC>& a, int64_t, TC& rho, TC (&u)[3],
+                                             CellExtras<TC>& ex) {
+    collide_cumulant_d3q27<TC>(f, a, rho, u, ex);
   }
 };
 
@@ -81,13 +85,15 @@ __device__ __forceinline__ void store_nt(TS* p, TC v) {
 template <class S, class TC>
 struct Collider<S, LBM355_FAMILY_CENTRAL, TC> {
   static constexpr bool deviation_form = true;
-  static __device__ __forceinline__ void run(TC (&f)[S::Q], const SweepArgs<TC>& a, TC& rho, TC (&u)[3]) {
+  static __device__ __forceinline__ void run(TC (&f)[S::Q], const SweepArgs<TC>& a, int64_t, TC& rho, TC (&u)[3],
+                                             CellExtras<TC>&) {
     collide_central<S, TC>(f, a, rho, u);
   }
 };
 
-// Collision of one cell's populations plus the optional density / velocity outputs.  Shared by every
-// sweep kernel (single domain, row-LDS and batched), so all of them run the same arithmetic.
+// Collision of one cell's populations plus the optional per-cell outputs (density, velocity, the Smagorinsky
+// rate and eddy viscosity).  Shared by every sweep kernel (single domain, row-LDS and batched), so all of them
+// run the same arithmetic.
 template <class S, int FAM, class TC>
 __device__ __forceinline__ void collide_cell(TC (&f)[S::Q], const SweepArgs<TC>& a, int x, int y, int z) {
   constexpr int Q = S::Q, D = S::D;
@@ -96,20 +102,22 @@ __device__ __forceinline__ void collide_cell(TC (&f)[S::Q], const SweepArgs<TC>&
 #pragma unroll
     for (int i = 0; i < Q; ++i) f[i] -= TC(weight<S>(i));
   }
+  // offset of the cell in the optional per-cell fields
+  const int64_t o = (int64_t)(x + a.g) + (int64_t)(y + a.g) * a.osy + (D == 3 ? (int64_t)(z + a.g) * a.osz : 0);
   TC rho, u[3];
-  Collider<S, FAM, TC>::run(f, a, rho, u);
+  CellExtras<TC> ex = {a.omega_shear, TC(0)};
+  Collider<S, FAM, TC>::run(f, a, o, rho, u, ex);
   if (Collider<S, FAM, TC>::deviation_form && absolute) {
 #pragma unroll
     for (int i = 0; i < Q; ++i) f[i] += TC(weight<S>(i));
   }
-  if (a.rho_out || a.u_out) {
-    const int64_t o = (int64_t)(x + a.g) + (int64_t)(y + a.g) * a.osy + (D == 3 ? (int64_t)(z + a.g) * a.osz : 0);
-    if (a.rho_out) a.rho_out[o] = rho;
-    if (a.u_out) {
+  if (a.rho_out) a.rho_out[o] = rho;
+  if (a.u_out) {
 #pragma unroll
-      for (int d = 0; d < D; ++d) a.u_out[o + d * a.osc] = u[d];
-    }
+    for (int d = 0; d < D; ++d) a.u_out[o + d * a.osc] = u[d];
   }
+  if (a.omega_out) a.omega_out[o] = ex.omega_sgs;
+  if (a.nut_out) a.nut_out[o] = a.nut_scale * ex.omega_sgs * ex.pi;
 }
 
 // One cell of the sweep with direct (non-LDS) addressing: load with streaming, collide, store.
@@ -244,6 +252,11 @@ inline SweepArgs<TC> make_args(const lbm355_sweep_desc* d, const void* src, void
   a.smag_cs2x18 = (TC)(18.0 * d->smagorinsky_constant * d->smagorinsky_constant);
   a.omega_shear = (TC)d->omega_shear;
   a.closure = d->closure;
+  a.omega_out = (TC*)d->omega_out;
+  a.nut_out = (TC*)d->eddy_viscosity_out;
+  a.rho_in = (const TC*)d->density_in;
+  a.u_in = (const TC*)d->velocity_in;
+  a.nut_scale = (TC)(1.5 * d->smagorinsky_constant * d->smagorinsky_constant);
   return a;
 }
 

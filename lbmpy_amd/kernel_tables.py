@@ -99,8 +99,11 @@ def _split_equilibrium(feq_i, compressible):
     return A, B, lin, quad_d, quad_o
 
 
-def moment_table(method, bindings, dtype=np.float64):
-    """Flat table for the moment-family kernel (layout MomentTab in collide_moment.h)."""
+def moment_table(method, bindings, dtype=np.float64, inputs=False):
+    """Flat table for the moment-family kernel (layout MomentTab in collide_moment.h).  With ``inputs`` the
+    equilibrium takes density and velocity from input fields, so m_eq - m of a conserved moment is no longer a
+    known constant: conserved moments whose rate differs from their parity's base rate become active moments
+    instead of being folded into G0."""
     st = method.stencil
     Q, D = st.Q, st.D
     feq, M, Minv = _storage_equilibrium(method)
@@ -170,7 +173,7 @@ def moment_table(method, bindings, dtype=np.float64):
         dr = rates[k] - rb
         if dr == 0.0:
             continue
-        if conserved[k]:
+        if conserved[k] and not inputs:
             # m_eq - m for a conserved moment: 0 for the density part; F/2 for momentum when the
             # equilibrium velocity is shifted by F/(2 rho_ref)
             lin_coef = np.zeros(3)
@@ -416,12 +419,15 @@ def fixed_density_table(method):
     return tab
 
 
-def method_table(method, bindings, dtype=np.float64, corrections=None):
+def method_table(method, bindings, dtype=np.float64, corrections=None, inputs=False):
+    """``inputs``: the sweep reads density / velocity from input fields (moment family only)."""
     fam = family_of(method)
     if corrections is not None and fam != FAMILY_CUMULANT:
         raise ValueError("Galilean / fourth-order corrections are only defined for D3Q27 cumulant methods.")
+    if inputs and fam != FAMILY_MOMENT:
+        raise UnsupportedMethod("density / velocity inputs are implemented for the moment family")
     if fam == FAMILY_MOMENT:
-        return moment_table(method, bindings, dtype)
+        return moment_table(method, bindings, dtype, inputs=inputs)
     if fam == FAMILY_CUMULANT:
         return cumulant_table(method, bindings, dtype, corrections)
     return central_table(method, bindings, dtype)

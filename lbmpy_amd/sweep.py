@@ -85,12 +85,19 @@ class SweepKernel:
             raise NotImplementedError("the GPU sweep uses the fzyx (SoA) layout")
         if m.stencil.name not in STENCIL_CODES or not m.stencil.is_gpu_layout():
             raise NotImplementedError(f"stencil {m.stencil.name} ({m.stencil.ordering}) has no GPU sweep")
-        if lc.velocity_input is not None or lc.density_input is not None:
-            raise NotImplementedError("density/velocity input fields are not supported by the GPU sweep")
         self.method = m
         self.lbm_config = lc
         self.config = cfg
         self.family = family_of(m)
+        if lc.velocity_input is not None or lc.density_input is not None:
+            if lc.velocity_input is None:
+                raise ValueError("When setting 'density_input' parameter, 'velocity_input' has to be specified "
+                                 "as well.")
+            if self.family != nat.FAMILY_MOMENT:
+                raise NotImplementedError("density/velocity input fields are implemented for the SRT/TRT/MRT sweep")
+        unknown = set(lc.output) - {"density", "velocity"}
+        if unknown:
+            raise NotImplementedError(f"the GPU sweep writes the outputs 'density' and 'velocity', not {sorted(unknown)}")
         self.store = nat.DTYPE_CODES[cfg.data_type]
         self.compute = nat.DTYPE_CODES[cfg.compute_type]
         if self.store == nat.F16 and self.compute != nat.F32:
@@ -113,6 +120,9 @@ class SweepKernel:
             if model != SubgridScaleModel.SMAGORINSKY:
                 raise NotImplementedError(f"subgrid model {model} is not implemented on the GPU path")
             self.smagorinsky = float(const) if const is not None else 0.12
+        if (lc.omega_output_field is not None or lc.eddy_viscosity_field is not None) and self.smagorinsky is None:
+            raise NotImplementedError("omega_output_field / eddy_viscosity_field are written by the Smagorinsky "
+                                      "model on the GPU path; set subgrid_scale_model")
         self.closure = None
         self._shear_rate = None          # (bindings key, bound shear rate): the lookup is symbolic and slow
         self._tables = {}
@@ -127,7 +137,7 @@ class SweepKernel:
         # validate the method once with dummy bindings so unsupported configurations fail here
         # (rates of 1.7 / 1.1 keep the fourth-order parametrisation regular)
         probe = {s: (1.7 if k == 0 else 1.1) for k, s in enumerate(self.free_symbols)}
-        method_table(m, probe, corrections=self.corrections)
+        method_table(m, probe, corrections=self.corrections, inputs=lc.velocity_input is not None)
         nat.load()
 
     # ------------------------------------------------------------------ configuration
@@ -153,14 +163,23 @@ class SweepKernel:
         self.faces = faces
         return self
 
+    #: method tables kept per kernel (one per binding of the free rate symbols and device); the least recently
+    #: used one is dropped beyond this, so a loop over time-varying rates holds a bounded amount of GPU memory
+    TABLE_CACHE_SIZE = 8
+
     def table(self, bindings, device):
         key = (tuple(sorted(bindings.items())), str(device))
-        t = self._tables.get(key)
+        t = self._tables.pop(key, None)
         if t is None:
-            arr, nact = method_table(self.method, bindings, corrections=self.corrections)
+            arr, nact = method_table(self.method, bindings, corrections=self.corrections,
+                                     inputs=self.lbm_config.velocity_input is not None)
             dt = torch.float64 if self.compute == nat.F64 else torch.float32
             t = (torch.tensor(arr, dtype=dt, device=device), nact)
-            self._tables[key] = t
+            while len(self._tables) >= self.TABLE_CACHE_SIZE:
+                # a launch queued with an older table may still read it: the stream orders the caching
+                # allocator's reuse of the freed block after that launch
+                self._tables.pop(next(iter(self._tables)))
+        self._tables[key] = t
         return t
 
     def shear_rate(self, bindings):
@@ -243,37 +262,76 @@ class SweepKernel:
             if s not in kwargs:
                 raise ValueError(f"missing value for free symbol '{s}'")
             bindings[s] = float(kwargs.pop(s))
-        outputs = {}
-        for q, target in lc.output.items():
-            arr = kwargs.pop(target, None) if isinstance(target, str) else target
-            if arr is not None:
-                outputs[q] = arr
+
+        def field(target):
+            # a tensor given in the config, or the name under which the call passes it (a pystencils field's name)
+            if target is None:
+                return None
+            if isinstance(target, str):
+                return kwargs.pop(target, None)
+            name = getattr(target, "name", None)
+            if not isinstance(target, torch.Tensor) and isinstance(name, str):
+                return kwargs.pop(name, None)
+            return target
+        fields = {"density": field(lc.output.get("density")), "velocity": field(lc.output.get("velocity")),
+                  "omega": field(lc.omega_output_field), "eddy_viscosity": field(lc.eddy_viscosity_field),
+                  "density_input": field(lc.density_input), "velocity_input": field(lc.velocity_input)}
         self._check_tensor(src)
         self._check_tensor(dst)
         if self.two_arrays and src.shape != dst.shape:
             raise ValueError("src and dst must have the same shape")
         if self.two_arrays and src.stride() != dst.stride():
             raise ValueError("src and dst must have the same strides")
-        rho_o = outputs.get("density")
-        u_o = outputs.get("velocity")
-        out_strides = (0, 0, 0)
-        if rho_o is not None or u_o is not None:
-            ref = rho_o if rho_o is not None else u_o
-            D = self.method.dim
-            ct = torch.float64 if self.compute == nat.F64 else torch.float32
-            for o in (rho_o, u_o):
-                if o is not None and (o.dtype != ct or o.stride(0) != 1 or o.device != src.device):
-                    raise ValueError(f"output fields must be {ct} fzyx tensors on {src.device}")
-            if rho_o is not None and u_o is not None and rho_o.stride()[:D] != u_o.stride()[:D]:
-                raise ValueError("density and velocity outputs must share the spatial strides")
-            out_strides = (ref.stride(1), ref.stride(2) if D == 3 else 0, u_o.stride(D) if u_o is not None else 0)
+        if (lc.velocity_input is not None) != (fields["velocity_input"] is not None) or \
+                (lc.density_input is not None) != (fields["density_input"] is not None):
+            raise ValueError("the density / velocity input fields are missing from the call")
+        out_strides = self._cell_field_strides(src, fields)
         d = self.descriptor(src, bindings, out_strides)
+        ptr = (lambda t: t.data_ptr() if t is not None else None)
+        d.omega_out = ptr(fields["omega"])
+        d.eddy_viscosity_out = ptr(fields["eddy_viscosity"])
+        d.density_in = ptr(fields["density_input"])
+        d.velocity_in = ptr(fields["velocity_input"])
         lib = nat.load()
         nat.check(lib.lbm355_sweep(ctypes.byref(d), ctypes.c_void_p(src.data_ptr()),
-                                   ctypes.c_void_p(dst.data_ptr()),
-                                   ctypes.c_void_p(rho_o.data_ptr() if rho_o is not None else 0),
-                                   ctypes.c_void_p(u_o.data_ptr() if u_o is not None else 0),
-                                   nat.current_stream_handle()))
+                                   ctypes.c_void_p(dst.data_ptr()), ctypes.c_void_p(ptr(fields["density"])),
+                                   ctypes.c_void_p(ptr(fields["velocity"])), nat.current_stream_handle()))
+
+    def _cell_field_strides(self, src, fields):
+        """(stride_y, stride_z, component stride) shared by the per-cell fields beside the PDFs, after checking
+        that each is a compute-type tensor on the PDFs' device, indexed [x, y, (z,) (component)] over the PDF
+        array's spatial extent with x contiguous: the kernel writes and reads every cell of the sweep there."""
+        D = self.method.dim
+        ct = torch.float64 if self.compute == nat.F64 else torch.float32
+        given = {k: t for k, t in fields.items() if t is not None}
+        if not given:
+            return (0, 0, 0)
+        spatial = tuple(src.shape[:D])
+        strides = None
+        comp = None
+        for name, t in given.items():
+            vector = name in ("velocity", "velocity_input")
+            want = spatial + ((D,) if vector else ())
+            if not isinstance(t, torch.Tensor) or t.device != src.device or t.dtype != ct:
+                raise ValueError(f"the {name} field must be a {ct} tensor on {src.device}")
+            if tuple(t.shape) != want:
+                raise ValueError(f"the {name} field must have shape {want} (the PDF array's spatial shape"
+                                 f"{' and D components' if vector else ''}), not {tuple(t.shape)}")
+            if t.stride(0) != 1:
+                raise ValueError(f"the {name} field must be contiguous in x (fzyx layout)")
+            if strides is None:
+                strides = t.stride()[:D]
+            elif t.stride()[:D] != strides:
+                raise ValueError("the per-cell fields (outputs and inputs) must share their spatial strides")
+            if vector:
+                if comp is None:
+                    comp = t.stride(D)
+                elif t.stride(D) != comp:
+                    raise ValueError("the velocity output and input must share their component stride")
+        volume = max((spatial[a] - 1) * strides[a] for a in range(D)) + 1
+        if comp is not None and comp < volume:
+            raise ValueError("velocity components must not overlap (fzyx layout: component slowest)")
+        return (strides[1], strides[2] if D == 3 else 0, comp if comp is not None else 0)
 
     def conserved(self, pdf, **bindings):
         """FP64 sums over the interior on the GPU: (mass, momentum_x, momentum_y, momentum_z) of the stored

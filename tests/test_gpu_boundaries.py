@@ -302,32 +302,57 @@ def test_simple_extrapolation_outflow_channel(pattern):
         bad.run(1)
 
 
+@pytest.mark.parametrize("compressible", [True, False], ids=["comp", "incomp"])
 @pytest.mark.parametrize("pattern", ["pull", "aa"])
-def test_ubb_adapt_velocity_to_force(pattern):
-    """UBB(adapt_velocity_to_force=True) (boundaryconditions.py:808-812) drives the wall at u_w - F/2, the
-    equilibrium input velocity at density 1: the same run as a UBB at that velocity, for a constant wall velocity
-    and for a callback, and without a force it changes nothing."""
+def test_ubb_adapt_velocity_to_force(pattern, compressible):
+    """UBB(adapt_velocity_to_force=True) (boundaryconditions.py:808-812) drives the wall at the equilibrium input
+    velocity u_w - F / (2 rho_ref), rho_ref the link's fluid density for compressible methods and 1 otherwise: the
+    GPU run matches the oracle with that wall to 1e-12, for a constant wall velocity and for a callback.  For an
+    incompressible method this is the UBB at u_w - F / 2 exactly; without a force the option changes nothing."""
+    from oracle.lbm_oracle import OracleDomain, equilibrium_pdfs
     st = LBStencil(Stencil.D2Q9)
     F = (2e-4, -1e-4)
+    shape = (16, 10)
     runs = {}
+    lc = LBMConfig(stencil=st, method=Method.TRT, relaxation_rates=[1.7, 1.2], compressible=compressible, force=F,
+                   streaming_pattern=pattern)
     for key, ubb in [("adapt", UBB((0.03, 0.0), adapt_velocity_to_force=True)),
                      ("shifted", UBB((0.03 - F[0] / 2, -F[1] / 2))),
                      ("callback", UBB(lambda x, y, d: np.tile([0.03, 0.0], (len(d), 1)), dim=2,
                                       adapt_velocity_to_force=True))]:
-        lc = LBMConfig(stencil=st, method=Method.TRT, relaxation_rates=[1.7, 1.2], compressible=True, force=F,
-                       streaming_pattern=pattern)
-        step = _make((16, 10), (True, False), lc)
+        step = _make(shape, (True, False), lc)
         step.boundary_handling.set_boundary(ubb, slice_from_direction("N", 2))
         step.boundary_handling.set_boundary(NoSlip(), slice_from_direction("S", 2))
+        start = step.pdf_array.cpu().numpy()
         step.run(20)
         runs[key] = step.pdf_array.cpu().numpy()
-    np.testing.assert_array_equal(runs["adapt"], runs["shifted"])
-    np.testing.assert_array_equal(runs["callback"], runs["shifted"])
-    lc = LBMConfig(stencil=st, method=Method.TRT, relaxation_rates=[1.7, 1.2], compressible=True,
+    m = step.method
+    top = np.zeros((shape[0] + 2, shape[1] + 2), dtype=bool)
+    top[:, -1] = True
+    bottom = np.zeros_like(top)
+    bottom[:, 0] = True
+    dom = OracleDomain(m, shape, faces={0: "periodic", 1: "ghost"}, pattern=pattern,
+                       boundaries=[(top, {"kind": "ubb", "velocity": (0.03, 0.0), "density": None, "adapt": True}),
+                                   (bottom, {"kind": "noslip"})])
+    f0 = equilibrium_pdfs(m, np.ones((10, 16)), np.zeros((2, 10, 16)))
+    dom.set_pdfs_interior(f0[dom.inv] if pattern == "aa" else f0)
+    for _ in range(20):
+        dom.step()
+    got = np.transpose(runs["adapt"][1:-1, 1:-1], (2, 1, 0))
+    if pattern == "aa":
+        got = got[dom.inv]
+    assert np.abs(got - dom.pdfs_in_natural_layout()).max() <= 1e-12
+    np.testing.assert_array_equal(runs["callback"], runs["adapt"])
+    if compressible:
+        assert np.abs(runs["adapt"] - runs["shifted"]).max() > 1e-12       # the local density enters the shift
+    else:
+        np.testing.assert_array_equal(runs["adapt"], runs["shifted"])
+    assert np.abs(runs["adapt"] - start).max() > 1e-4
+    lc = LBMConfig(stencil=st, method=Method.TRT, relaxation_rates=[1.7, 1.2], compressible=compressible,
                    streaming_pattern=pattern)
     outs = []
     for adapt in (True, False):
-        step = _make((16, 10), (True, False), lc)
+        step = _make(shape, (True, False), lc)
         step.boundary_handling.set_boundary(UBB((0.03, 0.0), adapt_velocity_to_force=adapt),
                                             slice_from_direction("N", 2))
         step.boundary_handling.set_boundary(NoSlip(), slice_from_direction("S", 2))

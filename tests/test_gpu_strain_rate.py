@@ -14,9 +14,8 @@ import torch
 
 from lbmpy_amd import LBMConfig, LBStencil, Stencil, Method, ForceModel, create_lb_method, Timestep
 from lbmpy_amd.macroscopic_value_kernels import strain_rate_tensor_getter, macroscopic_values_setter
-from lbmpy_amd.relaxationrates import get_shear_relaxation_rate
 from lbmpy_amd.sweep import create_pdf_array
-from oracle.lbm_oracle import OracleCollision, equilibrium_pdfs
+from oracle.lbm_oracle import strain_rate_tensor
 from tests.lbm_helpers import seeded_state, storage_view
 from tests.test_gpu_streaming_patterns import _place
 
@@ -32,21 +31,6 @@ CASES = [
     ("D3Q27", dict(method=Method.CUMULANT, relaxation_rate=1.9, compressible=True, force=(1e-4, 0, 0)), (4, 4, 3)),
 ]
 KERNELS = [("pull", 0), ("aa", 0), ("aa", 1), ("esotwist", 1)]
-
-
-def _oracle_strain(m, f):
-    """S [D*D, cells] from natural populations f (Q, cells)."""
-    coll = OracleCollision(m, {s: 1.0 for s in m.relaxation_rate_symbols()})
-    rho, _, u_eq, _ = coll.macroscopic(f)
-    ref = rho if m.compressible else np.ones_like(rho)
-    u_in = u_eq + (coll.F[:, None] / (2.0 * ref[None, :]) if m.force_model is not None else 0.0)
-    feq = equilibrium_pdfs(m, rho, u_in)
-    c = np.array(m.stencil.stencil_entries, dtype=np.float64)
-    fneq = f - feq
-    D = m.dim
-    pi = np.stack([(c[:, a] * c[:, b]) @ fneq for a in range(D) for b in range(D)])
-    omega = float(get_shear_relaxation_rate(m))
-    return -1.5 * omega / ref * pi
 
 
 @pytest.mark.parametrize("kernel", KERNELS, ids=lambda k: f"{k[0]}-{k[1]}")
@@ -67,7 +51,7 @@ def test_strain_rate_matches_oracle(case, kernel):
     inner = tuple(slice(1, -1) for _ in range(D))
     got = out[inner].cpu().numpy()                                     # [x, y, (z), D*D]
     got = np.moveaxis(got, -1, 0).reshape(D * D, -1, order="F")        # cells in x-fastest order
-    want = _oracle_strain(m, f0.reshape(Q, -1))
+    want = strain_rate_tensor(m, f0.reshape(Q, -1))
     assert np.abs(got - want).max() <= 1e-12
     assert np.abs(want).max() > 1e-5
     S = got.reshape(D, D, -1)

@@ -49,8 +49,8 @@ def test_every_declared_symbol_is_exported(lib):
 
 def test_descriptor_layout_matches(lib):
     from lbmpy_amd import _native
-    assert lib.lbm355_desc_size() == ctypes.sizeof(_native.SweepDesc) == 160
-    assert lib.lbm355_version() >= 1
+    assert lib.lbm355_desc_size() == ctypes.sizeof(_native.SweepDesc) == 192
+    assert lib.lbm355_version() >= 4
 
 
 def test_header_constants_match_binding():
@@ -151,6 +151,31 @@ def test_sweep_rejects_bad_correction_combinations(lib):
     assert rc != 0
     rc, msg = _sweep_error(lib, _desc(pattern="ESOPUSH_EVEN", flags=n.FLAG_LEARNED_CLOSURE))
     assert rc != 0 and "pull or AA" in msg
+
+
+def test_sweep_rejects_unusable_cell_fields(lib):
+    from lbmpy_amd import _native as n
+    d = _desc()
+    d.omega_out = 0x40000
+    rc, msg = _sweep_error(lib, d)
+    assert rc != 0 and "Smagorinsky" in msg
+    d = _desc(stencil="D3Q19", family="moment")
+    d.density_in = 0x40000
+    rc, msg = _sweep_error(lib, d)
+    assert rc != 0 and "velocity input" in msg
+    d = _desc()
+    d.velocity_in = 0x40000
+    rc, msg = _sweep_error(lib, d)
+    assert rc != 0 and "moment family" in msg
+    d = _desc(stencil="D3Q19", family="moment")
+    d.velocity_in = 0x40000
+    d.out_stride_y, d.out_stride_z, d.out_stride_c = 66, 66 * 10, 66 * 10 * 10
+    d.out_stride_y = 65                                          # a row of 64 cells and 2 ghost cells needs 66
+    rc, msg = _sweep_error(lib, d)
+    assert rc != 0 and "out_stride_y" in msg
+    d.out_stride_y, d.out_stride_c = 66, 66 * 10 * 10 - 1
+    rc, msg = _sweep_error(lib, d)
+    assert rc != 0 and "out_stride_c" in msg
 
 
 def test_boundary_links_reject_bad_patterns(lib):
