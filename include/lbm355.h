@@ -82,6 +82,10 @@ extern "C" {
 #define LBM355_FLAG_GALILEAN 64     /* D3Q27 cumulant: Galilean correction (galilean_correction.py:21-91) */
 #define LBM355_FLAG_FOURTH_ORDER 128 /* D3Q27 cumulant: fourth-order correction with limited third-order rates
                                         (fourth_order_correction.py:21-237); parameters in the table */
+#define LBM355_FLAG_STOCHASTIC_ROUNDING 512 /* FP16 storage: each stored population is rounded to one of its two
+                                                neighbouring FP16 values with probabilities that make the rounding
+                                                unbiased, from a hash of (cell, direction, rounding_seed); no lbmpy
+                                                counterpart (lbmpy has no FP16 storage) */
 #define LBM355_FLAG_ADAPT_VELOCITY 256 /* boundary links, UBB: drive the wall at its equilibrium input velocity
                                           u_w - F / (2 rho_ref), rho_ref the link's density (the UBB's own, or
                                           the fluid cell's for compressible methods) or 1 for incompressible ones
@@ -122,6 +126,8 @@ typedef struct lbm355_sweep_desc {
                                creationfunctions.py:662-673); needs velocity_in */
   const void* velocity_in;  /* moment family: equilibrium velocity read instead of computed (lbmpy velocity_input,
                                creationfunctions.py:675-677); D components */
+  uint32_t rounding_seed;   /* LBM355_FLAG_STOCHASTIC_ROUNDING: seed of this launch's rounding (a new one per launch) */
+  int32_t reserved;         /* 0 */
 } lbm355_sweep_desc;
 
 /* One stream-collide time step over the interior.

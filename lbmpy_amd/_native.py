@@ -24,6 +24,7 @@ F64, F32, F16 = 0, 1, 2
 FACE_GHOST, FACE_PERIODIC, FACE_NOSLIP = 0, 1, 2
 FLAG_ZERO_CENTERED, FLAG_COMPRESSIBLE, FLAG_EQ_SHIFT, FLAG_FORCE, FLAG_SMAGORINSKY = 1, 2, 4, 8, 16
 FLAG_LEARNED_CLOSURE, FLAG_GALILEAN, FLAG_FOURTH_ORDER, FLAG_ADAPT_VELOCITY = 32, 64, 128, 256
+FLAG_STOCHASTIC_ROUNDING = 512
 
 DTYPE_CODES = {"float64": F64, "float32": F32, "float16": F16}
 STENCIL_NAMES = {"D2Q9": D2Q9, "D3Q19": D3Q19, "D3Q27": D3Q27}
@@ -53,7 +54,7 @@ class SweepDesc(ctypes.Structure):
                 ("force", c_double * 3), ("smagorinsky_constant", c_double), ("omega_shear", c_double),
                 ("tables", c_void_p), ("closure", c_void_p),
                 ("omega_out", c_void_p), ("eddy_viscosity_out", c_void_p), ("density_in", c_void_p),
-                ("velocity_in", c_void_p)]
+                ("velocity_in", c_void_p), ("rounding_seed", ctypes.c_uint32), ("reserved", c_int32)]
 
 
 EXPORTED = ("lbm355_sweep", "lbm355_table_elements", "lbm355_noslip_links", "lbm355_reduce_conserved",

@@ -82,7 +82,12 @@ class CreateKernelConfig:
     ``compute_type`` (an extension, default = ``data_type`` widened to at
     least float32) is the arithmetic type.  ``ghost_layers`` defaults to 1 as
     in lbmpy (``creationfunctions.py:576``); 0 is accepted for periodic or
-    wall-bounded domains that need no ghost cells."""
+    wall-bounded domains that need no ghost cells.
+
+    ``stochastic_rounding`` (an extension, FP16 storage only) rounds every stored population to one of its two
+    neighbouring FP16 values with probabilities that make the rounding unbiased, from a counter-based hash of
+    (cell, direction, launch) seeded by ``rounding_seed``: runs are reproducible for a seed, and increments
+    smaller than half an FP16 ulp (a weak body force) survive on average instead of being rounded away."""
     target: Target = Target.CPU
     data_type: str = "float64"
     compute_type: str = None
@@ -91,6 +96,8 @@ class CreateKernelConfig:
     gpu_indexing_params: Any = None
     backend: Any = None
     cpu_vectorize_info: Any = None
+    stochastic_rounding: bool = False
+    rounding_seed: int = 0
 
     def __post_init__(self):
         if isinstance(self.data_type, dict):
@@ -99,6 +106,8 @@ class CreateKernelConfig:
         if self.compute_type is None:
             self.compute_type = "float64" if self.data_type == "float64" else "float32"
         self.compute_type = _normalise_dtype(self.compute_type)
+        if self.stochastic_rounding and self.data_type != "float16":
+            raise ValueError("stochastic rounding applies to FP16 storage (data_type='float16')")
 
 
 def _normalise_dtype(dt):
@@ -242,7 +251,7 @@ class LBMConfig:
         elif isinstance(self.force_model, type) and issubclass(self.force_model, forcemodels.AbstractForceModel):
             self.force_model = self.force_model(self.force[:self.stencil.D])
 
-        if self.density_input or self.velocity_input:
+        if self.density_input is not None or self.velocity_input is not None:
             self.conserved_moments = False
 
 

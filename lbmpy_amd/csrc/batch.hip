@@ -62,6 +62,7 @@ __device__ __forceinline__ SweepArgs<TC> load_args(const CONST_AS SweepArgs<TC>*
   a.rho_in = nullptr;
   a.u_in = nullptr;
   a.nut_scale = TC(0);
+  a.seed = 0;
   return a;
 }
 
@@ -235,6 +236,8 @@ int lbm355_batch_create(const lbm355_sweep_desc* descs, const void* const* pdf_s
     if (table_pattern(d.pattern)) return berr(LBM355_ERR_UNSUPPORTED, "batched ensembles stream with pull or AA");
     if (d.flags & (LBM355_FLAG_GALILEAN | LBM355_FLAG_FOURTH_ORDER))
       return berr(LBM355_ERR_UNSUPPORTED, "the cumulant corrections are not batched");
+    if (d.flags & LBM355_FLAG_STOCHASTIC_ROUNDING)
+      return berr(LBM355_ERR_UNSUPPORTED, "stochastic rounding is not batched");
     if (d.omega_out || d.eddy_viscosity_out || d.density_in || d.velocity_in)
       return berr(LBM355_ERR_UNSUPPORTED, "per-cell subgrid outputs and density / velocity inputs are not batched");
     if (!pdf_src[k] || !pdf_dst[k]) return berr(LBM355_ERR_ARG, "NULL member PDF pointer");

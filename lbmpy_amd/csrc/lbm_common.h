@@ -187,6 +187,7 @@ struct SweepArgs {
   const TC* rho_in;           // moment family: density input
   const TC* u_in;             // moment family: equilibrium-velocity input
   TC nut_scale;               // 3/2 Cs^2
+  uint32_t seed;              // stochastic FP16 rounding: this launch's seed
 };
 
 // What a collision hands back to the cell beyond the populations: the Smagorinsky shear rate and Pi

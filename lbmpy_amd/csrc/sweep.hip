@@ -562,6 +562,8 @@ int lbm355_sweep(const lbm355_sweep_desc* d, const void* pdf_src, void* pdf_dst,
     if (!d->closure) return fail(LBM355_ERR_ARG, "closure parameter pointer is NULL");
     if (d->nx % 64) return fail(LBM355_ERR_UNSUPPORTED, "the learned closure needs nx % 64 == 0 (whole wavefronts)");
   }
+  if ((d->flags & LBM355_FLAG_STOCHASTIC_ROUNDING) && (d->store_dtype != LBM355_F16 || d->compute_dtype != LBM355_F32))
+    return fail(LBM355_ERR_ARG, "stochastic rounding is for FP16 storage with FP32 arithmetic");
   if ((d->omega_out || d->eddy_viscosity_out) && !(d->flags & LBM355_FLAG_SMAGORINSKY))
     return fail(LBM355_ERR_ARG, "the shear-rate and eddy-viscosity outputs belong to the Smagorinsky model");
   if (d->density_in && !d->velocity_in)
@@ -678,7 +680,7 @@ int lbm355_reduce_conserved(const lbm355_sweep_desc* d, const void* pdf, double*
 
 const char* lbm355_last_error(void) { return g_err.c_str(); }
 
-int32_t lbm355_version(void) { return 4; }
+int32_t lbm355_version(void) { return 5; }
 
 #ifndef LBM355_SOURCE_HASH
 #define LBM355_SOURCE_HASH "unknown"

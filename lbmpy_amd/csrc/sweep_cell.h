@@ -91,10 +91,43 @@ struct Collider<S, LBM355_FAMILY_CENTRAL, TC> {
   }
 };
 
+// 32-bit integer hash (xorshift-multiply): mixes a cell index with a launch seed
+__device__ __forceinline__ uint32_t mix32(uint32_t v) {
+  v ^= v >> 16;
+  v *= 0x7FEB352Du;
+  v ^= v >> 15;
+  v *= 0x846CA68Bu;
+  v ^= v >> 16;
+  return v;
+}
+
+// Stochastic rounding of a cell's populations to FP16 values (kept in TC, so that the store's conversion is
+// exact): each value goes to one of its two neighbouring FP16 numbers with probability proportional to its
+// closeness, gfx950's v_cvt_sr_f16_f32, which compares the high bits of its random operand with the dropped
+// mantissa bits (measured: a value 1/8, 2/8, ... 7/8 of an FP16 ulp above an FP16 number rounds up with those
+// frequencies when bits 19..31 of the operand are uniform, and never when only bits 0..12 vary).  The operand of
+// direction i is the low 24 bits of the cell's hash times an odd 24-bit constant of i (one full-rate v_mul_u32_u24),
+// whose high bits are well mixed.
+template <class S, class TC>
+__device__ __forceinline__ void round_stochastic_f16(TC (&f)[S::Q], uint32_t seed, int x, int y, int z, int nx,
+                                                     int ny) {
+  typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+  const uint32_t cell = (uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)z);
+  const uint32_t h = mix32(cell * 0x9E3779B9u + seed);
+#pragma unroll
+  for (int i = 0; i < S::Q; ++i) {
+    const uint32_t k = (0x5BD1E9u + 0x3C6EF2u * (uint32_t)i) & 0xFFFFFFu;   // odd (bit 0 of both terms: 1, 0)
+    const uint32_t r = __umul24(h, k);
+    const half2_t zero = {0, 0};
+    const half2_t v = __builtin_amdgcn_cvt_sr_f16_f32(zero, (float)f[i], r, false);
+    f[i] = (TC)(float)v[0];
+  }
+}
+
 // Collision of one cell's populations plus the optional per-cell outputs (density, velocity, the Smagorinsky
 // rate and eddy viscosity).  Shared by every sweep kernel (single domain, row-LDS and batched), so all of them
-// run the same arithmetic.
-template <class S, int FAM, class TC>
+// run the same arithmetic.  TS is the storage type: FP16 storage may round stochastically.
+template <class S, int FAM, class TC, class TS = TC>
 __device__ __forceinline__ void collide_cell(TC (&f)[S::Q], const SweepArgs<TC>& a, int x, int y, int z) {
   constexpr int Q = S::Q, D = S::D;
   const bool absolute = !(a.flags & LBM355_FLAG_ZERO_CENTERED);
@@ -118,6 +151,9 @@ __device__ __forceinline__ void collide_cell(TC (&f)[S::Q], const SweepArgs<TC>&
   }
   if (a.omega_out) a.omega_out[o] = ex.omega_sgs;
   if (a.nut_out) a.nut_out[o] = a.nut_scale * ex.omega_sgs * ex.pi;
+  if constexpr (std::is_same<TS, _Float16>::value) {
+    if (a.flags & LBM355_FLAG_STOCHASTIC_ROUNDING) round_stochastic_f16<S, TC>(f, a.seed, x, y, z, a.nx, a.ny);
+  }
 }
 
 // One cell of the sweep with direct (non-LDS) addressing: load with streaming, collide, store.
@@ -154,7 +190,7 @@ __device__ __forceinline__ void sweep_cell(const SweepArgs<TC>& a, int x, int y,
       const TS* p = src + t.rslot * sq + X[1 + t.r[0]] + Y[1 + t.r[1]] + Z[1 + t.r[2]];
       f[i] = t.r[0] == 0 ? load_nt<TS, TC>(p) : load_as<TS, TC>(p);
     }
-    collide_cell<S, FAM, TC>(f, a, x, y, z);
+    collide_cell<S, FAM, TC, TS>(f, a, x, y, z);
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       const Access& t = acc.a[i];
@@ -209,7 +245,7 @@ __device__ __forceinline__ void sweep_cell(const SweepArgs<TC>& a, int x, int y,
     load(std::false_type{});
   }
 
-  collide_cell<S, FAM, TC>(f, a, x, y, z);
+  collide_cell<S, FAM, TC, TS>(f, a, x, y, z);
 
   if (PAT == LBM355_PULL) {
 #pragma unroll
@@ -257,6 +293,7 @@ inline SweepArgs<TC> make_args(const lbm355_sweep_desc* d, const void* src, void
   a.rho_in = (const TC*)d->density_in;
   a.u_in = (const TC*)d->velocity_in;
   a.nut_scale = (TC)(1.5 * d->smagorinsky_constant * d->smagorinsky_constant);
+  a.seed = d->rounding_seed;
   return a;
 }
 

@@ -76,7 +76,7 @@ __global__ void __launch_bounds__(512) sweep_eso_rows_kernel(SweepArgs<TC> a) {
   }
   __syncthreads();
 
-  collide_cell<S, FAM, TC>(f, a, x, y, z);
+  collide_cell<S, FAM, TC, TS>(f, a, x, y, z);
 
   j = 0;
 #pragma unroll

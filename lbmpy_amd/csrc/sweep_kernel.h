@@ -128,7 +128,7 @@ __global__ void __launch_bounds__(ROWLDS ? 512 : 256) sweep_kernel(SweepArgs<TC>
       load(std::false_type{});
     }
 
-    collide_cell<S, FAM, TC>(f, a, x, y, z);
+    collide_cell<S, FAM, TC, TS>(f, a, x, y, z);
 
     if (PAT == LBM355_PULL) {
 #pragma unroll

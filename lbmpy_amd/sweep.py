@@ -104,6 +104,15 @@ class SweepKernel:
             raise ValueError("FP16 storage needs FP32 arithmetic")
         if self.compute not in (nat.F32, nat.F64):
             raise ValueError("compute type must be float32 or float64")
+        self.stochastic_rounding = bool(getattr(cfg, "stochastic_rounding", False))
+        if self.stochastic_rounding and self.store != nat.F16:
+            raise ValueError("stochastic rounding applies to FP16 storage")
+        self.rounding_seed = int(getattr(cfg, "rounding_seed", 0)) & 0xFFFFFFFF
+        #: launches so far; with stochastic rounding the random stream of a launch depends on (seed, this count,
+        #: the kernel's streaming pattern code), so a run restarted with the same count repeats itself bit for bit
+        self.rounding_launches = 0
+        if self.store == nat.F16 and not self.stochastic_rounding:
+            _warn_if_force_is_rounded_away(m)
         self.ghost_layers = int(cfg.ghost_layers)
         if lc.streaming_pattern == "pull":
             self.pattern = nat.PULL
@@ -287,6 +296,10 @@ class SweepKernel:
             raise ValueError("the density / velocity input fields are missing from the call")
         out_strides = self._cell_field_strides(src, fields)
         d = self.descriptor(src, bindings, out_strides)
+        if self.stochastic_rounding:
+            d.flags |= nat.FLAG_STOCHASTIC_ROUNDING
+            d.rounding_seed = launch_seed(self.rounding_seed, self.rounding_launches, self.pattern)
+            self.rounding_launches += 1
         ptr = (lambda t: t.data_ptr() if t is not None else None)
         d.omega_out = ptr(fields["omega"])
         d.eddy_viscosity_out = ptr(fields["eddy_viscosity"])
@@ -358,3 +371,52 @@ class SweepKernel:
 
     def __repr__(self):
         return f"SweepKernel({self.update_rule!r})"
+
+
+def _mix32(x):
+    """A 32-bit integer hash (xorshift-multiply), the same mixing the kernels use for their rounding bits."""
+    x &= 0xFFFFFFFF
+    x ^= x >> 16
+    x = (x * 0x7FEB352D) & 0xFFFFFFFF
+    x ^= x >> 15
+    x = (x * 0x846CA68B) & 0xFFFFFFFF
+    x ^= x >> 16
+    return x
+
+
+def launch_seed(seed, launch, pattern):
+    """Seed of one stochastically rounded launch: distinct per (seed, launch count, streaming-pattern code), so
+    the two kernels of an in-place pattern never share a random stream."""
+    return _mix32((seed * 0x9E3779B1 + launch * 0x85EBCA77 + pattern * 0xC2B2AE3D + 0x165667B1) & 0xFFFFFFFF)
+
+
+#: lattice velocity at which the FP16 ulp of the populations is judged against a force increment
+FP16_REFERENCE_VELOCITY = 0.01
+
+
+def force_rounded_away_in_fp16(method):
+    """True when round-to-nearest FP16 storage drops the body force: for every direction with c.F != 0 the per-step
+    force increment 3 w |c.F| is below half an FP16 ulp of a zero-centred population at lattice velocity 0.01
+    (3 w 0.01).  Such a force only survives with stochastic rounding (or FP32 storage)."""
+    fm = method.force_model
+    if fm is None:
+        return False
+    F = np.zeros(3)
+    F[:method.dim] = [float(v) for v in fm.force]
+    c = np.array([tuple(e) + (0,) * (3 - method.dim) for e in method.stencil], dtype=np.float64)
+    w = np.array([float(v) for v in method.weights])
+    incr = 3.0 * w * np.abs(c @ F)
+    sel = incr > 0
+    if not sel.any():
+        return False
+    dev = (3.0 * w[sel] * FP16_REFERENCE_VELOCITY).astype(np.float16)
+    half_ulp = 0.5 * np.spacing(dev).astype(np.float64)
+    return bool(np.all(incr[sel] < half_ulp))
+
+
+def _warn_if_force_is_rounded_away(method):
+    if force_rounded_away_in_fp16(method):
+        import warnings
+        warnings.warn("FP16 storage with round-to-nearest drops this body force: its per-step increment on every "
+                      "population is below half an FP16 ulp; use CreateKernelConfig(stochastic_rounding=True) or "
+                      "FP32 storage", RuntimeWarning, stacklevel=3)

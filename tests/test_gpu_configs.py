@@ -1,12 +1,15 @@
 """Two benchmark configurations at their real sizes, and the batched path against the oracle.
 
 * D3Q19 MRT (weighted-orthogonal) Poiseuille channel at 512 x 256 x 256: NoSlip walls in y, periodic x and z,
-  Guo force 1e-6, omega_shear 1.8 and every other rate 1, FP32 arithmetic on zero-centred PDFs stored in FP64
-  and FP32, pull streaming, 2000 steps (the configuration runs 20000; the start-up series is exact at any t).
-  FP32 storage equals FP64 storage bit for bit (every stored value is an FP32 number), the x-velocity profile
-  follows the analytic start-up of plane Poiseuille flow (analytical_solutions.py:5-20 for its steady limit),
-  and the FP64 mass sum drifts by less than 1e-7 per cell.  A small FP32-arithmetic copy of the channel is
-  compared with the FP64 oracle.
+  Guo force 1e-6, omega_shear 1.8 and every other rate 1, FP32 arithmetic on zero-centred PDFs stored in FP64,
+  FP32 and FP16, pull streaming, 2000 steps (the configuration runs 20000; the start-up series is exact at any
+  t).  FP32 storage equals FP64 storage bit for bit (every stored value is an FP32 number), the x-velocity
+  profile follows the analytic start-up of plane Poiseuille flow (analytical_solutions.py:5-20 for its steady
+  limit) to 4e-3 of its maximum (the lattice's own error near the walls at this early time is 2.7e-3), and the
+  FP64 mass sum drifts by less than 1e-7 per cell.  FP16 storage with round-to-nearest drops the force (its
+  per-step increment is below half an FP16 ulp; the sweep warns) and is 1.4e-2 off the FP32 run already; with
+  stochastic rounding it stays within 2e-3 of the FP32 run (7e-4 measured).  A small FP32-arithmetic copy of the
+  channel is compared with the FP64 oracle.
 * 256 D3Q19 TRT channels of seeded uneven sizes {32..96 step 8}^3 (numpy default_rng(2024)), per-member force
   log-uniform in [1e-7, 1e-5] and omega uniform in [1.5, 1.95], NoSlip y, FP32 pull, 1000 steps as one replay
   of a captured HIP graph of the autotuned batched sweep: every member equals its single-domain run bit for bit.
@@ -32,9 +35,9 @@ def poiseuille_startup(y, t, force, nu, h, terms=4000):
                 * np.exp(-(n ** 2) * np.pi ** 2 * nu * t / (h * h))).sum(axis=0)
 
 
-def _mrt_channel_kernel(data_type, compute_type="float32", ghost_layers=0, **extra):
+def _mrt_channel_kernel(data_type, compute_type="float32", ghost_layers=0, stochastic_rounding=False, **extra):
     cfg = CreateKernelConfig(target=Target.GPU, data_type=data_type, compute_type=compute_type,
-                             ghost_layers=ghost_layers)
+                             ghost_layers=ghost_layers, stochastic_rounding=stochastic_rounding)
     k = create_lb_function(lbm_config=LBMConfig(stencil=LBStencil(Stencil.D3Q19), method=Method.MRT,
                                                 relaxation_rate=1.8, force=(1e-6, 0.0, 0.0), zero_centered=True,
                                                 compressible=False, **extra), config=cfg)
@@ -56,10 +59,16 @@ def test_mrt_channel_512x256x256_storage_forms():
     nu = (1.0 / omega - 0.5) / 3.0
     y = np.arange(ny) + 0.5
     u_an = poiseuille_startup(y, steps - 0.5, force, nu, float(ny))
-    final = {}
-    for dt in ("float64", "float32"):
-        print(f"MRT channel 512x256x256 {dt} storage, {steps} steps", flush=True)
-        k = _mrt_channel_kernel(dt)
+    final, profiles = {}, {}
+    import warnings
+    for dt, sr in (("float64", False), ("float32", False), ("float16", False), ("float16", True)):
+        print(f"MRT channel 512x256x256 {dt} storage{' (stochastic rounding)' if sr else ''}, {steps} steps",
+              flush=True)
+        with warnings.catch_warnings(record=True) as caught:
+            warnings.simplefilter("always")
+            k = _mrt_channel_kernel(dt, stochastic_rounding=sr)
+        warned = any(issubclass(w.category, RuntimeWarning) for w in caught)
+        assert warned == (dt == "float16" and not sr)
         src = create_pdf_array((nx, ny, nz), 19, ghost_layers=0, dtype=dt)
         dst = create_pdf_array((nx, ny, nz), 19, ghost_layers=0, dtype=dt)
         mass0 = k.conserved(src)[0]
@@ -70,15 +79,26 @@ def test_mrt_channel_512x256x256_storage_forms():
         mass1 = k.conserved(src)[0]
         prof = _x_velocity_profile(src, force)
         err = np.abs(prof - u_an).max() / np.abs(u_an).max()
-        print(f"  profile error vs analytic {err:.3e}, mass drift per cell {(mass1 - mass0) / (nx * ny * nz):.3e}",
-              flush=True)
-        assert err <= 2e-3
-        assert abs(mass1 - mass0) / (nx * ny * nz) <= 1e-7
-        # the profile is the same in every x-z column of the periodic directions
-        assert np.abs(_x_velocity_profile(src[nx // 3:], force) - prof).max() <= 1e-9
-        final[dt] = src
+        drift = (mass1 - mass0) / (nx * ny * nz)
+        print(f"  profile error vs analytic {err:.3e}, mass drift per cell {drift:.3e}", flush=True)
+        key = dt + ("-sr" if sr else "")
+        profiles[key] = prof
+        if dt != "float16":
+            assert err <= 4e-3
+            assert abs(drift) <= 1e-7
+            # the profile is the same in every x-z column of the periodic directions
+            assert np.abs(_x_velocity_profile(src[nx // 3:], force) - prof).max() <= 1e-9
+            final[dt] = src
+        else:
+            assert abs(drift) <= 1e-5
         del dst
     assert torch.equal(final["float64"].float(), final["float32"])
+    umax = np.abs(u_an).max()
+    rne = np.abs(profiles["float16"] - profiles["float32"]).max() / umax
+    sr = np.abs(profiles["float16-sr"] - profiles["float32"]).max() / umax
+    print(f"FP16 storage vs FP32 storage: round-to-nearest {rne:.3e}, stochastic rounding {sr:.3e}", flush=True)
+    assert sr <= 2e-3
+    assert rne >= 5e-3
 
 
 def test_mrt_channel_fp32_arithmetic_against_fp64_oracle():

@@ -1,5 +1,6 @@
 """Reduced-precision storage on the D3Q19 MRT force-driven channel (NoSlip y, periodic x/z, Guo force 1e-6,
-shear rate 1.8, other rates 1, zero-centred PDFs, FP32 arithmetic), 64 x 48 x 4 cells, 3000 pull steps.
+shear rate 1.8, other rates 1, zero-centred PDFs, FP32 arithmetic), 64 x 48 x 4 cells, 3000 pull steps; FP16
+storage with round-to-nearest and with stochastic rounding.
 
 The FP64-storage run is compared with the analytic start-up solution of plane Poiseuille flow (the FP64
 oracle itself is 1.8e-3 from it on this grid, the lattice's discretisation error) and the FP32 and FP16
@@ -16,11 +17,12 @@ STEPS = 3000
 FORCE, OMEGA = 1e-6, 1.8
 
 
-def _run(dtype):
+def _run(dtype, stochastic_rounding=False):
     import torch
     from lbmpy_amd import LBMConfig, LBStencil, Stencil, Method, create_lb_function, CreateKernelConfig, Target
     from lbmpy_amd.sweep import create_pdf_array
-    cfg = CreateKernelConfig(target=Target.GPU, data_type=dtype, compute_type="float32", ghost_layers=0)
+    cfg = CreateKernelConfig(target=Target.GPU, data_type=dtype, compute_type="float32", ghost_layers=0,
+                             stochastic_rounding=stochastic_rounding)
     k = create_lb_function(lbm_config=LBMConfig(stencil=LBStencil(Stencil.D3Q19), method=Method.MRT,
                                                 relaxation_rate=OMEGA, force=(FORCE, 0.0, 0.0), zero_centered=True,
                                                 compressible=False), config=cfg)
@@ -41,10 +43,15 @@ def _run(dtype):
 
 @pytest.fixture(scope="module")
 def runs():
+    import warnings
     out = {}
-    for dt in ("float64", "float32", "float16"):
-        print(f"channel storage form {dt}", flush=True)
-        out[dt] = _run(dt)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", RuntimeWarning)           # FP16 round-to-nearest drops this force
+        for dt in ("float64", "float32", "float16"):
+            print(f"channel storage form {dt}", flush=True)
+            out[dt] = _run(dt)
+    print("channel storage form float16 with stochastic rounding", flush=True)
+    out["float16-sr"] = _run("float16", stochastic_rounding=True)
     return out
 
 
@@ -71,13 +78,19 @@ def test_fp32_storage_equals_fp64_storage(runs):
     assert abs(runs["float32"][1] - runs["float64"][1]) <= 1e-12
 
 
-@pytest.mark.parametrize("dtype,tol,drift_tol", [("float16", 2e-2, 1e-6)])
-def test_reduced_storage_vs_fp64_storage(runs, dtype, tol, drift_tol):
+def test_fp16_storage_forms_vs_fp64_storage(runs):
+    """FP16 storage on this small copy: round-to-nearest loses part of the force (7e-3 of the profile measured
+    here; 0.41 on the full 512 x 256 x 256 channel after 20000 steps, tests/test_gpu_configs.py), stochastic
+    rounding keeps it (within 2e-3: its noise, 7e-4 measured)."""
     ref = runs["float64"][0][0, :, 0]
-    ux, drift = runs[dtype]
-    prof = ux[0, :, 0]
-    err = np.abs(prof - ref).max() / np.abs(ref).max()
-    print(f"{dtype} storage: profile error vs FP64 storage {err:.3e}, mass drift per cell {drift:.3e}", flush=True)
-    assert np.isfinite(prof).all()
-    assert err <= tol
-    assert abs(drift) <= drift_tol
+    errs = {}
+    for key in ("float16", "float16-sr"):
+        ux, drift = runs[key]
+        prof = ux[0, :, 0]
+        errs[key] = np.abs(prof - ref).max() / np.abs(ref).max()
+        print(f"{key} storage: profile error vs FP64 storage {errs[key]:.3e}, mass drift per cell {drift:.3e}",
+              flush=True)
+        assert np.isfinite(prof).all()
+        assert abs(drift) <= 1e-6
+    assert errs["float16-sr"] <= 2e-3
+    assert errs["float16-sr"] < errs["float16"]

@@ -49,8 +49,8 @@ def test_every_declared_symbol_is_exported(lib):
 
 def test_descriptor_layout_matches(lib):
     from lbmpy_amd import _native
-    assert lib.lbm355_desc_size() == ctypes.sizeof(_native.SweepDesc) == 192
-    assert lib.lbm355_version() >= 4
+    assert lib.lbm355_desc_size() == ctypes.sizeof(_native.SweepDesc) == 200
+    assert lib.lbm355_version() >= 5
 
 
 def test_header_constants_match_binding():
