@@ -86,6 +86,9 @@ extern "C" {
                                                 neighbouring FP16 values with probabilities that make the rounding
                                                 unbiased, from a hash of (cell, direction, rounding_seed); no lbmpy
                                                 counterpart (lbmpy has no FP16 storage) */
+#define LBM355_FLAG_GENERAL_CUMULANT 1024 /* D3Q27 cumulant: a cumulant set other than the default polynomial
+                                             one (lbmpy MONOMIAL_CUMULANT, custom nested_moments): the relaxation
+                                             comes as the table's linear maps on groups of monomial cumulants */
 #define LBM355_FLAG_ADAPT_VELOCITY 256 /* boundary links, UBB: drive the wall at its equilibrium input velocity
                                           u_w - F / (2 rho_ref), rho_ref the link's density (the UBB's own, or
                                           the fluid cell's for compressible methods) or 1 for incompressible ones

@@ -238,6 +238,8 @@ int lbm355_batch_create(const lbm355_sweep_desc* descs, const void* const* pdf_s
       return berr(LBM355_ERR_UNSUPPORTED, "the cumulant corrections are not batched");
     if (d.flags & LBM355_FLAG_STOCHASTIC_ROUNDING)
       return berr(LBM355_ERR_UNSUPPORTED, "stochastic rounding is not batched");
+    if (d.flags & LBM355_FLAG_GENERAL_CUMULANT)
+      return berr(LBM355_ERR_UNSUPPORTED, "only the default polynomial cumulant set is batched");
     if (d.omega_out || d.eddy_viscosity_out || d.density_in || d.velocity_in)
       return berr(LBM355_ERR_UNSUPPORTED, "per-cell subgrid outputs and density / velocity inputs are not batched");
     if (!pdf_src[k] || !pdf_dst[k]) return berr(LBM355_ERR_ARG, "NULL member PDF pointer");

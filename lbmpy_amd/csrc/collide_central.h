@@ -145,7 +145,7 @@ __device__ __forceinline__ void collide_central(T (&f)[S::Q], const SweepArgs<T>
   const bool comp = a.flags & LBM355_FLAG_COMPRESSIBLE;
   const T drho = s;  // deviation storage (absolute input is shifted by the caller)
   const T rho = T(1) + drho;
-  const T inv_ref = comp ? T(1) / rho : T(1);
+  const T inv_ref = comp ? recip(rho) : T(1);
   T u[3] = {0, 0, 0};
 #pragma unroll
   for (int d = 0; d < D; ++d) {

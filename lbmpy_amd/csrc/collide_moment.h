@@ -53,7 +53,7 @@ __device__ __forceinline__ void collide_moment(T (&f)[S::Q], const SweepArgs<T>&
   const bool comp = a.flags & LBM355_FLAG_COMPRESSIBLE;
   T rho = zc ? T(1) + s : s;
   T drho = zc ? s : s - T(1);
-  const T inv_ref = comp ? T(1) / rho : T(1);
+  const T inv_ref = comp ? recip(rho) : T(1);
   T u[3] = {0, 0, 0};
 #pragma unroll
   for (int d = 0; d < D; ++d) {

@@ -163,6 +163,19 @@ template <class TS, class TC> __device__ __forceinline__ TC load_as(const TS* p)
 template <> __device__ __forceinline__ float load_as<_Float16, float>(const _Float16* p) { return (float)(*p); }
 template <class TS, class TC> __device__ __forceinline__ void store_as(TS* p, TC v) { *p = (TS)v; }
 
+// ---------------------------------------------------------------- arithmetic helpers
+// 1 / x.  FP32: the hardware reciprocal (about 1 ulp) refined by one Newton step, 3 instructions instead of the
+// 10 of an IEEE division; FP64: division.
+template <class T>
+__device__ __forceinline__ T recip(T x) {
+  return T(1) / x;
+}
+template <>
+__device__ __forceinline__ float recip<float>(float x) {
+  const float r = __builtin_amdgcn_rcpf(x);
+  return __builtin_fmaf(r, __builtin_fmaf(-x, r, 1.0f), r);
+}
+
 // ---------------------------------------------------------------- launch arguments
 template <class TC>
 struct SweepArgs {

@@ -18,6 +18,7 @@ struct Collider;
 template <class S, class TC>
 struct Collider<S, LBM355_FAMILY_MOMENT, TC> {
   static constexpr bool deviation_form = false;
+  template <bool NEQ>
   static __device__ __forceinline__ void run(TC (&f)[S::Q], const SweepArgs<TC>& a, int64_t o, TC& rho, TC (&u)[3],
                                              CellExtras<TC>&) {
     collide_moment<S, TC>(f, a, o, rho, u);
@@ -30,9 +31,10 @@ constexpr int FAMILY_CUMULANT_LEARNED = 0x102;
 template <class TC>
 struct Collider<D3Q27, FAMILY_CUMULANT_LEARNED, TC> {
   static constexpr bool deviation_form = true;
+  template <bool NEQ>
   static __device__ __forceinline__ void run(TC (&f)[27], const SweepArgs<TC>& a, int64_t, TC& rho, TC (&u)[3],
                                              CellExtras<TC>& ex) {
-    collide_cumulant_d3q27<TC, true>(f, a, rho, u, ex);
+    collide_cumulant_d3q27<TC, true, false, NEQ>(f, a, rho, u, ex);
   }
 };
 
@@ -42,18 +44,33 @@ constexpr int FAMILY_CUMULANT_CORRECTED = 0x103;
 template <class TC>
 struct Collider<D3Q27, FAMILY_CUMULANT_CORRECTED, TC> {
   static constexpr bool deviation_form = true;
+  template <bool NEQ>
   static __device__ __forceinline__ void run(TC (&f)[27], const SweepArgs<TC>& a, int64_t, TC& rho, TC (&u)[3],
                                              CellExtras<TC>& ex) {
-    collide_cumulant_d3q27<TC, false, true>(f, a, rho, u, ex);
+    collide_cumulant_d3q27<TC, false, true, NEQ>(f, a, rho, u, ex);
+  }
+};
+
+// kernel-variant family code of the cumulant sweep for cumulant sets other than the default polynomial one
+constexpr int FAMILY_CUMULANT_GENERAL = 0x104;
+
+template <class TC>
+struct Collider<D3Q27, FAMILY_CUMULANT_GENERAL, TC> {
+  static constexpr bool deviation_form = true;
+  template <bool NEQ>
+  static __device__ __forceinline__ void run(TC (&f)[27], const SweepArgs<TC>& a, int64_t, TC& rho, TC (&u)[3],
+                                             CellExtras<TC>& ex) {
+    collide_cumulant_d3q27<TC, false, false, NEQ, true>(f, a, rho, u, ex);
   }
 };
 
 template <class TC>
 struct Collider<D3Q27, LBM355_FAMILY_CUMULANT, TC> {
   static constexpr bool deviation_form = true;
+  template <bool NEQ>
   static __device__ __forceinline__ void run(TC (&f)[27], const SweepArgs<TC>& a, int64_t, TC& rho, TC (&u)[3],
                                              CellExtras<TC>& ex) {
-    collide_cumulant_d3q27<TC>(f, a, rho, u, ex);
+    collide_cumulant_d3q27<TC, false, false, NEQ>(f, a, rho, u, ex);
   }
 };
 
@@ -85,6 +102,7 @@ __device__ __forceinline__ void store_nt(TS* p, TC v) {
 template <class S, class TC>
 struct Collider<S, LBM355_FAMILY_CENTRAL, TC> {
   static constexpr bool deviation_form = true;
+  template <bool NEQ>
   static __device__ __forceinline__ void run(TC (&f)[S::Q], const SweepArgs<TC>& a, int64_t, TC& rho, TC (&u)[3],
                                              CellExtras<TC>&) {
     collide_central<S, TC>(f, a, rho, u);
@@ -139,7 +157,9 @@ __device__ __forceinline__ void collide_cell(TC (&f)[S::Q], const SweepArgs<TC>&
   const int64_t o = (int64_t)(x + a.g) + (int64_t)(y + a.g) * a.osy + (D == 3 ? (int64_t)(z + a.g) * a.osz : 0);
   TC rho, u[3];
   CellExtras<TC> ex = {a.omega_shear, TC(0)};
-  Collider<S, FAM, TC>::run(f, a, o, rho, u, ex);
+  // the cumulant collision transforms f - f_eq for FP32 storage with FP32 arithmetic (collide_cumulant.h)
+  constexpr bool NEQ = std::is_same<TC, float>::value && std::is_same<TS, float>::value;
+  Collider<S, FAM, TC>::template run<NEQ>(f, a, o, rho, u, ex);
   if (Collider<S, FAM, TC>::deviation_form && absolute) {
 #pragma unroll
     for (int i = 0; i < Q; ++i) f[i] += TC(weight<S>(i));
@@ -153,6 +173,41 @@ __device__ __forceinline__ void collide_cell(TC (&f)[S::Q], const SweepArgs<TC>&
   if (a.nut_out) a.nut_out[o] = a.nut_scale * ex.omega_sgs * ex.pi;
   if constexpr (std::is_same<TS, _Float16>::value) {
     if (a.flags & LBM355_FLAG_STOCHASTIC_ROUNDING) round_stochastic_f16<S, TC>(f, a.seed, x, y, z, a.nx, a.ny);
+  }
+}
+
+// Loads and stores of element `off` (0 <= off < 2^31) of a population plane.  With 2-byte storage the byte offset
+// fits in 32 bits (a plane holds < 2^31 elements), so they are raw buffer accesses: the plane's base lives in a
+// buffer descriptor that scalar instructions build, one 32-bit vector offset serves every plane, and no population
+// needs 64-bit address arithmetic.  Wider storage uses ordinary pointers.  NT: non-temporal.
+constexpr int BUFFER_DWORD3 = 0x00020000;   // raw (untyped) buffer access on gfx9 family targets
+
+template <class TS, class TC, bool NT>
+__device__ __forceinline__ TC plane_load(const TS* plane, int off) {
+  if constexpr (sizeof(TS) == 2) {
+    const __amdgpu_buffer_rsrc_t r =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<TS*>(plane), (short)0, -1, BUFFER_DWORD3);
+    const unsigned short bits = __builtin_amdgcn_raw_buffer_load_b16(r, (int)((uint32_t)off * 2u), 0, NT ? 2 : 0);
+    TS v;
+    __builtin_memcpy(&v, &bits, 2);
+    return (TC)v;
+  } else {
+    return NT ? load_nt<TS, TC>(plane + off) : load_as<TS, TC>(plane + off);
+  }
+}
+template <class TS, class TC, bool NT>
+__device__ __forceinline__ void plane_store(TS* plane, int off, TC value) {
+  if constexpr (sizeof(TS) == 2) {
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(plane, (short)0, -1, BUFFER_DWORD3);
+    const TS v = (TS)value;
+    unsigned short bits;
+    __builtin_memcpy(&bits, &v, 2);
+    __builtin_amdgcn_raw_buffer_store_b16(bits, r, (int)((uint32_t)off * 2u), 0, NT ? 2 : 0);
+  } else {
+    if (NT)
+      store_nt<TS, TC>(plane + off, value);
+    else
+      store_as<TS, TC>(plane + off, value);
   }
 }
 
@@ -216,8 +271,8 @@ __device__ __forceinline__ void sweep_cell(const SweepArgs<TC>& a, int x, int y,
       if (W)
         wall = (cx == 1 && ax.wall_lo) || (cx == -1 && ax.wall_hi) || (cy == 1 && ay.wall_lo) ||
                (cy == -1 && ay.wall_hi) || (cz == 1 && az.wall_lo) || (cz == -1 && az.wall_hi);
-      const TS* p = wall ? (src + wplane * sq + center) : (src + plane * sq + X[1 - cx] + Y[1 - cy] + Z[1 - cz]);
-      f[i] = load_as<TS, TC>(p);
+      f[i] = wall ? plane_load<TS, TC, false>(src + wplane * sq, center)
+                  : plane_load<TS, TC, false>(src + plane * sq, X[1 - cx] + Y[1 - cy] + Z[1 - cz]);
     }
   };
   // AA odd writes f_i to x + c_i slot i; at a wall it is bounced into the cell's slot inv(i)
@@ -231,14 +286,16 @@ __device__ __forceinline__ void sweep_cell(const SweepArgs<TC>& a, int x, int y,
       if (W)
         wall = (cx == -1 && ax.wall_lo) || (cx == 1 && ax.wall_hi) || (cy == -1 && ay.wall_lo) ||
                (cy == 1 && ay.wall_hi) || (cz == -1 && az.wall_lo) || (cz == 1 && az.wall_hi);
-      TS* p = wall ? (dst + ii * sq + center) : (dst + i * sq + X[1 + cx] + Y[1 + cy] + Z[1 + cz]);
-      store_as<TS, TC>(p, f[i]);
+      if (wall)
+        plane_store<TS, TC, false>(dst + ii * sq, center, f[i]);
+      else
+        plane_store<TS, TC, false>(dst + i * sq, X[1 + cx] + Y[1 + cy] + Z[1 + cz], f[i]);
     }
   };
 
   if (PAT == LBM355_AA_EVEN) {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) f[i] = load_nt<TS, TC>(src + i * sq + center);
+    for (int i = 0; i < Q; ++i) f[i] = plane_load<TS, TC, true>(src + i * sq, center);
   } else if (near_wall) {
     load(std::true_type{});
   } else {
@@ -249,10 +306,10 @@ __device__ __forceinline__ void sweep_cell(const SweepArgs<TC>& a, int x, int y,
 
   if (PAT == LBM355_PULL) {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) store_nt<TS, TC>(dst + i * sq + center, f[i]);
+    for (int i = 0; i < Q; ++i) plane_store<TS, TC, true>(dst + i * sq, center, f[i]);
   } else if (PAT == LBM355_AA_EVEN) {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) store_nt<TS, TC>(dst + S::inv[i] * sq + center, f[i]);
+    for (int i = 0; i < Q; ++i) plane_store<TS, TC, true>(dst + S::inv[i] * sq, center, f[i]);
   } else if (near_wall) {
     store(std::true_type{});
   } else {

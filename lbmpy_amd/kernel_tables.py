@@ -275,17 +275,75 @@ def fourth_order_parameters(w1, w2):
     return a, b, w3, w4, w5
 
 
+# monomial groups of the general cumulant relaxation, in the order of CumulantTab's BLOCKS
+_CUMULANT_BLOCKS = (
+    ((2, 0, 0), (0, 2, 0), (0, 0, 2)),
+    ((1, 1, 0), (1, 0, 1), (0, 1, 1)),
+    ((1, 2, 0), (1, 0, 2), (2, 1, 0), (0, 1, 2), (2, 0, 1), (0, 2, 1)),
+    ((1, 1, 1),),
+    ((2, 2, 0), (2, 0, 2), (0, 2, 2)),
+    ((2, 1, 1), (1, 2, 1), (1, 1, 2)),
+    ((2, 2, 1), (2, 1, 2), (1, 2, 2)),
+    ((2, 2, 2),),
+)
+_CONSERVED = ((0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1))
+
+
+def cumulant_is_default_set(method):
+    """True for lbmpy's default polynomial D3Q27 cumulant set (cascaded_moment_sets_literature), which the
+    cumulant sweep relaxes with per-polynomial rates; other sets take the general kernel variant."""
+    reference = [p for g in cascaded_moment_sets_literature(method.stencil) for p in g]
+    return list(method.cumulants) == reference
+
+
+def cumulant_blocks(method, rates):
+    """The relaxation of the monomial cumulant deviations as linear maps per monomial group: with P the method's
+    polynomial cumulants in monomial coordinates and D their rates, post = P^-1 (I - D) P pre (the equilibrium
+    deviation of every cumulant is zero, cumulantbasedmethod.py:374-379).  Returns the blocks of
+    _CUMULANT_BLOCKS flattened row-major; raises UnsupportedMethod when the set couples monomials of different
+    groups, or relaxes a conserved (order 0 / 1) cumulant."""
+    monos = [tuple(e) for e in _CANON]
+    index = {e: i for i, e in enumerate(monos)}
+    polys = list(method.cumulants)
+    P = np.zeros((27, 27))
+    for r, p in enumerate(polys):
+        for e, cf in p.terms.items():
+            P[r, index[tuple(e[:3]) + (0,) * (3 - len(e[:3]))]] += float(cf)
+    if abs(np.linalg.det(P)) < 1e-12:
+        raise UnsupportedMethod("the cumulant polynomials are not independent")
+    M = np.linalg.solve(P, np.diag(1.0 - np.asarray(rates, dtype=np.float64)) @ P)
+    for e in _CONSERVED:
+        i = index[e]
+        row, col = M[i].copy(), M[:, i].copy()
+        row[i] -= 1.0
+        col[i] -= 1.0
+        if np.abs(row).max() > 1e-12 or np.abs(col).max() > 1e-12:
+            raise UnsupportedMethod("the sweep keeps the conserved (order 0 and 1) cumulants unrelaxed")
+    covered = np.zeros((27, 27), dtype=bool)
+    for e in _CONSERVED:
+        covered[index[e], index[e]] = True
+    out = []
+    for group in _CUMULANT_BLOCKS:
+        ids = [index[e] for e in group]
+        covered[np.ix_(ids, ids)] = True
+        out.extend(M[np.ix_(ids, ids)].reshape(-1))
+    if np.abs(M[~covered]).max(initial=0.0) > 1e-12:
+        raise UnsupportedMethod("the cumulant set couples monomials the sweep relaxes in separate groups")
+    return np.array(out)
+
+
 def cumulant_table(method, bindings, dtype=np.float64, corrections=None):
-    """Per-polynomial rates in the order of the default D3Q27 polynomial cumulant set, then the
-    fourth-order correction's a, b, omega_3..5 and limiter (layout CumulantTab in
-    csrc/collide_cumulant.h)."""
+    """Per-polynomial rates (for the default D3Q27 polynomial cumulant set, in its order), then the
+    fourth-order correction's a, b, omega_3..5 and limiter, then the relaxation as linear maps on groups of
+    monomial cumulants (cumulant_blocks; what the general kernel variant reads) -- layout CumulantTab in
+    csrc/collide_cumulant.h."""
     st = method.stencil
     if st.name != "D3Q27":
         raise UnsupportedMethod("the cumulant sweep is implemented for D3Q27")
-    reference = [p for g in cascaded_moment_sets_literature(st) for p in g]
-    polys = list(method.cumulants)
-    if polys != reference:
-        raise UnsupportedMethod("the cumulant sweep implements the default polynomial cumulant set")
+    default_set = cumulant_is_default_set(method)
+    if not default_set and corrections is not None:
+        raise UnsupportedMethod("the Galilean / fourth-order corrections are defined for the default polynomial "
+                                "cumulant set")
     fm = method.force_model
     if fm is not None and type(fm).__name__ != "CentralMoment":
         raise UnsupportedMethod("cumulant sweep supports the CentralMoment force model")
@@ -309,7 +367,10 @@ def cumulant_table(method, bindings, dtype=np.float64, corrections=None):
         raise UnsupportedMethod("the zeroth cumulant must not be relaxed")
     if corrections is not None and corrections.galilean and rates[7] != rates[8]:
         raise ValueError("Cumulants (x^2 - y^2) and (x^2 - z^2) must have the same relaxation rate")
-    return np.concatenate([rates, extra]).astype(dtype), 0
+    blocks = cumulant_blocks(method, rates)
+    if not default_set:
+        rates = np.zeros(27)
+    return np.concatenate([rates, extra, blocks]).astype(dtype), 0
 
 
 # ------------------------------------------------------------------------------- central-moment family

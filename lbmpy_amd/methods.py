@@ -317,18 +317,64 @@ def create_with_default_polynomial_cumulants(stencil, relaxation_rates, conserve
 
 
 def _get_relaxation_info_dict(relaxation_rates, nested_moments, dim, conserved_moments=True):
-    """Moment -> relaxation rate (lbmpy ``methods/creationfunctions.py:600-706``).
+    """Moment -> relaxation rate under lbmpy's rules for a rate list (methods/creationfunctions.py:600-706):
 
-    One rate: order <= 1 conserved (0), shear moments get the rate, the rest 1.
-    Q rates: one per moment.  Otherwise: shear rate, bulk rate, then one rate per
-    further group."""
-    result = OrderedDict()
+    * a callable maps each group to the rates of its moments;
+    * one rate per moment: taken in order;
+    * a single rate: the shear moments take it, moments of order <= 1 take 0 (or the rate, when they are not
+      conserved) and every other moment relaxes with 1;
+    * any other count: the shear rate (when the set has shear moments), the bulk rate (when it has a bulk
+      moment), then one rate per further group that holds a moment of neither kind.  A rate is taken from the
+      list when a group begins and no rate is waiting, and it waits for the next group when this one has no
+      moment that needs it -- so a list that runs out raises ValueError even at such a group."""
+    moments = [m for g in nested_moments for m in g]
     if callable(relaxation_rates):
-        for group in nested_moments:
-            rr = iter(relaxation_rates(group))
-            for m in group:
-                result[m] = next(rr)
-        return result
+        return OrderedDict((m, w) for g in nested_moments for m, w in zip(g, relaxation_rates(g)))
+    rates = list(relaxation_rates)
+    if len(rates) == len(moments):
+        return OrderedDict(zip(moments, rates))
+
+    def shear(m):
+        return is_shear_moment(m, dim)
+
+    def low(m):
+        return get_order(m) <= 1
+
+    if len(rates) == 1:
+        w = rates[0]
+        return OrderedDict((m, w if shear(m) else (0.0 if conserved_moments else w) if low(m) else 1.0)
+                           for m in moments)
+
+    supply = iter(rates)
+
+    def take():
+        for w in supply:
+            return w
+        raise ValueError(f"{len(rates)} relaxation rates do not cover {len(moments)} moments: pass a single "
+                         "(shear) rate, one rate per moment, or the shear and bulk rates followed by one rate for "
+                         "each further group of moments")
+
+    w_shear = take() if any(shear(m) for m in moments) else None
+    w_bulk = take() if any(is_bulk_moment(m, dim) for m in moments) else None
+    result = OrderedDict()
+    waiting = None
+    for group in nested_moments:
+        if waiting is None:
+            waiting = take()
+        spent = False
+        for m in group:
+            if low(m) and conserved_moments:
+                result[m] = 0.0
+            elif shear(m) or low(m):
+                result[m] = w_shear
+            elif is_bulk_moment(m, dim):
+                result[m] = w_bulk
+            else:
+                result[m] = waiting
+                spent = True
+        if spent:
+            waiting = None
+    return result
 
     relaxation_rates = list(relaxation_rates)
     n = sum(len(g) for g in nested_moments)

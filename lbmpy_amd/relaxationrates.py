@@ -23,20 +23,21 @@ def relaxation_rate_from_magic_number(hydrodynamic_relaxation_rate, magic_number
 
 
 def get_shear_relaxation_rate(method):
-    """Rate shared by all shear moments (lbmpy ``relaxationrates.py:23-46``)."""
-    rates = set()
-    for moment, info in method.relaxation_info_dict.items():
-        if is_shear_moment(moment, method.dim):
-            rates.add(info.relaxation_rate)
-    if len(rates) == 1:
-        return rates.pop()
-    if len(rates) > 1:
-        raise ValueError(f"Shear moments are relaxed with different relaxation times: {rates}")
-    all_rates = set(method.relaxation_rates)
-    if len(all_rates) == 1:
-        return all_rates.pop()
-    raise NotImplementedError("Shear moments seem to be not relaxed separately - "
-                              "Can not determine their relaxation rate automatically")
+    """The one rate that relaxes the method's shear moments (lbmpy ``relaxationrates.py:23-46``).  A method
+    without shear moments qualifies only when a single rate relaxes all of its moments.  Shear moments with
+    different rates raise ValueError; no shear moment and several rates raise NotImplementedError."""
+    dim = method.dim
+    found = {info.relaxation_rate for moment, info in method.relaxation_info_dict.items()
+             if is_shear_moment(moment, dim)}
+    if len(found) > 1:
+        raise ValueError(f"the shear moments of this method relax with {len(found)} different rates "
+                         f"({', '.join(sorted(map(str, found)))}); a single shear rate is needed")
+    if not found:
+        found = set(method.relaxation_rates)
+        if len(found) != 1:
+            raise NotImplementedError("this method relaxes no moment as a shear moment and uses "
+                                      f"{len(found)} different rates, so it has no single shear rate")
+    return next(iter(found))
 
 
 def get_bulk_relaxation_rate(method):

@@ -17,6 +17,7 @@ import torch
 
 from . import _native as nat
 from .kernel_tables import method_table, family_of, STENCIL_CODES, UnsupportedMethod, bind_rate, \
+    cumulant_is_default_set, \
     cumulant_corrections
 from .relaxationrates import get_shear_relaxation_rate
 from .enums import SubgridScaleModel, Target
@@ -132,6 +133,11 @@ class SweepKernel:
         if (lc.omega_output_field is not None or lc.eddy_viscosity_field is not None) and self.smagorinsky is None:
             raise NotImplementedError("omega_output_field / eddy_viscosity_field are written by the Smagorinsky "
                                       "model on the GPU path; set subgrid_scale_model")
+        #: a cumulant set other than the default polynomial one: relaxed by the general kernel variant
+        self.general_cumulant = self.family == nat.FAMILY_CUMULANT and not cumulant_is_default_set(m)
+        if self.general_cumulant and self.smagorinsky is not None:
+            raise NotImplementedError("the Smagorinsky model acts on the shear rates of the default polynomial "
+                                      "cumulant set")
         self.closure = None
         self._shear_rate = None          # (bindings key, bound shear rate): the lookup is symbolic and slow
         self._tables = {}
@@ -160,6 +166,9 @@ class SweepKernel:
                 raise ValueError("a sweep uses either the Smagorinsky model or a learned closure")
             if self.corrections is not None:
                 raise NotImplementedError("the learned closure is not combined with the cumulant corrections")
+            if self.general_cumulant:
+                raise NotImplementedError("the learned closure sets the shear rate of the default polynomial "
+                                          "cumulant set")
         self.closure = model
         return self
 
@@ -246,6 +255,8 @@ class SweepKernel:
                 flags |= nat.FLAG_GALILEAN
             if self.corrections.fourth_order:
                 flags |= nat.FLAG_FOURTH_ORDER
+        if self.general_cumulant:
+            flags |= nat.FLAG_GENERAL_CUMULANT
         if self.closure is not None:
             flags |= nat.FLAG_LEARNED_CLOSURE
             d.closure = self.closure.params.data_ptr()

@@ -49,7 +49,7 @@ def test_table_matches_oracle_parametrisation(limiter):
     tab, _ = method_table(m, {"omega": 1.93}, corrections=cumulant_corrections(lc))
     a, b = O._fourth_order_ab(1.93, 1.2)
     w3, w4, w5 = O._fourth_order_omegas(1.93, 1.2)
-    np.testing.assert_allclose(tab[27:], [a, b, w3, w4, w5, 1e6 if limiter is True else limiter], rtol=1e-14)
+    np.testing.assert_allclose(tab[27:33], [a, b, w3, w4, w5, 1e6 if limiter is True else limiter], rtol=1e-14)
     np.testing.assert_allclose(tab[4:10], [1.93] * 5 + [1.2])
     np.testing.assert_allclose(tab[10:17], [w3] * 3 + [w4] * 3 + [w5], rtol=1e-14)
     np.testing.assert_array_equal(tab[17:27], 1.0)
@@ -58,7 +58,7 @@ def test_table_matches_oracle_parametrisation(limiter):
 def test_plain_table_has_zero_correction_entries():
     m = create_lb_method(_lc(relaxation_rate=1.9))
     tab, _ = method_table(m, {})
-    assert tab.size == 33 and np.all(tab[27:] == 0.0)
+    assert tab.size == 116 and np.all(tab[27:33] == 0.0)
 
 
 def test_fourth_order_with_smagorinsky_is_rejected():
