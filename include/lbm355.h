@@ -299,6 +299,19 @@ int lbm355_batch_info(const lbm355_batch* batch, int64_t* n_tiles, int64_t* n_ce
 /* Frees the member table (synchronous). */
 int lbm355_batch_destroy(lbm355_batch* batch);
 
+/* One batched step without any device allocation: lbm355_batch_create + lbm355_batch_sweep in one call, with the
+ * member table and the tile map in a file-scope device array (8 MiB of tile map and about 0.25 MiB for 1024
+ * member entries) that each GPU holds until the process ends, instead of memory the library allocates.  No
+ * counterpart in lbmpy: it is for callers that cannot allocate device memory (serving loops and harnesses that
+ * forbid hipMalloc).  Limits: at most 1024 members and
+ * 2^21 tiles of 64 cells (134,217,728 interior cells); larger ensembles are refused with LBM355_ERR_ARG.  Same
+ * arguments, checks and kernel as lbm355_batch_create / lbm355_batch_sweep, so results are identical.  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (other streams or host threads) race.  The
+ * call waits on `stream` once (for the copy of the member table), so it is not meant for graph capture. */
+int lbm355_batch_sweep_static(const lbm355_sweep_desc* descs, const void* const* pdf_src, void* const* pdf_dst,
+                              int32_t n_members, int32_t step_kind, int32_t threads, int32_t blocks,
+                              int32_t xcd_order, void* stream);
+
 /* ---- Learned closure ----------------------------------------------------------------------------
  * No lbmpy counterpart: a per-cell MLP (10 -> 32 -> 32 -> 1, tanh, BF16 weights, FP32 accumulation
  * on the MFMA units) that maps a cell's non-equilibrium second moments to an eddy viscosity and

@@ -60,7 +60,7 @@ class SweepDesc(ctypes.Structure):
 EXPORTED = ("lbm355_sweep", "lbm355_table_elements", "lbm355_noslip_links", "lbm355_reduce_conserved",
             "lbm355_set_equilibrium", "lbm355_get_macroscopic", "lbm355_stream_copy", "lbm355_desc_size",
             "lbm355_last_error", "lbm355_version", "lbm355_batch_create", "lbm355_batch_sweep",
-            "lbm355_batch_info", "lbm355_batch_destroy", "lbm355_closure_params_size",
+            "lbm355_batch_info", "lbm355_batch_destroy", "lbm355_batch_sweep_static", "lbm355_closure_params_size",
             "lbm355_closure_master_size", "lbm355_closure_features", "lbm355_closure_forward",
             "lbm355_closure_train_step", "lbm355_closure_adam", "lbm355_boundary_links", "lbm355_copy_boxes",
             "lbm355_pattern_access", "lbm355_get_strain_rate", "lbm355_plane_copy",
@@ -107,6 +107,9 @@ def load(path=None):
     lib.lbm355_batch_info.restype = c_int32
     lib.lbm355_batch_destroy.argtypes = [c_void_p]
     lib.lbm355_batch_destroy.restype = c_int32
+    lib.lbm355_batch_sweep_static.argtypes = [ctypes.POINTER(SweepDesc), c_void_p, c_void_p, c_int32, c_int32, c_int32,
+                                              c_int32, c_int32, c_void_p]
+    lib.lbm355_batch_sweep_static.restype = c_int32
     lib.lbm355_closure_params_size.restype = c_int64
     lib.lbm355_closure_master_size.restype = c_int32
     lib.lbm355_closure_features.argtypes = [c_void_p, c_void_p, c_int64, c_void_p, c_void_p]

@@ -14,6 +14,7 @@
 // (each with its own L2).  With `xcd_order`, the workgroups of one XCD take one contiguous
 // stretch of the tile sequence, so the rows one tile's neighbour reads touch are fetched into the
 // same L2 that the neighbouring tiles are processed on.
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -182,21 +183,34 @@ struct lbm355_batch {
 namespace {
 int berr(int code, const std::string& msg) { return lbm355_set_error(code, msg.c_str()); }
 
+// The member table: each member's kernel arguments, its first tile and its cell count.  Returns the tile and
+// cell totals through `tiles` / `cells`.
+template <class TC>
+int fill_entries(const lbm355_sweep_desc* d, const void* const* src, void* const* dst, int32_t n, BatchEntry<TC>* out,
+                 int64_t* tiles_out, int64_t* cells_out) {
+  int64_t tiles = 0, cells = 0;
+  for (int k = 0; k < n; ++k) {
+    out[k].a = make_args<TC>(&d[k], src[k], dst[k], nullptr, nullptr);
+    out[k].first_tile = tiles;
+    out[k].ncells = (int64_t)d[k].nx * d[k].ny * d[k].nz;
+    if (out[k].ncells >= (int64_t(1) << 31)) return berr(LBM355_ERR_ARG, "a batch member must hold < 2^31 cells");
+    tiles += (out[k].ncells + 63) / 64;
+    cells += out[k].ncells;
+  }
+  *tiles_out = tiles;
+  *cells_out = cells;
+  return LBM355_OK;
+}
+
 template <class TC>
 int upload(const lbm355_sweep_desc* d, const void* const* src, void* const* dst, int32_t n, lbm355_batch* b) {
   std::vector<BatchEntry<TC>> host(n);
-  std::vector<int32_t> map;
   int64_t tiles = 0, cells = 0;
-  for (int k = 0; k < n; ++k) {
-    host[k].a = make_args<TC>(&d[k], src[k], dst[k], nullptr, nullptr);
-    host[k].first_tile = tiles;
-    host[k].ncells = (int64_t)d[k].nx * d[k].ny * d[k].nz;
-    if (host[k].ncells >= (int64_t(1) << 31)) return berr(LBM355_ERR_ARG, "a batch member must hold < 2^31 cells");
-    const int64_t t = (host[k].ncells + 63) / 64;
-    map.insert(map.end(), (size_t)t, k);
-    tiles += t;
-    cells += host[k].ncells;
-  }
+  int rc = fill_entries<TC>(d, src, dst, n, host.data(), &tiles, &cells);
+  if (rc) return rc;
+  std::vector<int32_t> map;
+  map.reserve((size_t)tiles);
+  for (int k = 0; k < n; ++k) map.insert(map.end(), (size_t)((host[k].ncells + 63) / 64), k);
   void* dev = nullptr;
   void* dmap = nullptr;
   hipError_t e = hipMalloc(&dev, sizeof(BatchEntry<TC>) * (size_t)n);
@@ -218,12 +232,17 @@ int upload(const lbm355_sweep_desc* d, const void* const* src, void* const* dst,
 
 extern "C" {
 
-int lbm355_batch_create(const lbm355_sweep_desc* descs, const void* const* pdf_src, void* const* pdf_dst,
-                        int32_t n_members, lbm355_batch** out) {
-  if (!descs || !pdf_src || !pdf_dst || !out || n_members <= 0) return berr(LBM355_ERR_ARG, "bad batch arguments");
-  *out = nullptr;
+}  // extern "C"
+
+namespace {
+// The members must share stencil, family, pattern kind and dtypes, and use only what the batched kernel runs.
+// Sets *kind to LBM355_PULL or LBM355_AA_EVEN.
+int check_members(const lbm355_sweep_desc* descs, const void* const* pdf_src, void* const* pdf_dst, int32_t n_members,
+                  int* kind) {
+  if (!descs || !pdf_src || !pdf_dst || n_members <= 0) return berr(LBM355_ERR_ARG, "bad batch arguments");
   const lbm355_sweep_desc& d0 = descs[0];
   const int kind0 = d0.pattern == LBM355_PULL ? LBM355_PULL : LBM355_AA_EVEN;
+  *kind = kind0;
   for (int k = 0; k < n_members; ++k) {
     const lbm355_sweep_desc& d = descs[k];
     int rc = lbm355_validate_desc(&d);
@@ -248,6 +267,59 @@ int lbm355_batch_create(const lbm355_sweep_desc* descs, const void* const* pdf_s
   }
   void* probe = d0.compute_dtype == LBM355_F64 ? batch_pick<double>(&d0, kind0) : batch_pick<float>(&d0, kind0);
   if (!probe) return berr(LBM355_ERR_UNSUPPORTED, "no batched kernel for this stencil/family/dtype combination");
+  return LBM355_OK;
+}
+
+// Checks the launch shape and step kind of a batched step.
+int check_launch(int32_t pattern_kind, int32_t step_kind, int32_t threads, int32_t blocks) {
+  if (threads != 64 && threads != 128 && threads != 192 && threads != 256)
+    return berr(LBM355_ERR_ARG, "threads per block must be 64, 128, 192 or 256");
+  if (blocks <= 0) return berr(LBM355_ERR_ARG, "blocks must be positive");
+  if (pattern_kind == LBM355_PULL && step_kind != 0 && step_kind != 1)
+    return berr(LBM355_ERR_ARG, "pull step_kind is 0 (src->dst) or 1 (dst->src)");
+  if (pattern_kind != LBM355_PULL && step_kind != LBM355_AA_EVEN && step_kind != LBM355_AA_ODD)
+    return berr(LBM355_ERR_ARG, "AA step_kind is LBM355_AA_EVEN or LBM355_AA_ODD");
+  return LBM355_OK;
+}
+
+// The launch of one batched step (shared by the allocating and the allocation-free entry points).
+int launch_batch(int32_t stencil, int32_t family, int32_t store_dtype, int32_t compute_dtype, int32_t pattern_kind,
+                 const void* entries, const int32_t* tile_member, int64_t ntiles, int32_t step_kind, int32_t threads,
+                 int32_t blocks, int32_t xcd_order, hipStream_t s) {
+  int rc = check_launch(pattern_kind, step_kind, threads, blocks);
+  if (rc) return rc;
+  const int pattern = pattern_kind == LBM355_PULL ? LBM355_PULL : step_kind;
+  const int swap = pattern_kind == LBM355_PULL ? step_kind : 0;
+  lbm355_sweep_desc d{};
+  d.stencil = stencil;
+  d.family = family;
+  d.store_dtype = store_dtype;
+  void* k = compute_dtype == LBM355_F64 ? batch_pick<double>(&d, pattern) : batch_pick<float>(&d, pattern);
+  if (!k) return berr(LBM355_ERR_UNSUPPORTED, "no batched kernel");
+  const int64_t max_blocks = (ntiles * 64 + threads - 1) / threads;
+  const int grid = (int)(blocks < max_blocks ? blocks : max_blocks);
+  if (compute_dtype == LBM355_F64)
+    hipLaunchKernelGGL((BatchFn<double>)k, dim3(grid), dim3(threads), 0, s, (const BatchEntry<double>*)entries,
+                       tile_member, ntiles, swap, xcd_order);
+  else
+    hipLaunchKernelGGL((BatchFn<float>)k, dim3(grid), dim3(threads), 0, s, (const BatchEntry<float>*)entries,
+                       tile_member, ntiles, swap, xcd_order);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return berr(LBM355_ERR_LAUNCH, std::string("launch failed: ") + hipGetErrorString(e));
+  return LBM355_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int lbm355_batch_create(const lbm355_sweep_desc* descs, const void* const* pdf_src, void* const* pdf_dst,
+                        int32_t n_members, lbm355_batch** out) {
+  if (!out) return berr(LBM355_ERR_ARG, "bad batch arguments");
+  *out = nullptr;
+  int kind0 = 0;
+  int rc0 = check_members(descs, pdf_src, pdf_dst, n_members, &kind0);
+  if (rc0) return rc0;
+  const lbm355_sweep_desc& d0 = descs[0];
   lbm355_batch* b = new lbm355_batch();
   b->stencil = d0.stencil;
   b->family = d0.family;
@@ -269,37 +341,8 @@ int lbm355_batch_create(const lbm355_sweep_desc* descs, const void* const* pdf_s
 int lbm355_batch_sweep(const lbm355_batch* b, int32_t step_kind, int32_t threads, int32_t blocks,
                        int32_t xcd_order, void* stream) {
   if (!b) return berr(LBM355_ERR_ARG, "batch is NULL");
-  if (threads != 64 && threads != 128 && threads != 192 && threads != 256)
-    return berr(LBM355_ERR_ARG, "threads per block must be 64, 128, 192 or 256");
-  if (blocks <= 0) return berr(LBM355_ERR_ARG, "blocks must be positive");
-  int pattern, swap = 0;
-  if (b->pattern_kind == LBM355_PULL) {
-    if (step_kind != 0 && step_kind != 1) return berr(LBM355_ERR_ARG, "pull step_kind is 0 (src->dst) or 1 (dst->src)");
-    pattern = LBM355_PULL;
-    swap = step_kind;
-  } else {
-    if (step_kind != LBM355_AA_EVEN && step_kind != LBM355_AA_ODD)
-      return berr(LBM355_ERR_ARG, "AA step_kind is LBM355_AA_EVEN or LBM355_AA_ODD");
-    pattern = step_kind;
-  }
-  lbm355_sweep_desc d{};
-  d.stencil = b->stencil;
-  d.family = b->family;
-  d.store_dtype = b->store_dtype;
-  void* k = b->compute_dtype == LBM355_F64 ? batch_pick<double>(&d, pattern) : batch_pick<float>(&d, pattern);
-  if (!k) return berr(LBM355_ERR_UNSUPPORTED, "no batched kernel");
-  const int64_t max_blocks = (b->ntiles * 64 + threads - 1) / threads;
-  const int grid = (int)(blocks < max_blocks ? blocks : max_blocks);
-  hipStream_t s = (hipStream_t)stream;
-  if (b->compute_dtype == LBM355_F64)
-    hipLaunchKernelGGL((BatchFn<double>)k, dim3(grid), dim3(threads), 0, s, (const BatchEntry<double>*)b->entries,
-                       b->tile_member, b->ntiles, swap, xcd_order);
-  else
-    hipLaunchKernelGGL((BatchFn<float>)k, dim3(grid), dim3(threads), 0, s, (const BatchEntry<float>*)b->entries,
-                       b->tile_member, b->ntiles, swap, xcd_order);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return berr(LBM355_ERR_LAUNCH, std::string("launch failed: ") + hipGetErrorString(e));
-  return LBM355_OK;
+  return launch_batch(b->stencil, b->family, b->store_dtype, b->compute_dtype, b->pattern_kind, b->entries,
+                      b->tile_member, b->ntiles, step_kind, threads, blocks, xcd_order, (hipStream_t)stream);
 }
 
 int lbm355_batch_info(const lbm355_batch* b, int64_t* n_tiles, int64_t* n_cells) {
@@ -307,6 +350,115 @@ int lbm355_batch_info(const lbm355_batch* b, int64_t* n_tiles, int64_t* n_cells)
   if (n_tiles) *n_tiles = b->ntiles;
   if (n_cells) *n_cells = b->ncells;
   return LBM355_OK;
+}
+
+}  // extern "C"
+
+// ---- allocation-free batched step ---------------------------------------------------------------------------
+// lbm355_batch_sweep_static keeps the member table and the tile -> member map in `scratch`, a file-scope device
+// array every GPU the library runs on holds until the process ends, instead of the device memory
+// lbm355_batch_create allocates.  Layout: the member table (up to STATIC_MAX_MEMBERS entries of the larger entry
+// type) at offset 0, the tile map (up to STATIC_MAX_TILES int32) after it.
+namespace {
+constexpr int32_t STATIC_MAX_MEMBERS = 1024;
+constexpr int64_t STATIC_MAX_TILES = int64_t(1) << 21;            // 2^21 tiles of 64 cells: 134,217,728 cells
+constexpr size_t STATIC_TABLE_BYTES = STATIC_MAX_MEMBERS * sizeof(BatchEntry<double>);
+constexpr size_t STATIC_MAP_OFFSET = (STATIC_TABLE_BYTES + 255) / 256 * 256;
+constexpr size_t STATIC_SCRATCH_BYTES = STATIC_MAP_OFFSET + STATIC_MAX_TILES * sizeof(int32_t);
+static_assert(sizeof(BatchEntry<double>) >= sizeof(BatchEntry<float>), "the table is sized for the larger entry");
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[STATIC_SCRATCH_BYTES];
+
+namespace {
+constexpr int STATIC_MAX_DEVICES = 64;
+std::mutex g_static_mutex;                                  // guards the host staging table and the pointers below
+void* g_scratch_ptr[STATIC_MAX_DEVICES] = {};               // device address of `scratch` on each GPU
+alignas(256) unsigned char g_static_table[STATIC_TABLE_BYTES];   // host staging of the member table
+
+// map[t] = the member whose tiles contain tile t (the last member with first_tile <= t)
+template <class TC>
+__global__ void __launch_bounds__(256) static_tile_map_kernel(const BatchEntry<TC>* __restrict__ entries,
+                                                              int32_t n, int32_t* __restrict__ map, int64_t ntiles) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (entries[mid].first_tile <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  map[t] = lo;
+}
+
+// The device address of `scratch` on the current device, looked up once per device (hipGetSymbolAddress: the
+// array's name is a host-side stand-in, not a device address).  Called with g_static_mutex held.
+int scratch_address(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return berr(LBM355_ERR_LAUNCH, std::string("hipGetDevice failed: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= STATIC_MAX_DEVICES) return berr(LBM355_ERR_UNSUPPORTED, "device index beyond 63");
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return berr(LBM355_ERR_LAUNCH, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return LBM355_OK;
+}
+
+template <class TC>
+int static_step(const lbm355_sweep_desc* descs, const void* const* src, void* const* dst, int32_t n, int kind,
+                int32_t step_kind, int32_t threads, int32_t blocks, int32_t xcd_order, hipStream_t s) {
+  BatchEntry<TC>* table = reinterpret_cast<BatchEntry<TC>*>(g_static_table);
+  int64_t tiles = 0, cells = 0;
+  int rc = fill_entries<TC>(descs, src, dst, n, table, &tiles, &cells);
+  if (rc) return rc;
+  if (tiles > STATIC_MAX_TILES)
+    return berr(LBM355_ERR_ARG, "lbm355_batch_sweep_static: the members hold more than 2^21 tiles of 64 cells "
+                                "(134,217,728 cells); use lbm355_batch_create");
+  void* base = nullptr;
+  rc = scratch_address(&base);
+  if (rc) return rc;
+  BatchEntry<TC>* dev_table = reinterpret_cast<BatchEntry<TC>*>(base);
+  int32_t* dev_map = reinterpret_cast<int32_t*>(static_cast<unsigned char*>(base) + STATIC_MAP_OFFSET);
+  // stream-ordered after the previous step's reads of the scratch; the host staging is reused only after the
+  // copy has finished
+  hipError_t e = hipMemcpyAsync(dev_table, table, sizeof(BatchEntry<TC>) * (size_t)n, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess)
+    return berr(LBM355_ERR_LAUNCH, std::string("member-table copy failed: ") + hipGetErrorString(e));
+  hipLaunchKernelGGL(static_tile_map_kernel<TC>, dim3((unsigned)((tiles + 255) / 256)), dim3(256), 0, s, dev_table,
+                     n, dev_map, tiles);
+  e = hipGetLastError();
+  if (e != hipSuccess) return berr(LBM355_ERR_LAUNCH, std::string("tile-map launch failed: ") + hipGetErrorString(e));
+  const lbm355_sweep_desc& d0 = descs[0];
+  return launch_batch(d0.stencil, d0.family, d0.store_dtype, d0.compute_dtype, kind, dev_table, dev_map, tiles,
+                      step_kind, threads, blocks, xcd_order, s);
+}
+}  // namespace
+
+extern "C" {
+
+int lbm355_batch_sweep_static(const lbm355_sweep_desc* descs, const void* const* pdf_src, void* const* pdf_dst,
+                              int32_t n_members, int32_t step_kind, int32_t threads, int32_t blocks,
+                              int32_t xcd_order, void* stream) {
+  int kind = 0;
+  int rc = check_members(descs, pdf_src, pdf_dst, n_members, &kind);
+  if (rc) return rc;
+  rc = check_launch(kind, step_kind, threads, blocks);
+  if (rc) return rc;
+  if (n_members > STATIC_MAX_MEMBERS)
+    return berr(LBM355_ERR_ARG, "lbm355_batch_sweep_static takes at most 1024 members; use lbm355_batch_create");
+  std::lock_guard<std::mutex> lock(g_static_mutex);
+  hipStream_t s = (hipStream_t)stream;
+  if (descs[0].compute_dtype == LBM355_F64)
+    return static_step<double>(descs, pdf_src, pdf_dst, n_members, kind, step_kind, threads, blocks, xcd_order, s);
+  return static_step<float>(descs, pdf_src, pdf_dst, n_members, kind, step_kind, threads, blocks, xcd_order, s);
 }
 
 int lbm355_batch_destroy(lbm355_batch* b) {
