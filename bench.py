@@ -86,17 +86,17 @@ def cpu_baseline(n=64, budget_s=12.0):
                       f"D3Q27 cumulant AA TGV {n}^3, {done} steps in {dt:.1f} s"}
 
 
-PROFILE_COUNTERS = os.path.join(ROOT, "profiles", "sweep_d3q27_cumulant_aa_f32_256_hbm_counters.md")
+PROFILE_COUNTERS = os.path.join(ROOT, "profiles", "sweep_d3q27_cumulant_aa_256_three_forms_counters.md")
 
 
 def profiled_traffic(n, dtype):
-    """HBM bytes per sweep launch from the committed FETCH_SIZE/WRITE_SIZE summary (copy-calibrated),
-    when it covers this workload; otherwise None."""
-    if n != 256 or dtype != "float32" or not os.path.exists(PROFILE_COUNTERS):
+    """HBM bytes per sweep launch from the committed FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), when it
+    covers this workload (256^3, the storage type's line); otherwise None."""
+    if n != 256 or not os.path.exists(PROFILE_COUNTERS):
         return None
     import re
     with open(PROFILE_COUNTERS) as fh:
-        m = re.search(r"traffic_bytes_per_launch\s*=\s*(\d+)", fh.read())
+        m = re.search(r"traffic_bytes_per_launch\[" + re.escape(dtype) + r"\]\s*=\s*(\d+)", fh.read())
     return int(m.group(1)) if m else None
 
 
@@ -141,6 +141,31 @@ def calibration_copies(n, dtype, elem_bytes, reps=5):
     del a_buf, b_buf
     torch.cuda.empty_cache()
     return out
+
+
+def valu_issue_ceiling(reps=5, waves=16384, iters=8192):
+    """Achievable VALU issue rate, in billions of wave instructions per second: lbm355_valu_probe runs `waves`
+    wavefronts of 8 independent FP32 multiply-add chains (8 * iters VALU instructions each), timed with HIP
+    events on the launch stream after the timed region.  The sweeps' SQ_INSTS_VALU per launch divided by their
+    launch time is held against it."""
+    import ctypes
+    import torch
+    from lbmpy_amd import _native as nat
+    lib = nat.load()
+    n = waves * 64
+    out = torch.empty(n, dtype=torch.float32, device="cuda")
+    stream = nat.current_stream_handle()
+    run = lambda: nat.check(lib.lbm355_valu_probe(ctypes.c_void_p(out.data_ptr()), n, iters, stream))
+    run()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        run()
+    e1.record()
+    torch.cuda.synchronize()
+    rate = round(waves * 8.0 * iters * reps / (e0.elapsed_time(e1) / 1e3) / 1e9, 1)
+    print(f"VALU issue ceiling: {rate} G wave-instructions/s ({waves} waves x {8 * iters} FMA)", flush=True)
+    return rate
 
 
 def macroscopic_from_aa_storage(storage, zero_centered=True):
@@ -496,9 +521,9 @@ def main():
     ap.add_argument("--channel-size", type=int, nargs=3, default=[512, 256, 256])
     ap.add_argument("--channel-steps", type=int, default=20000)
     ap.add_argument("--no-calibrate", action="store_true",
-                    help="skip the copy measurements after the timed region (a flat copy of the PDF field's size, "
-                         "which also scales the FETCH_SIZE / WRITE_SIZE counters, and the 27-plane copy of the "
-                         "sweep's access pattern)")
+                    help="skip the calibrations after the timed region (a flat copy of the PDF field's size, "
+                         "which also scales the FETCH_SIZE / WRITE_SIZE counters, the 27-plane copy of the "
+                         "sweep's access pattern, and the VALU issue probe)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (pass --gpus 1)", file=sys.stderr)
@@ -556,6 +581,7 @@ def main():
 
     head = results[args.dtype]
     copy_gbs = None if args.no_calibrate else calibration_copies(n, args.dtype, size[args.dtype])
+    valu_ceiling = None if args.no_calibrate else valu_issue_ceiling()
     cpu = None if args.no_cpu else cpu_baseline()
     b = head["bytes_per_cell_step"]
     achieved = b * cells / (head["_launch_ms"] / 1e3) / 1e9
@@ -575,7 +601,8 @@ def main():
                      "kernel": "lbm355::sweep_kernel<D3Q27, cumulant, AA odd/even> (both parities, same bytes)",
                      "launch_ms": round(head["_launch_ms"], 4), "kernel_ms": head["kernel_ms"],
                      "algorithmic_bytes_per_launch": b * cells, "copy_gbs": copy_gbs,
-                     "frac_of_plane27_copy": (round(achieved / copy_gbs["plane27"], 4) if copy_gbs else None)},
+                     "frac_of_plane27_copy": (round(achieved / copy_gbs["plane27"], 4) if copy_gbs else None),
+                     "valu_issue_ceiling_ginst_per_s": valu_ceiling},
         "forms": forms_out,
         "cpu_baseline": cpu,
         "finite": all(v["finite"] for v in results.values()),

@@ -89,6 +89,12 @@ extern "C" {
 #define LBM355_FLAG_GENERAL_CUMULANT 1024 /* D3Q27 cumulant: a cumulant set other than the default polynomial
                                              one (lbmpy MONOMIAL_CUMULANT, custom nested_moments): the relaxation
                                              comes as the table's linear maps on groups of monomial cumulants */
+#define LBM355_FLAG_HIGHER_ORDERS_RATE1 2048 /* D3Q27 cumulant, default set: the table's rates of every cumulant of
+                                                order >= 3 are 1 (lbmpy's default when a method is given by its
+                                                shear rate), so the sweep sets those cumulants to equilibrium
+                                                without computing them.  The host sets it after checking the rates;
+                                                set with other rates, the sweep still relaxes them with rate 1.
+                                                Members of one batch must agree on it. */
 #define LBM355_FLAG_ADAPT_VELOCITY 256 /* boundary links, UBB: drive the wall at its equilibrium input velocity
                                           u_w - F / (2 rho_ref), rho_ref the link's density (the UBB's own, or
                                           the fluid cell's for compressible methods) or 1 for incompressible ones
@@ -195,6 +201,11 @@ int lbm355_get_strain_rate(const lbm355_sweep_desc* desc, const void* pdf, const
  * the sweeps use them.  No lbmpy counterpart: a copy of known size that calibrates achievable
  * bandwidth and the FETCH_SIZE / WRITE_SIZE counters against the sweep. */
 int lbm355_stream_copy(const void* src, void* dst, int64_t n_floats, int32_t nontemporal, void* stream);
+
+/* n_threads threads (n_threads / 64 waves) each running 8 independent FP32 multiply-add chains of `iters` steps,
+ * 8 * iters VALU instructions per wave, the sums written to out[n_threads].  No lbmpy counterpart: it measures
+ * the achievable VALU issue rate that the sweeps' instruction counts are held against. */
+int lbm355_valu_probe(float* out, int64_t n_threads, int32_t iters, void* stream);
 
 /* Copy of n_planes (19 or 27) planes of plane_elems elements each, plane q at q * plane_stride elements, with
  * one thread per element index loading that element of every plane and then storing them all: the sweep's

@@ -24,7 +24,7 @@ F64, F32, F16 = 0, 1, 2
 FACE_GHOST, FACE_PERIODIC, FACE_NOSLIP = 0, 1, 2
 FLAG_ZERO_CENTERED, FLAG_COMPRESSIBLE, FLAG_EQ_SHIFT, FLAG_FORCE, FLAG_SMAGORINSKY = 1, 2, 4, 8, 16
 FLAG_LEARNED_CLOSURE, FLAG_GALILEAN, FLAG_FOURTH_ORDER, FLAG_ADAPT_VELOCITY = 32, 64, 128, 256
-FLAG_STOCHASTIC_ROUNDING, FLAG_GENERAL_CUMULANT = 512, 1024
+FLAG_STOCHASTIC_ROUNDING, FLAG_GENERAL_CUMULANT, FLAG_HIGHER_ORDERS_RATE1 = 512, 1024, 2048
 
 DTYPE_CODES = {"float64": F64, "float32": F32, "float16": F16}
 STENCIL_NAMES = {"D2Q9": D2Q9, "D3Q19": D3Q19, "D3Q27": D3Q27}
@@ -63,7 +63,7 @@ EXPORTED = ("lbm355_sweep", "lbm355_table_elements", "lbm355_noslip_links", "lbm
             "lbm355_batch_info", "lbm355_batch_destroy", "lbm355_batch_sweep_static", "lbm355_closure_params_size",
             "lbm355_closure_master_size", "lbm355_closure_features", "lbm355_closure_forward",
             "lbm355_closure_train_step", "lbm355_closure_adam", "lbm355_boundary_links", "lbm355_copy_boxes",
-            "lbm355_pattern_access", "lbm355_get_strain_rate", "lbm355_plane_copy",
+            "lbm355_pattern_access", "lbm355_get_strain_rate", "lbm355_plane_copy", "lbm355_valu_probe",
             "lbm355_source_hash")
 
 _lib = None
@@ -98,6 +98,8 @@ def load(path=None):
     lib.lbm355_stream_copy.restype = c_int32
     lib.lbm355_plane_copy.argtypes = [c_int32, c_int32, c_void_p, c_void_p, c_int64, c_int64, c_int32, c_void_p]
     lib.lbm355_plane_copy.restype = c_int32
+    lib.lbm355_valu_probe.argtypes = [c_void_p, c_int64, c_int32, c_void_p]
+    lib.lbm355_valu_probe.restype = c_int32
     lib.lbm355_batch_create.argtypes = [ctypes.POINTER(SweepDesc), ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
                                         c_int32, ctypes.POINTER(c_void_p)]
     lib.lbm355_batch_create.restype = c_int32

@@ -157,6 +157,8 @@ void* batch_pick(const lbm355_sweep_desc* d, int pattern) {
     if (d->stencil == LBM355_D3Q19) return batch_pick_store<D3Q19, LBM355_FAMILY_MOMENT, TC>(d->store_dtype, pattern);
     if (d->stencil == LBM355_D3Q27) return batch_pick_store<D3Q27, LBM355_FAMILY_MOMENT, TC>(d->store_dtype, pattern);
   } else if (d->family == LBM355_FAMILY_CUMULANT && d->stencil == LBM355_D3Q27) {
+    if (d->flags & LBM355_FLAG_HIGHER_ORDERS_RATE1)
+      return batch_pick_store<D3Q27, FAMILY_CUMULANT_EQ3, TC>(d->store_dtype, pattern);
     return batch_pick_store<D3Q27, LBM355_FAMILY_CUMULANT, TC>(d->store_dtype, pattern);
   } else if (d->family == LBM355_FAMILY_CENTRAL) {
     if (d->stencil == LBM355_D2Q9) return batch_pick_store<D2Q9, LBM355_FAMILY_CENTRAL, TC>(d->store_dtype, pattern);
@@ -172,6 +174,7 @@ using namespace lbm355;
 
 struct lbm355_batch {
   int32_t stencil, family, store_dtype, compute_dtype, pattern_kind;  // pattern_kind: PULL or AA
+  int32_t variant_flags;  // the kernel-selecting flags the members share (LBM355_FLAG_HIGHER_ORDERS_RATE1)
   int32_t nmembers;
   int64_t ntiles;
   int64_t ncells;
@@ -250,6 +253,8 @@ int check_members(const lbm355_sweep_desc* descs, const void* const* pdf_src, vo
     if (d.stencil != d0.stencil || d.family != d0.family || d.store_dtype != d0.store_dtype ||
         d.compute_dtype != d0.compute_dtype || (d.pattern == LBM355_PULL ? LBM355_PULL : LBM355_AA_EVEN) != kind0)
       return berr(LBM355_ERR_ARG, "batch members must share stencil, family, streaming pattern and dtypes");
+    if ((d.flags ^ d0.flags) & LBM355_FLAG_HIGHER_ORDERS_RATE1)
+      return berr(LBM355_ERR_ARG, "batch members must agree on LBM355_FLAG_HIGHER_ORDERS_RATE1");
     if (d.flags & (LBM355_FLAG_SMAGORINSKY | LBM355_FLAG_LEARNED_CLOSURE))
       return berr(LBM355_ERR_UNSUPPORTED, "subgrid models are not batched");
     if (table_pattern(d.pattern)) return berr(LBM355_ERR_UNSUPPORTED, "batched ensembles stream with pull or AA");
@@ -284,8 +289,8 @@ int check_launch(int32_t pattern_kind, int32_t step_kind, int32_t threads, int32
 
 // The launch of one batched step (shared by the allocating and the allocation-free entry points).
 int launch_batch(int32_t stencil, int32_t family, int32_t store_dtype, int32_t compute_dtype, int32_t pattern_kind,
-                 const void* entries, const int32_t* tile_member, int64_t ntiles, int32_t step_kind, int32_t threads,
-                 int32_t blocks, int32_t xcd_order, hipStream_t s) {
+                 int32_t variant_flags, const void* entries, const int32_t* tile_member, int64_t ntiles,
+                 int32_t step_kind, int32_t threads, int32_t blocks, int32_t xcd_order, hipStream_t s) {
   int rc = check_launch(pattern_kind, step_kind, threads, blocks);
   if (rc) return rc;
   const int pattern = pattern_kind == LBM355_PULL ? LBM355_PULL : step_kind;
@@ -294,6 +299,7 @@ int launch_batch(int32_t stencil, int32_t family, int32_t store_dtype, int32_t c
   d.stencil = stencil;
   d.family = family;
   d.store_dtype = store_dtype;
+  d.flags = variant_flags;
   void* k = compute_dtype == LBM355_F64 ? batch_pick<double>(&d, pattern) : batch_pick<float>(&d, pattern);
   if (!k) return berr(LBM355_ERR_UNSUPPORTED, "no batched kernel");
   const int64_t max_blocks = (ntiles * 64 + threads - 1) / threads;
@@ -326,6 +332,7 @@ int lbm355_batch_create(const lbm355_sweep_desc* descs, const void* const* pdf_s
   b->store_dtype = d0.store_dtype;
   b->compute_dtype = d0.compute_dtype;
   b->pattern_kind = kind0;
+  b->variant_flags = d0.flags & LBM355_FLAG_HIGHER_ORDERS_RATE1;
   b->nmembers = n_members;
   (void)hipGetDevice(&b->device);
   int rc = d0.compute_dtype == LBM355_F64 ? upload<double>(descs, pdf_src, pdf_dst, n_members, b)
@@ -341,8 +348,9 @@ int lbm355_batch_create(const lbm355_sweep_desc* descs, const void* const* pdf_s
 int lbm355_batch_sweep(const lbm355_batch* b, int32_t step_kind, int32_t threads, int32_t blocks,
                        int32_t xcd_order, void* stream) {
   if (!b) return berr(LBM355_ERR_ARG, "batch is NULL");
-  return launch_batch(b->stencil, b->family, b->store_dtype, b->compute_dtype, b->pattern_kind, b->entries,
-                      b->tile_member, b->ntiles, step_kind, threads, blocks, xcd_order, (hipStream_t)stream);
+  return launch_batch(b->stencil, b->family, b->store_dtype, b->compute_dtype, b->pattern_kind, b->variant_flags,
+                      b->entries, b->tile_member, b->ntiles, step_kind, threads, blocks, xcd_order,
+                      (hipStream_t)stream);
 }
 
 int lbm355_batch_info(const lbm355_batch* b, int64_t* n_tiles, int64_t* n_cells) {
@@ -437,7 +445,8 @@ int static_step(const lbm355_sweep_desc* descs, const void* const* src, void* co
   e = hipGetLastError();
   if (e != hipSuccess) return berr(LBM355_ERR_LAUNCH, std::string("tile-map launch failed: ") + hipGetErrorString(e));
   const lbm355_sweep_desc& d0 = descs[0];
-  return launch_batch(d0.stencil, d0.family, d0.store_dtype, d0.compute_dtype, kind, dev_table, dev_map, tiles,
+  return launch_batch(d0.stencil, d0.family, d0.store_dtype, d0.compute_dtype, kind,
+                      d0.flags & LBM355_FLAG_HIGHER_ORDERS_RATE1, dev_table, dev_map, tiles,
                       step_kind, threads, blocks, xcd_order, s);
 }
 }  // namespace

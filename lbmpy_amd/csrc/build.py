@@ -17,7 +17,8 @@ LIBNAME = "liblbm355.so"
 SOURCES = ["sweep.hip", "sweep_moment_f64.hip", "sweep_moment_f32.hip", "sweep_central_f64.hip",
            "sweep_central_f32.hip", "sweep_cumulant_f64.hip", "sweep_cumulant_f32.hip",
            "sweep_cumulant_corrected_f64.hip", "sweep_cumulant_corrected_f32.hip",
-           "sweep_cumulant_general_f64.hip", "sweep_cumulant_general_f32.hip", "sweep_cumulant_learned_f32.hip",
+           "sweep_cumulant_general_f64.hip", "sweep_cumulant_general_f32.hip", "sweep_cumulant_eq3_f64.hip",
+           "sweep_cumulant_eq3_f32.hip", "sweep_cumulant_learned_f32.hip",
            "sweep_eso.hip", "macroscopic.hip", "batch.hip", "closure.hip", "periodic.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -47,6 +47,34 @@ __device__ __forceinline__ T beta(const T (&u)[3], const T (&u2)[3]) {
   return r;
 }
 
+// gamma_e of a monomial as a compile-time number
+template <int E>
+constexpr double gam_v = E == 0 ? 1.0 : (E == 1 ? 0.0 : 1.0 / 3.0);
+
+// Central moment about u of the post-collision deviation from its normalised deviation v: rho v for f* - f_eq
+// (NEQ), rho v + drho gamma_e - beta_e for f* - w.  The drho term is left out where gamma_e = 0 (an odd exponent).
+template <int EX, int EY, int EZ, bool NEQ, class T>
+__device__ __forceinline__ T post_moment(T v, T rho, T drho, const T (&u)[3], const T (&u2)[3]) {
+  constexpr double g = gam_v<EX> * gam_v<EY> * gam_v<EZ>;
+  if constexpr (NEQ)
+    return rho * v;
+  else if constexpr (g == 0.0)
+    return rho * v - beta<EX, EY, EZ>(u, u2);
+  else
+    return rho * v + drho * T(g) - beta<EX, EY, EZ>(u, u2);
+}
+// the same for a deviation that vanishes (v = 0)
+template <int EX, int EY, int EZ, bool NEQ, class T>
+__device__ __forceinline__ T post_moment_zero(T drho, const T (&u)[3], const T (&u2)[3]) {
+  constexpr double g = gam_v<EX> * gam_v<EY> * gam_v<EZ>;
+  if constexpr (NEQ)
+    return T(0);
+  else if constexpr (g == 0.0)
+    return -beta<EX, EY, EZ>(u, u2);
+  else
+    return drho * T(g) - beta<EX, EY, EZ>(u, u2);
+}
+
 // raw chimera along one axis: values at c = -1, 0, +1  ->  sums of c^0, c^1, c^2
 template <class T>
 __device__ __forceinline__ void chim_fwd(T& a, T& b, T& c) {
@@ -113,7 +141,12 @@ struct CumulantTab {
 // bounds) and FP64 arithmetic do not.
 // GENERAL: a cumulant set other than lbmpy's default polynomial one; its relaxation comes as the linear maps
 // of the table's BLOCKS (P^-1 (I - D) P restricted to each group of monomials), not as per-polynomial rates.
-template <class T, bool LEARNED = false, bool CORRECTED = false, bool NEQ = false, bool GENERAL = false>
+// EQ3: every cumulant of order >= 3 relaxes with rate 1 (lbmpy's default for a method given by its shear rate, and
+// LBM355_FLAG_HIGHER_ORDERS_RATE1), so it is set to its equilibrium, zero, without being computed: the transforms
+// and cumulant relations of orders >= 3 drop out before the collision, and after it the central moments of orders
+// 3 and 5 are those of the equilibrium and those of orders 4 and 6 are products of the second-order cumulants.
+template <class T, bool LEARNED = false, bool CORRECTED = false, bool NEQ = false, bool GENERAL = false,
+          bool EQ3 = false>
 __device__ __forceinline__ void collide_cumulant_d3q27(T (&f)[27], const SweepArgs<T>& a, T& rho_out_v,
                                                        T (&u_out_v)[3], CellExtras<T>& ex) {
   using S = D3Q27;
@@ -346,6 +379,18 @@ __device__ __forceinline__ void collide_cumulant_d3q27(T (&f)[27], const SweepAr
     mv3(CumulantTab::B4O, k211, k121, k112, c211, c121, c112);
     mv3(CumulantTab::B5, k221, k212, k122, c221, c212, c122);
     c222 = b[CumulantTab::B6] * k222;
+  } else if constexpr (EQ3) {
+    c110 = (T(1) - rs[0]) * mu110;
+    c101 = (T(1) - rs[1]) * mu101;
+    c011 = (T(1) - rs[2]) * mu011;
+    const T d1 = (T(1) - rs[3]) * (mu200 - mu020);
+    const T d2 = (T(1) - rs[4]) * (mu200 - mu002);
+    const T tr = (T(1) - r[9]) * (mu200 + mu020 + mu002);
+    c200 = (d1 + d2 + tr) * third;
+    c020 = (tr - T(2) * d1 + d2) * third;
+    c002 = (tr + d1 - T(2) * d2) * third;
+    c120 = c102 = c210 = c012 = c201 = c021 = c111 = T(0);
+    c220 = c202 = c022 = c211 = c121 = c112 = c221 = c212 = c122 = c222 = T(0);
   } else {
     c110 = (T(1) - rs[0]) * mu110;
     c101 = (T(1) - rs[1]) * mu101;
@@ -409,36 +454,12 @@ __device__ __forceinline__ void collide_cumulant_d3q27(T (&f)[27], const SweepAr
     c222 = (T(1) - r[26]) * k222;
   }
 
-  // ---- back to normalised central-moment deviations
-  const T n220 = c220 + (c020 + c200) * third + c020 * c200 + T(2) * c110 * c110;
-  const T n202 = c202 + (c002 + c200) * third + c002 * c200 + T(2) * c101 * c101;
-  const T n022 = c022 + (c002 + c020) * third + c002 * c020 + T(2) * c011 * c011;
-  const T n211 = c211 + c011 * third + c200 * c011 + T(2) * c101 * c110;
-  const T n121 = c121 + c101 * third + c020 * c101 + T(2) * c011 * c110;
-  const T n112 = c112 + c110 * third + c002 * c110 + T(2) * c011 * c101;
-  const T n221 = c221 + (c021 + c201) * third + c020 * c201 + c200 * c021 + T(4) * c110 * c111 +
-                 T(2) * (c011 * c210 + c101 * c120);
-  const T n212 = c212 + (c012 + c210) * third + c002 * c210 + c200 * c012 + T(4) * c101 * c111 +
-                 T(2) * (c011 * c201 + c102 * c110);
-  const T n122 = c122 + (c102 + c120) * third + c002 * c120 + c020 * c102 + T(4) * c011 * c111 +
-                 T(2) * (c012 * c110 + c021 * c101);
-  const T n222 = c222 + (c202 + c220 + c022) * third + (c200 + c020 + c002) * ninth +
-                 (c002 * c220 + c020 * c202 + c200 * c022) +
-                 (c002 * c020 + c002 * c200 + c020 * c200) * third + c002 * c020 * c200 +
-                 T(2) * (c002 * c110 * c110 + c200 * c011 * c011 + c020 * c101 * c101) +
-                 T(2.0 / 3.0) * (c011 * c011 + c101 * c101 + c110 * c110) + T(8) * c011 * c101 * c110 +
-                 T(4) * (c011 * c211 + c101 * c121 + c110 * c112) +
-                 T(2) * (c012 * c210 + c021 * c201 + c102 * c120) + T(4) * c111 * c111;
-
   // ---- central moments of the post-collision deviation f~* about u
   // NEQ: of the post-collision f* - f_eq.  Its zeroth and first moments are kept from before the collision, the
   // first ones plus the force F: exactly they are 0 and -F/2 -> +F/2, but in FP32 they also hold what the
   // rounded f_eq(rho, u) misses of the cell's mass and momentum (u = (j + F/2) / rho is rounded), so keeping them
   // conserves both to the rounding of the stored populations.  Rebuilding them from (rho, u) would instead add
   // that rounding error, which repeats from step to step in a slowly varying flow, to the momentum every step.
-#define LBM_KT(EX, EY, EZ, V)                                                                                    \
-  m[EX][EY][EZ] = NEQ ? rho * (V)                                                                                \
-                      : rho * (V) + drho * (gam<EX, T>() * gam<EY, T>() * gam<EZ, T>()) - beta<EX, EY, EZ>(u, u2)
   if constexpr (NEQ) {
     m[1][0][0] += a.force[0];
     m[0][1][0] += a.force[1];
@@ -449,15 +470,57 @@ __device__ __forceinline__ void collide_cumulant_d3q27(T (&f)[27], const SweepAr
     m[0][1][0] = hF[1] + u[1];
     m[0][0][1] = hF[2] + u[2];
   }
+#define LBM_KT(EX, EY, EZ, V) m[EX][EY][EZ] = post_moment<EX, EY, EZ, NEQ>(V, rho, drho, u, u2)
+#define LBM_KT0(EX, EY, EZ) m[EX][EY][EZ] = post_moment_zero<EX, EY, EZ, NEQ>(drho, u, u2)
   LBM_KT(2, 0, 0, c200); LBM_KT(0, 2, 0, c020); LBM_KT(0, 0, 2, c002);
   LBM_KT(1, 1, 0, c110); LBM_KT(1, 0, 1, c101); LBM_KT(0, 1, 1, c011);
-  LBM_KT(1, 2, 0, c120); LBM_KT(1, 0, 2, c102); LBM_KT(2, 1, 0, c210);
-  LBM_KT(0, 1, 2, c012); LBM_KT(2, 0, 1, c201); LBM_KT(0, 2, 1, c021);
-  LBM_KT(1, 1, 1, c111);
-  LBM_KT(2, 2, 0, n220); LBM_KT(2, 0, 2, n202); LBM_KT(0, 2, 2, n022);
-  LBM_KT(2, 1, 1, n211); LBM_KT(1, 2, 1, n121); LBM_KT(1, 1, 2, n112);
-  LBM_KT(2, 2, 1, n221); LBM_KT(2, 1, 2, n212); LBM_KT(1, 2, 2, n122);
-  LBM_KT(2, 2, 2, n222);
+  // normalised central-moment deviations of orders 4 and 6 from the cumulants (with EQ3 the cumulants of
+  // orders >= 3 are zero and only the products of the second-order ones remain)
+  if constexpr (EQ3) {
+    const T n220 = (c020 + c200) * third + c020 * c200 + T(2) * c110 * c110;
+    const T n202 = (c002 + c200) * third + c002 * c200 + T(2) * c101 * c101;
+    const T n022 = (c002 + c020) * third + c002 * c020 + T(2) * c011 * c011;
+    const T n211 = c011 * third + c200 * c011 + T(2) * c101 * c110;
+    const T n121 = c101 * third + c020 * c101 + T(2) * c011 * c110;
+    const T n112 = c110 * third + c002 * c110 + T(2) * c011 * c101;
+    const T n222 = (c200 + c020 + c002) * ninth + (c002 * c020 + c002 * c200 + c020 * c200) * third +
+                   c002 * c020 * c200 + T(2) * (c002 * c110 * c110 + c200 * c011 * c011 + c020 * c101 * c101) +
+                   T(2.0 / 3.0) * (c011 * c011 + c101 * c101 + c110 * c110) + T(8) * c011 * c101 * c110;
+    LBM_KT0(1, 2, 0); LBM_KT0(1, 0, 2); LBM_KT0(2, 1, 0); LBM_KT0(0, 1, 2); LBM_KT0(2, 0, 1); LBM_KT0(0, 2, 1);
+    LBM_KT0(1, 1, 1);
+    LBM_KT(2, 2, 0, n220); LBM_KT(2, 0, 2, n202); LBM_KT(0, 2, 2, n022);
+    LBM_KT(2, 1, 1, n211); LBM_KT(1, 2, 1, n121); LBM_KT(1, 1, 2, n112);
+    LBM_KT0(2, 2, 1); LBM_KT0(2, 1, 2); LBM_KT0(1, 2, 2);
+    LBM_KT(2, 2, 2, n222);
+  } else {
+    const T n220 = c220 + (c020 + c200) * third + c020 * c200 + T(2) * c110 * c110;
+    const T n202 = c202 + (c002 + c200) * third + c002 * c200 + T(2) * c101 * c101;
+    const T n022 = c022 + (c002 + c020) * third + c002 * c020 + T(2) * c011 * c011;
+    const T n211 = c211 + c011 * third + c200 * c011 + T(2) * c101 * c110;
+    const T n121 = c121 + c101 * third + c020 * c101 + T(2) * c011 * c110;
+    const T n112 = c112 + c110 * third + c002 * c110 + T(2) * c011 * c101;
+    const T n221 = c221 + (c021 + c201) * third + c020 * c201 + c200 * c021 + T(4) * c110 * c111 +
+                   T(2) * (c011 * c210 + c101 * c120);
+    const T n212 = c212 + (c012 + c210) * third + c002 * c210 + c200 * c012 + T(4) * c101 * c111 +
+                   T(2) * (c011 * c201 + c102 * c110);
+    const T n122 = c122 + (c102 + c120) * third + c002 * c120 + c020 * c102 + T(4) * c011 * c111 +
+                   T(2) * (c012 * c110 + c021 * c101);
+    const T n222 = c222 + (c202 + c220 + c022) * third + (c200 + c020 + c002) * ninth +
+                   (c002 * c220 + c020 * c202 + c200 * c022) +
+                   (c002 * c020 + c002 * c200 + c020 * c200) * third + c002 * c020 * c200 +
+                   T(2) * (c002 * c110 * c110 + c200 * c011 * c011 + c020 * c101 * c101) +
+                   T(2.0 / 3.0) * (c011 * c011 + c101 * c101 + c110 * c110) + T(8) * c011 * c101 * c110 +
+                   T(4) * (c011 * c211 + c101 * c121 + c110 * c112) +
+                   T(2) * (c012 * c210 + c021 * c201 + c102 * c120) + T(4) * c111 * c111;
+    LBM_KT(1, 2, 0, c120); LBM_KT(1, 0, 2, c102); LBM_KT(2, 1, 0, c210);
+    LBM_KT(0, 1, 2, c012); LBM_KT(2, 0, 1, c201); LBM_KT(0, 2, 1, c021);
+    LBM_KT(1, 1, 1, c111);
+    LBM_KT(2, 2, 0, n220); LBM_KT(2, 0, 2, n202); LBM_KT(0, 2, 2, n022);
+    LBM_KT(2, 1, 1, n211); LBM_KT(1, 2, 1, n121); LBM_KT(1, 1, 2, n112);
+    LBM_KT(2, 2, 1, n221); LBM_KT(2, 1, 2, n212); LBM_KT(1, 2, 2, n122);
+    LBM_KT(2, 2, 2, n222);
+  }
+#undef LBM_KT0
 #undef LBM_KT
 
   // raw moments (shift by +u), then populations (inverse chimera)

@@ -465,6 +465,34 @@ extern "C" int lbm355_plane_copy(int32_t dtype, int32_t n_planes, const void* sr
   }
 }
 
+namespace {
+// VALU issue ceiling: 8 independent FP32 multiply-add chains of `iters` steps per thread (8 * iters v_fma_f32
+// per wave), their sum stored so that nothing is dropped.  x -> 0.999 x + 0.001 keeps the values bounded.
+__global__ void __launch_bounds__(256) valu_probe_kernel(float* __restrict__ out, int64_t n, int32_t iters) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  float x[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) x[k] = (float)(t & 255) * 1e-3f + (float)k;
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = fmaf(x[k], 0.999f, 1e-3f);
+  }
+  float sum = 0.f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) sum += x[k];
+  out[t] = sum;
+}
+}  // namespace
+
+extern "C" int lbm355_valu_probe(float* out, int64_t n_threads, int32_t iters, void* stream) {
+  if (!out || n_threads < 0 || iters < 0) return lbm355_set_error(LBM355_ERR_ARG, "bad VALU-probe arguments");
+  if (n_threads == 0) return LBM355_OK;
+  const int64_t blocks = (n_threads + 255) / 256;
+  hipLaunchKernelGGL(valu_probe_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, out, n_threads, iters);
+  return hipGetLastError() == hipSuccess ? LBM355_OK : lbm355_set_error(LBM355_ERR_LAUNCH, "VALU probe launch failed");
+}
+
 extern "C" int lbm355_stream_copy(const void* src, void* dst, int64_t n_floats, int32_t nontemporal, void* stream) {
   if (!src || !dst || n_floats < 0) return lbm355_set_error(LBM355_ERR_ARG, "bad copy arguments");
   if (n_floats == 0) return LBM355_OK;

@@ -360,6 +360,9 @@ void* pick_kernel(const lbm355_sweep_desc* d, bool walls, bool rowlds) {
     if (d->flags & LBM355_FLAG_GENERAL_CUMULANT)
       return f64 ? pick_sweep_cumulant_general_f64(st, sd, p, walls, rowlds)
                  : pick_sweep_cumulant_general_f32(st, sd, p, walls, rowlds);
+    if (d->flags & LBM355_FLAG_HIGHER_ORDERS_RATE1)
+      return f64 ? pick_sweep_cumulant_eq3_f64(st, sd, p, walls, rowlds)
+                 : pick_sweep_cumulant_eq3_f32(st, sd, p, walls, rowlds);
     return f64 ? pick_sweep_cumulant_f64(st, sd, p, walls, rowlds) : pick_sweep_cumulant_f32(st, sd, p, walls, rowlds);
   }
   return nullptr;
@@ -571,6 +574,12 @@ int lbm355_sweep(const lbm355_sweep_desc* d, const void* pdf_src, void* pdf_dst,
                     LBM355_FLAG_FOURTH_ORDER))))
     return fail(LBM355_ERR_UNSUPPORTED, "cumulant sets other than the default polynomial one run without subgrid "
                                         "models and corrections");
+  if ((d->flags & LBM355_FLAG_HIGHER_ORDERS_RATE1) &&
+      (d->family != LBM355_FAMILY_CUMULANT ||
+       (d->flags & (LBM355_FLAG_GENERAL_CUMULANT | LBM355_FLAG_LEARNED_CLOSURE | LBM355_FLAG_GALILEAN |
+                    LBM355_FLAG_FOURTH_ORDER))))
+    return fail(LBM355_ERR_UNSUPPORTED, "LBM355_FLAG_HIGHER_ORDERS_RATE1 is for the default cumulant set without "
+                                        "the learned closure and the corrections");
   if ((d->flags & LBM355_FLAG_STOCHASTIC_ROUNDING) && (d->store_dtype != LBM355_F16 || d->compute_dtype != LBM355_F32))
     return fail(LBM355_ERR_ARG, "stochastic rounding is for FP16 storage with FP32 arithmetic");
   if ((d->omega_out || d->eddy_viscosity_out) && !(d->flags & LBM355_FLAG_SMAGORINSKY))
@@ -689,7 +698,7 @@ int lbm355_reduce_conserved(const lbm355_sweep_desc* d, const void* pdf, double*
 
 const char* lbm355_last_error(void) { return g_err.c_str(); }
 
-int32_t lbm355_version(void) { return 6; }
+int32_t lbm355_version(void) { return 7; }
 
 #ifndef LBM355_SOURCE_HASH
 #define LBM355_SOURCE_HASH "unknown"

@@ -27,6 +27,8 @@ void* eso_pick(const lbm355_sweep_desc* d, bool rows) {
     if (d->flags & LBM355_FLAG_GENERAL_CUMULANT)
       return f64 ? eso_pick_cumulant_general_f64(sten, st, pat, rows)
                  : eso_pick_cumulant_general_f32(sten, st, pat, rows);
+    if (d->flags & LBM355_FLAG_HIGHER_ORDERS_RATE1)
+      return f64 ? eso_pick_cumulant_eq3_f64(sten, st, pat, rows) : eso_pick_cumulant_eq3_f32(sten, st, pat, rows);
     return f64 ? eso_pick_cumulant_f64(sten, st, pat, rows) : eso_pick_cumulant_f32(sten, st, pat, rows);
   }
   return nullptr;

@@ -170,5 +170,7 @@ LBM355_ESO_PICKER(eso_pick_cumulant_corrected_f64);
 LBM355_ESO_PICKER(eso_pick_cumulant_corrected_f32);
 LBM355_ESO_PICKER(eso_pick_cumulant_general_f64);
 LBM355_ESO_PICKER(eso_pick_cumulant_general_f32);
+LBM355_ESO_PICKER(eso_pick_cumulant_eq3_f64);
+LBM355_ESO_PICKER(eso_pick_cumulant_eq3_f32);
 
 }  // namespace lbm355

@@ -183,6 +183,8 @@ LBM355_PICKER(pick_sweep_cumulant_corrected_f64);
 LBM355_PICKER(pick_sweep_cumulant_corrected_f32);
 LBM355_PICKER(pick_sweep_cumulant_general_f64);
 LBM355_PICKER(pick_sweep_cumulant_general_f32);
+LBM355_PICKER(pick_sweep_cumulant_eq3_f64);
+LBM355_PICKER(pick_sweep_cumulant_eq3_f32);
 LBM355_PICKER(pick_sweep_cumulant_learned_f32);
 
 }  // namespace lbm355

@@ -1,0 +1,85 @@
+"""The D3Q27 cumulant sweep with every cumulant of order >= 3 relaxed with rate 1 (lbmpy's default for a method
+given by its shear rate) runs a kernel variant that sets those cumulants to equilibrium without computing them.
+It must agree with the per-rate kernel, which computes and relaxes them, in every storage form and streaming
+pattern, and with the FP64 oracle; methods with other higher-order rates must not take it."""
+import numpy as np
+import pytest
+
+from lbmpy_amd import LBMConfig, LBStencil, Stencil, Method, create_lb_method
+from lbmpy_amd import _native as nat
+from tests.lbm_helpers import seeded_state, oracle_run, gpu_kernels, gpu_run
+
+pytestmark = pytest.mark.gpu
+
+PERIODIC = {0: "periodic", 1: "periodic", 2: "periodic"}
+
+
+def _kw(pattern, **extra):
+    return dict(stencil=LBStencil(Stencil.D3Q27), method=Method.CUMULANT, relaxation_rate=1.93, compressible=True,
+                streaming_pattern=pattern, force=(2e-5, -1e-5, 1e-5), **extra)
+
+
+def _flags(kernel, shape, dtype):
+    from lbmpy_amd.sweep import create_pdf_array
+    return kernel.descriptor(create_pdf_array(shape, 27, 1, dtype), {}).flags
+
+
+@pytest.mark.parametrize("data_type,tol", [("float64", 1e-15), ("float32", 2e-7), ("float16", 2e-3)])
+@pytest.mark.parametrize("pattern", ["pull", "aa"])
+def test_shortcut_matches_per_rate_kernel(pattern, data_type, tol):
+    kw = _kw(pattern)
+    m = create_lb_method(LBMConfig(**kw))
+    shape = (64, 6, 5) if pattern == "aa" else (10, 7, 6)
+    f0 = seeded_state(m, shape, seed=12, du=0.04, noise=2e-4)
+    fast = gpu_kernels(kw, data_type=data_type)
+    slow = gpu_kernels(kw, data_type=data_type)
+    for k in slow:
+        k.higher_orders_shortcut = False
+    assert all(_flags(k, shape, data_type) & nat.FLAG_HIGHER_ORDERS_RATE1 for k in fast)
+    assert not any(_flags(k, shape, data_type) & nat.FLAG_HIGHER_ORDERS_RATE1 for k in slow)
+    print(f"rate-1 shortcut vs per-rate kernel: {pattern} {data_type}", flush=True)
+    a = gpu_run(fast, m, shape, PERIODIC, f0, 10, data_type=data_type)
+    b = gpu_run(slow, m, shape, PERIODIC, f0, 10, data_type=data_type)
+    # FP16 storage: a one-ulp difference in FP32 arithmetic may round a population to the neighbouring FP16 value
+    assert np.abs(a - b).max() <= tol
+
+
+@pytest.mark.parametrize("pattern", ["pull", "aa"])
+def test_shortcut_against_oracle_with_walls_fp64(pattern):
+    kw = _kw(pattern)
+    m = create_lb_method(LBMConfig(**kw))
+    shape = (9, 8, 7)
+    faces = {0: "periodic", 1: "noslip", 2: "noslip"}
+    f0 = seeded_state(m, shape, seed=13, du=0.03)
+    ref, _ = oracle_run(m, shape, faces, pattern, f0, 12)
+    ks = gpu_kernels(kw)
+    assert all(_flags(k, shape, "float64") & nat.FLAG_HIGHER_ORDERS_RATE1 for k in ks)
+    print(f"rate-1 shortcut vs oracle, {pattern}, noslip y/z", flush=True)
+    got = gpu_run(ks, m, shape, faces, f0, 12)
+    assert np.abs(got - ref).max() <= 1e-12
+
+
+def test_shortcut_with_smagorinsky_matches_per_rate_kernel():
+    from lbmpy_amd import SubgridScaleModel
+    kw = _kw("aa", subgrid_scale_model=(SubgridScaleModel.SMAGORINSKY, 0.16))
+    m = create_lb_method(LBMConfig(**kw))
+    shape = (64, 5, 4)
+    f0 = seeded_state(m, shape, seed=14, du=0.05, noise=1e-3)
+    fast = gpu_kernels(kw)
+    slow = gpu_kernels(kw)
+    for k in slow:
+        k.higher_orders_shortcut = False
+    print("rate-1 shortcut with the Smagorinsky model", flush=True)
+    a = gpu_run(fast, m, shape, PERIODIC, f0, 8)
+    b = gpu_run(slow, m, shape, PERIODIC, f0, 8)
+    assert np.abs(a - b).max() <= 1e-15
+
+
+def test_other_higher_order_rates_take_the_per_rate_kernel():
+    rates = [0.0] * 4 + [1.9] * 5 + [1.0] + [1.0] * 16 + [1.2]     # the sixth-order rate is not 1
+    kw = dict(stencil=LBStencil(Stencil.D3Q27), method=Method.CUMULANT, relaxation_rates=rates, compressible=True)
+    (k,) = gpu_kernels(kw)
+    assert not _flags(k, (8, 8, 8), "float64") & nat.FLAG_HIGHER_ORDERS_RATE1
+    rates[26] = 1.0
+    (k,) = gpu_kernels(dict(kw, relaxation_rates=rates))
+    assert _flags(k, (8, 8, 8), "float64") & nat.FLAG_HIGHER_ORDERS_RATE1
