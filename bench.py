@@ -437,8 +437,8 @@ def run_closure(args):
 
 def run_patterns(args):
     """The D3Q27 cumulant sweep (FP32, periodic, no ghost layers) with every streaming pattern -- pull and push
-    on two arrays, AA, EsoTwist, EsoPull and EsoPush in place -- and with the Galilean / fourth-order
-    corrections (AA): MLUPS and achieved bandwidth (2 x 27 x 4 B per cell) per kernel, from HIP events around
+    on two arrays, AA, EsoTwist, EsoPull and EsoPush in place -- with the Galilean / fourth-order corrections, the
+    per-rate kernel and the learned closure (AA): MLUPS and achieved bandwidth (2 x 27 x 4 B per cell) per kernel, from HIP events around
     each launch on the launch stream.  The state is the rest equilibrium (timing only; parity is in the tests)."""
     import torch
     from lbmpy_amd import (LBMConfig, LBStencil, Stencil, Method, create_lb_function, CreateKernelConfig, Target,
@@ -448,11 +448,16 @@ def run_patterns(args):
     cells = n ** 3
     cfg = CreateKernelConfig(target=Target.GPU, data_type="float32", ghost_layers=0)
     base = dict(stencil=LBStencil(Stencil.D3Q27), method=Method.CUMULANT, relaxation_rate=1.99, compressible=True)
+    from lbmpy_amd.closure import ClosureModel
     variants = [("pull", "pull", {}), ("push", "push", {}), ("aa", "aa", {}), ("esotwist", "esotwist", {}),
                 ("esopull", "esopull", {}), ("esopush", "esopush", {}),
                 ("aa+galilean", "aa", dict(galilean_correction=True)),
                 ("aa+fourth_order", "aa", dict(fourth_order_correction=0.01)),
-                ("aa+galilean+fourth_order", "aa", dict(galilean_correction=True, fourth_order_correction=0.01))]
+                ("aa+galilean+fourth_order", "aa", dict(galilean_correction=True, fourth_order_correction=0.01)),
+                # the per-rate kernel (orders >= 3 computed and relaxed with their table rates of 1)
+                ("aa per-rate kernel", "aa", {}),
+                # the learned closure in place of the shear rate (a seeded network: timing only)
+                ("aa+learned_closure", "aa", {})]
     out = {}
     for name, pattern, extra in variants:
         if pattern in ("pull", "push"):
@@ -462,6 +467,10 @@ def run_patterns(args):
                                      config=cfg) for t in (Timestep.ODD, Timestep.EVEN)]
         for k in ks:
             k.set_faces(["periodic"] * 3)
+            if name == "aa per-rate kernel":
+                k.higher_orders_shortcut = False
+            if name == "aa+learned_closure":
+                k.set_closure(ClosureModel(seed=0, nu_scale=2e-3))
         a = create_pdf_array((n, n, n), 27, ghost_layers=0, dtype="float32")
         b = create_pdf_array((n, n, n), 27, ghost_layers=0, dtype="float32") if len(ks) == 1 else a
 

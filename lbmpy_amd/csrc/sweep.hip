@@ -353,7 +353,10 @@ void* pick_kernel(const lbm355_sweep_desc* d, bool walls, bool rowlds) {
     return f64 ? pick_sweep_central_f64(st, sd, p, walls, rowlds) : pick_sweep_central_f32(st, sd, p, walls, rowlds);
   if (d->family == LBM355_FAMILY_CUMULANT) {
     if (d->flags & LBM355_FLAG_LEARNED_CLOSURE)
-      return f64 ? nullptr : pick_sweep_cumulant_learned_f32(st, sd, p, walls, rowlds);
+      return f64 ? nullptr
+                 : (d->flags & LBM355_FLAG_HIGHER_ORDERS_RATE1)
+                       ? pick_sweep_cumulant_learned_eq3_f32(st, sd, p, walls, rowlds)
+                       : pick_sweep_cumulant_learned_f32(st, sd, p, walls, rowlds);
     if (d->flags & (LBM355_FLAG_GALILEAN | LBM355_FLAG_FOURTH_ORDER))
       return f64 ? pick_sweep_cumulant_corrected_f64(st, sd, p, walls, rowlds)
                  : pick_sweep_cumulant_corrected_f32(st, sd, p, walls, rowlds);
@@ -576,10 +579,9 @@ int lbm355_sweep(const lbm355_sweep_desc* d, const void* pdf_src, void* pdf_dst,
                                         "models and corrections");
   if ((d->flags & LBM355_FLAG_HIGHER_ORDERS_RATE1) &&
       (d->family != LBM355_FAMILY_CUMULANT ||
-       (d->flags & (LBM355_FLAG_GENERAL_CUMULANT | LBM355_FLAG_LEARNED_CLOSURE | LBM355_FLAG_GALILEAN |
-                    LBM355_FLAG_FOURTH_ORDER))))
+       (d->flags & (LBM355_FLAG_GENERAL_CUMULANT | LBM355_FLAG_GALILEAN | LBM355_FLAG_FOURTH_ORDER))))
     return fail(LBM355_ERR_UNSUPPORTED, "LBM355_FLAG_HIGHER_ORDERS_RATE1 is for the default cumulant set without "
-                                        "the learned closure and the corrections");
+                                        "the corrections");
   if ((d->flags & LBM355_FLAG_STOCHASTIC_ROUNDING) && (d->store_dtype != LBM355_F16 || d->compute_dtype != LBM355_F32))
     return fail(LBM355_ERR_ARG, "stochastic rounding is for FP16 storage with FP32 arithmetic");
   if ((d->omega_out || d->eddy_viscosity_out) && !(d->flags & LBM355_FLAG_SMAGORINSKY))

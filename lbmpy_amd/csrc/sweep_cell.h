@@ -38,6 +38,19 @@ struct Collider<D3Q27, FAMILY_CUMULANT_LEARNED, TC> {
   }
 };
 
+// the learned-closure sweep when every cumulant of order >= 3 relaxes with rate 1
+constexpr int FAMILY_CUMULANT_LEARNED_EQ3 = 0x106;
+
+template <class TC>
+struct Collider<D3Q27, FAMILY_CUMULANT_LEARNED_EQ3, TC> {
+  static constexpr bool deviation_form = true;
+  template <bool NEQ>
+  static __device__ __forceinline__ void run(TC (&f)[27], const SweepArgs<TC>& a, int64_t, TC& rho, TC (&u)[3],
+                                             CellExtras<TC>& ex) {
+    collide_cumulant_d3q27<TC, true, false, NEQ, false, true>(f, a, rho, u, ex);
+  }
+};
+
 // kernel-variant family code of the cumulant sweep with the Galilean / fourth-order corrections (internal)
 constexpr int FAMILY_CUMULANT_CORRECTED = 0x103;
 

@@ -186,5 +186,6 @@ LBM355_PICKER(pick_sweep_cumulant_general_f32);
 LBM355_PICKER(pick_sweep_cumulant_eq3_f64);
 LBM355_PICKER(pick_sweep_cumulant_eq3_f32);
 LBM355_PICKER(pick_sweep_cumulant_learned_f32);
+LBM355_PICKER(pick_sweep_cumulant_learned_eq3_f32);
 
 }  // namespace lbm355

@@ -205,9 +205,9 @@ class SweepKernel:
 
     def _higher_orders_rate1(self, arr):
         """Whether this table lets the sweep set every cumulant of order >= 3 to equilibrium: the default D3Q27
-        cumulant set whose rates 10..26 (orders 3 to 6) are all 1, without corrections or a learned closure."""
+        cumulant set whose rates 10..26 (orders 3 to 6) are all 1, without corrections."""
         return (self.family == nat.FAMILY_CUMULANT and not self.general_cumulant and self.corrections is None
-                and self.closure is None and bool(np.all(np.asarray(arr[10:27]) == 1.0)))
+                and bool(np.all(np.asarray(arr[10:27]) == 1.0)))
 
     def shear_rate(self, bindings):
         """The method's shear relaxation rate with the free symbols bound (cached per binding set)."""
@@ -266,7 +266,7 @@ class SweepKernel:
                 flags |= nat.FLAG_FOURTH_ORDER
         if self.general_cumulant:
             flags |= nat.FLAG_GENERAL_CUMULANT
-        if higher_rate1 and self.closure is None and self.higher_orders_shortcut:
+        if higher_rate1 and self.higher_orders_shortcut:
             flags |= nat.FLAG_HIGHER_ORDERS_RATE1
         if self.closure is not None:
             flags |= nat.FLAG_LEARNED_CLOSURE
