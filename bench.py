@@ -468,7 +468,7 @@ def run_patterns(args):
         for k in ks:
             k.set_faces(["periodic"] * 3)
             if name == "aa per-rate kernel":
-                k.higher_orders_shortcut = False
+                k.shortcuts = False
             if name == "aa+learned_closure":
                 k.set_closure(ClosureModel(seed=0, nu_scale=2e-3))
         a = create_pdf_array((n, n, n), 27, ghost_layers=0, dtype="float32")

@@ -29,7 +29,7 @@ template <int Q>
 struct MomentTab {
   static constexpr int A = 0, B = Q, L = 2 * Q, QD = 5 * Q, QO = 8 * Q, G0 = 11 * Q, G = 12 * Q, RATES = 15 * Q,
                        COEF = 15 * Q + 2, MROW = 16 * Q + 2, MINV = 16 * Q + 2 + Q * Q,
-                       SIZE = 16 * Q + 2 + 2 * Q * Q;
+                       SHEAR_ON = 16 * Q + 2 + 2 * Q * Q, SHEAR = SHEAR_ON + 1, SIZE = SHEAR + Q;
 };
 
 // o: the cell's offset in the optional per-cell fields (density / velocity inputs).
@@ -111,7 +111,52 @@ __device__ __forceinline__ void collide_moment(T (&f)[S::Q], const SweepArgs<T>&
       post[i] = f[i] + re * (dl[i] + dl[k]) + ro * (dl[i] - dl[k]) + g;
     }
   }
-  // individually relaxed moments: rank-one projector corrections
+  // individually relaxed moments
+  if constexpr (D == 3) {
+    if (a.flags & LBM355_FLAG_SHEAR_PROJECTOR) {
+      // they are exactly the shear moments, with one rate: their projector in closed form,
+      //   post_i += s (9/2) w_i (c_i c_i) : Pi_dev,  Pi = sum_j c_j c_j (f_eq - f)_j,
+      // with s (9/2) w_i from the table (SHEAR); the host checked the identity against the rank-one sum
+      T pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        const int cx = S::c[i][0], cy = S::c[i][1], cz = S::c[i][2];
+        if (cx != 0) pxx += dl[i];
+        if (cy != 0) pyy += dl[i];
+        if (cz != 0) pzz += dl[i];
+        if (cx * cy > 0) pxy += dl[i];
+        if (cx * cy < 0) pxy -= dl[i];
+        if (cx * cz > 0) pxz += dl[i];
+        if (cx * cz < 0) pxz -= dl[i];
+        if (cy * cz > 0) pyz += dl[i];
+        if (cy * cz < 0) pyz -= dl[i];
+      }
+      const T t3 = (pxx + pyy + pzz) * T(1.0 / 3.0);
+      const T dxx = pxx - t3, dyy = pyy - t3, dzz = pzz - t3;
+      const T xy2 = pxy + pxy, xz2 = pxz + pxz, yz2 = pyz + pyz;
+      const TablePtr<T> __restrict__ sh = tab + L::SHEAR;
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        const int cx = S::c[i][0], cy = S::c[i][1], cz = S::c[i][2];
+        if (cx == 0 && cy == 0 && cz == 0) continue;
+        T q = 0;
+        if (cx != 0) q += dxx;
+        if (cy != 0) q += dyy;
+        if (cz != 0) q += dzz;
+        if (cx * cy > 0) q += xy2;
+        if (cx * cy < 0) q -= xy2;
+        if (cx * cz > 0) q += xz2;
+        if (cx * cz < 0) q -= xz2;
+        if (cy * cz > 0) q += yz2;
+        if (cy * cz < 0) q -= yz2;
+        post[i] += sh[i] * q;
+      }
+#pragma unroll
+      for (int i = 0; i < Q; ++i) f[i] = post[i];
+      return;
+    }
+  }
+  // rank-one projector corrections
   for (int k = 0; k < a.nactive; ++k) {
     const TablePtr<T> mrow = tab + L::MROW + k * Q;
     const TablePtr<T> minv = tab + L::MINV + k * Q;

@@ -582,6 +582,9 @@ int lbm355_sweep(const lbm355_sweep_desc* d, const void* pdf_src, void* pdf_dst,
        (d->flags & (LBM355_FLAG_GENERAL_CUMULANT | LBM355_FLAG_GALILEAN | LBM355_FLAG_FOURTH_ORDER))))
     return fail(LBM355_ERR_UNSUPPORTED, "LBM355_FLAG_HIGHER_ORDERS_RATE1 is for the default cumulant set without "
                                         "the corrections");
+  if ((d->flags & LBM355_FLAG_SHEAR_PROJECTOR) &&
+      (d->family != LBM355_FAMILY_MOMENT || d->stencil == LBM355_D2Q9))
+    return fail(LBM355_ERR_UNSUPPORTED, "LBM355_FLAG_SHEAR_PROJECTOR is for the moment family on 3-D stencils");
   if ((d->flags & LBM355_FLAG_STOCHASTIC_ROUNDING) && (d->store_dtype != LBM355_F16 || d->compute_dtype != LBM355_F32))
     return fail(LBM355_ERR_ARG, "stochastic rounding is for FP16 storage with FP32 arithmetic");
   if ((d->omega_out || d->eddy_viscosity_out) && !(d->flags & LBM355_FLAG_SMAGORINSKY))
@@ -609,7 +612,8 @@ int lbm355_sweep(const lbm355_sweep_desc* d, const void* pdf_src, void* pdf_dst,
 int64_t lbm355_table_elements(int32_t stencil, int32_t family) {
   const int Q = stencil_q(stencil);
   if (!Q) return fail(LBM355_ERR_ARG, "unknown stencil"), -1;
-  if (family == LBM355_FAMILY_MOMENT) return 16 * Q + 2 + 2 * Q * Q;
+  if (family == LBM355_FAMILY_MOMENT)
+    return Q == 9 ? MomentTab<9>::SIZE : (Q == 19 ? MomentTab<19>::SIZE : MomentTab<27>::SIZE);
   if (family == LBM355_FAMILY_CUMULANT) {
     if (stencil != LBM355_D3Q27) return fail(LBM355_ERR_UNSUPPORTED, "the cumulant sweep is D3Q27 only"), -1;
     return CumulantTab::SIZE;
@@ -700,7 +704,7 @@ int lbm355_reduce_conserved(const lbm355_sweep_desc* d, const void* pdf, double*
 
 const char* lbm355_last_error(void) { return g_err.c_str(); }
 
-int32_t lbm355_version(void) { return 7; }
+int32_t lbm355_version(void) { return 8; }
 
 #ifndef LBM355_SOURCE_HASH
 #define LBM355_SOURCE_HASH "unknown"

@@ -11,8 +11,8 @@ from fractions import Fraction
 import numpy as np
 
 from ._poly import Poly, MOMENT_VARS
-from .equilibrium import EQ_VARS
-from .moments import mat_inv, moment_matrix, cascaded_moment_sets_literature, get_order
+from .equilibrium import EQ_VARS, lattice_weights
+from .moments import mat_inv, moment_matrix, cascaded_moment_sets_literature, get_order, is_shear_moment
 
 FAMILY_MOMENT, FAMILY_CENTRAL, FAMILY_CUMULANT = 0, 1, 2
 STENCIL_CODES = {"D2Q9": 0, "D3Q19": 1, "D3Q27": 2}
@@ -167,26 +167,46 @@ def moment_table(method, bindings, dtype=np.float64, inputs=False):
         G0 += t0
         G[:, :D] += tu
 
-    active = []
-    for k in range(Q):
-        rb = r_even if even[k] else r_odd
-        dr = rates[k] - rb
-        if dr == 0.0:
-            continue
-        if conserved[k] and not inputs:
-            # m_eq - m for a conserved moment: 0 for the density part; F/2 for momentum when the
-            # equilibrium velocity is shifted by F/(2 rho_ref)
-            lin_coef = np.zeros(3)
-            for e, cf in polys[k].terms.items():
-                if sum(e) == 1:
-                    lin_coef[e.index(1)] = float(cf)
-            dm = 0.0
-            if fm is not None and fm.equilibrium_velocity_shift:
-                dm = float(lin_coef @ (0.5 * F))
-            G0 += dr * Minvf[:, k] * dm
-        else:
-            active.append((k, dr))
-    tab = np.zeros(16 * Q + 2 + 2 * Q * Q)
+    def split(r_even, r_odd):
+        """Moments relaxed individually (rate off their parity's base rate), and the constant force column
+        of the conserved ones among them."""
+        active, g0 = [], np.zeros(Q)
+        for k in range(Q):
+            rb = r_even if even[k] else r_odd
+            dr = rates[k] - rb
+            if dr == 0.0:
+                continue
+            if conserved[k] and not inputs:
+                # m_eq - m for a conserved moment: 0 for the density part; F/2 for momentum when the
+                # equilibrium velocity is shifted by F/(2 rho_ref)
+                lin_coef = np.zeros(3)
+                for e, cf in polys[k].terms.items():
+                    if sum(e) == 1:
+                        lin_coef[e.index(1)] = float(cf)
+                dm = 0.0
+                if fm is not None and fm.equilibrium_velocity_shift:
+                    dm = float(lin_coef @ (0.5 * F))
+                g0 += dr * Minvf[:, k] * dm
+            else:
+                active.append((k, dr))
+        return active, g0
+
+    active, g0 = split(r_even, r_odd)
+    shear = _shear_projector(st, active, Mf, Minvf)
+    if not shear[0]:
+        # the same collision with the even base rate taken from the moments that are not shear moments: when the
+        # shear moments are then the only active ones, the kernel applies their projector in closed form
+        non_shear = even & ~conserved & ~np.array([is_shear_moment(p, D) for p in polys])
+        if non_shear.any():
+            uniq, counts = np.unique(rates[non_shear], return_counts=True)
+            r_alt = float(uniq[np.argmax(counts)])
+            act2, g02 = split(r_alt, r_odd)
+            sh2 = _shear_projector(st, act2, Mf, Minvf)
+            if sh2[0]:
+                r_even, active, g0, shear = r_alt, act2, g02, sh2
+    G0 += g0
+    tab = np.zeros(16 * Q + 2 + 2 * Q * Q + 1 + Q)
+    tab[16 * Q + 2 + 2 * Q * Q:] = shear
     tab[0:Q] = A
     tab[Q:2 * Q] = B
     tab[2 * Q:5 * Q] = L.reshape(-1)
@@ -201,6 +221,38 @@ def moment_table(method, bindings, dtype=np.float64, inputs=False):
         tab[16 * Q + 2 + n * Q: 16 * Q + 2 + (n + 1) * Q] = Mf[k]
         tab[16 * Q + 2 + Q * Q + n * Q: 16 * Q + 2 + Q * Q + (n + 1) * Q] = Minvf[:, k]
     return tab.astype(dtype), len(active)
+
+
+def _shear_projector(st, active, Mf, Minvf):
+    """The table's SHEAR block: [1, s (9/2) w_i ...] when the sum of the active moments' rank-one corrections,
+    sum_k dr_k Minv[:, k] M[k], equals s times the w-weighted projector onto the traceless second-order (shear)
+    moments, P_ij = (9/2) w_i (c_i c_i) : (c_j c_j - |c_j|^2 I / 3) -- as for an orthogonal MRT whose only
+    rates off their parity's base rate are the shear moments' common rate (the default orthogonal D3Q19 / D3Q27
+    MRT given its shear rate).  The kernel then applies s (9/2) w_i (c_i c_i) : Pi_dev(f_eq - f) in closed form.
+    Zeros otherwise (3-D stencils only; checked to 1e-12 of the largest entry)."""
+    Q, D = st.Q, st.D
+    out = np.zeros(Q + 1)
+    if D != 3 or not active:
+        return out
+    C = np.zeros((Q, Q))
+    for k, dr in active:
+        C += dr * np.outer(Minvf[:, k], Mf[k])
+    w = np.array([float(x) for x in lattice_weights(st)])
+    c = np.array([list(st[i]) for i in range(Q)], dtype=float)
+    cc = np.einsum("ia,ib->iab", c, c)
+    dev = cc - np.einsum("i,ab->iab", (c * c).sum(1) / 3.0, np.eye(3))
+    P = 4.5 * w[:, None] * np.einsum("iab,jab->ij", cc, dev)
+    scale = float((C * P).sum() / (P * P).sum())
+    if np.abs(C - scale * P).max() > 1e-12 * max(1.0, np.abs(C).max()):
+        return out
+    out[0] = 1.0
+    out[1:] = scale * 4.5 * w
+    return out
+
+
+def moment_shear_projector(tab, q):
+    """Whether a moment-family table carries the closed-form shear projector (LBM355_FLAG_SHEAR_PROJECTOR)."""
+    return tab.size == 16 * q + 2 + 2 * q * q + 1 + q and tab[16 * q + 2 + 2 * q * q] == 1.0
 
 
 # ------------------------------------------------------------------------------- cumulant family

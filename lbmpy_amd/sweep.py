@@ -17,7 +17,7 @@ import torch
 
 from . import _native as nat
 from .kernel_tables import method_table, family_of, STENCIL_CODES, UnsupportedMethod, bind_rate, \
-    cumulant_is_default_set, \
+    cumulant_is_default_set, moment_shear_projector, \
     cumulant_corrections
 from .relaxationrates import get_shear_relaxation_rate
 from .enums import SubgridScaleModel, Target
@@ -133,9 +133,10 @@ class SweepKernel:
         if (lc.omega_output_field is not None or lc.eddy_viscosity_field is not None) and self.smagorinsky is None:
             raise NotImplementedError("omega_output_field / eddy_viscosity_field are written by the Smagorinsky "
                                       "model on the GPU path; set subgrid_scale_model")
-        #: with every cumulant of order >= 3 relaxed with rate 1, use the kernel variant that sets them to
-        #: equilibrium without computing them (False runs the per-rate kernel, for comparisons)
-        self.higher_orders_shortcut = True
+        #: use the closed-form kernel paths the method admits -- cumulants of order >= 3 at rate 1 set to
+        #: equilibrium without being computed, an MRT's shear moments relaxed through their projector in closed
+        #: form -- (False runs the generic table-driven forms, for comparisons)
+        self.shortcuts = True
         #: a cumulant set other than the default polynomial one: relaxed by the general kernel variant
         self.general_cumulant = self.family == nat.FAMILY_CUMULANT and not cumulant_is_default_set(m)
         if self.general_cumulant and self.smagorinsky is not None:
@@ -195,7 +196,7 @@ class SweepKernel:
             arr, nact = method_table(self.method, bindings, corrections=self.corrections,
                                      inputs=self.lbm_config.velocity_input is not None)
             dt = torch.float64 if self.compute == nat.F64 else torch.float32
-            t = (torch.tensor(arr, dtype=dt, device=device), nact, self._higher_orders_rate1(arr))
+            t = (torch.tensor(arr, dtype=dt, device=device), nact, self._table_flags(arr))
             while len(self._tables) >= self.TABLE_CACHE_SIZE:
                 # a launch queued with an older table may still read it: the stream orders the caching
                 # allocator's reuse of the freed block after that launch
@@ -203,11 +204,18 @@ class SweepKernel:
         self._tables[key] = t
         return t
 
-    def _higher_orders_rate1(self, arr):
-        """Whether this table lets the sweep set every cumulant of order >= 3 to equilibrium: the default D3Q27
-        cumulant set whose rates 10..26 (orders 3 to 6) are all 1, without corrections."""
-        return (self.family == nat.FAMILY_CUMULANT and not self.general_cumulant and self.corrections is None
-                and bool(np.all(np.asarray(arr[10:27]) == 1.0)))
+    def _table_flags(self, arr):
+        """The closed-form kernel paths this table admits: LBM355_FLAG_HIGHER_ORDERS_RATE1 for the default D3Q27
+        cumulant set whose rates 10..26 (orders 3 to 6) are all 1, without corrections;
+        LBM355_FLAG_SHEAR_PROJECTOR for a moment-family table whose individually relaxed moments are the shear
+        moments (moment_shear_projector)."""
+        flags = 0
+        if (self.family == nat.FAMILY_CUMULANT and not self.general_cumulant and self.corrections is None
+                and bool(np.all(np.asarray(arr[10:27]) == 1.0))):
+            flags |= nat.FLAG_HIGHER_ORDERS_RATE1
+        if self.family == nat.FAMILY_MOMENT and moment_shear_projector(np.asarray(arr), self.method.stencil.Q):
+            flags |= nat.FLAG_SHEAR_PROJECTOR
+        return flags
 
     def shear_rate(self, bindings):
         """The method's shear relaxation rate with the free symbols bound (cached per binding set)."""
@@ -227,7 +235,7 @@ class SweepKernel:
         sizes = [arr.shape[a] - 2 * g for a in range(D)]
         if min(sizes) <= 0:
             raise ValueError("PDF array smaller than its ghost layers")
-        tab, nact, higher_rate1 = self.table(bindings, arr.device)
+        tab, nact, table_flags = self.table(bindings, arr.device)
         d = nat.SweepDesc()
         d.stencil = STENCIL_CODES[m.stencil.name]
         d.family = self.family
@@ -266,8 +274,8 @@ class SweepKernel:
                 flags |= nat.FLAG_FOURTH_ORDER
         if self.general_cumulant:
             flags |= nat.FLAG_GENERAL_CUMULANT
-        if higher_rate1 and self.higher_orders_shortcut:
-            flags |= nat.FLAG_HIGHER_ORDERS_RATE1
+        if self.shortcuts:
+            flags |= table_flags
         if self.closure is not None:
             flags |= nat.FLAG_LEARNED_CLOSURE
             d.closure = self.closure.params.data_ptr()

@@ -34,7 +34,7 @@ def test_shortcut_matches_per_rate_kernel(pattern, data_type, tol):
     fast = gpu_kernels(kw, data_type=data_type)
     slow = gpu_kernels(kw, data_type=data_type)
     for k in slow:
-        k.higher_orders_shortcut = False
+        k.shortcuts = False
     assert all(_flags(k, shape, data_type) & nat.FLAG_HIGHER_ORDERS_RATE1 for k in fast)
     assert not any(_flags(k, shape, data_type) & nat.FLAG_HIGHER_ORDERS_RATE1 for k in slow)
     print(f"rate-1 shortcut vs per-rate kernel: {pattern} {data_type}", flush=True)
@@ -68,7 +68,7 @@ def test_shortcut_with_smagorinsky_matches_per_rate_kernel():
     fast = gpu_kernels(kw)
     slow = gpu_kernels(kw)
     for k in slow:
-        k.higher_orders_shortcut = False
+        k.shortcuts = False
     print("rate-1 shortcut with the Smagorinsky model", flush=True)
     a = gpu_run(fast, m, shape, PERIODIC, f0, 8)
     b = gpu_run(slow, m, shape, PERIODIC, f0, 8)
