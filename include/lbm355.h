@@ -170,6 +170,11 @@ int lbm355_noslip_links(int32_t stencil, int32_t pattern, int32_t dtype, void* p
  * out is a device buffer of 4 doubles (overwritten). Stored zero-centred PDFs add 1 per cell to rho. */
 int lbm355_reduce_conserved(const lbm355_sweep_desc* desc, const void* pdf, double* out, void* stream);
 
+/* Failure detection: the number of interior populations that are NaN or infinite, written to *out (a device
+ * uint64, overwritten).  lbmpy has no on-device check; its tests look for non-finite velocities on the host
+ * (tests/test_quicktests.py:26-27, tests/cumulantmethod/test_flow_around_sphere.py:137-139). */
+int lbm355_count_nonfinite(const lbm355_sweep_desc* desc, const void* pdf, uint64_t* out, void* stream);
+
 /* Equilibrium initialisation from macroscopic fields (lbmpy macroscopic_value_kernels.py:29-45,
  * pdf_initialization_assignments; run by LatticeBoltzmannStep.set_pdf_fields_from_macroscopic_values,
  * lbstep.py:268-269).  density / velocity: FP64 device arrays laid out with desc->out_stride_*

@@ -59,6 +59,7 @@ class SweepDesc(ctypes.Structure):
 
 
 EXPORTED = ("lbm355_sweep", "lbm355_table_elements", "lbm355_noslip_links", "lbm355_reduce_conserved",
+            "lbm355_count_nonfinite",
             "lbm355_set_equilibrium", "lbm355_get_macroscopic", "lbm355_stream_copy", "lbm355_desc_size",
             "lbm355_last_error", "lbm355_version", "lbm355_batch_create", "lbm355_batch_sweep",
             "lbm355_batch_info", "lbm355_batch_destroy", "lbm355_batch_sweep_static", "lbm355_closure_params_size",
@@ -101,6 +102,8 @@ def load(path=None):
     lib.lbm355_plane_copy.restype = c_int32
     lib.lbm355_valu_probe.argtypes = [c_void_p, c_int64, c_int32, c_void_p]
     lib.lbm355_valu_probe.restype = c_int32
+    lib.lbm355_count_nonfinite.argtypes = [ctypes.POINTER(SweepDesc), c_void_p, c_void_p, c_void_p]
+    lib.lbm355_count_nonfinite.restype = c_int32
     lib.lbm355_batch_create.argtypes = [ctypes.POINTER(SweepDesc), ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
                                         c_int32, ctypes.POINTER(c_void_p)]
     lib.lbm355_batch_create.restype = c_int32

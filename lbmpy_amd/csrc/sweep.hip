@@ -292,6 +292,38 @@ __global__ void __launch_bounds__(256) reduce_kernel(const TS* pdf, int nx, int 
   }
 }
 
+// Count of interior populations that are NaN or infinite (failure detection): per-wave ballot counts, one
+// 64-bit atomic per wave.
+__global__ void zero1_kernel(unsigned long long* out) {
+  if (threadIdx.x == 0) *out = 0ull;
+}
+
+template <class S, class TS>
+__global__ void __launch_bounds__(256) nonfinite_kernel(const TS* pdf, int nx, int g, int64_t sy, int64_t sz,
+                                                        int64_t sq, unsigned long long* out) {
+  constexpr int Q = S::Q;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y, z = blockIdx.z;
+  unsigned n = 0;
+  if (x < nx) {
+    const int64_t c = (x + g) + (y + g) * sy + (S::D == 3 ? (z + g) * sz : 0);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const TS v = pdf[i * sq + c];
+      bool fin;
+      if constexpr (sizeof(TS) == 8)
+        fin = __builtin_isfinite(v);
+      else
+        fin = __builtin_isfinite((float)v);   // exact for FP16 and FP32
+      n += fin ? 0u : 1u;
+    }
+  }
+  unsigned long long total = 0;
+#pragma unroll
+  for (int b = 0; b < 5; ++b) total += (unsigned long long)__popcll(__ballot(n & (1u << b))) << b;
+  if ((threadIdx.x & 63) == 0 && total) atomicAdd(out, total);
+}
+
 }  // namespace lbm355
 
 // ====================================================================== host side
@@ -531,6 +563,25 @@ int launch_reduce(const lbm355_sweep_desc* d, const void* pdf, double* out, hipS
   return LBM355_OK;
 }
 
+template <class S, class TS>
+int launch_nonfinite(const lbm355_sweep_desc* d, const void* pdf, unsigned long long* out, hipStream_t s) {
+  hipLaunchKernelGGL(zero1_kernel, dim3(1), dim3(64), 0, s, out);
+  Launch l = geometry(d->nx, d->ny, d->nz);
+  hipLaunchKernelGGL((nonfinite_kernel<S, TS>), l.grid, l.block, 0, s, (const TS*)pdf, d->nx, d->ghost_layers,
+                     d->stride_y, d->stride_z, d->stride_q, out);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(LBM355_ERR_LAUNCH, std::string("launch failed: ") + hipGetErrorString(e));
+  return LBM355_OK;
+}
+
+template <class S>
+int nonfinite_dtype(const lbm355_sweep_desc* d, const void* pdf, unsigned long long* out, hipStream_t s) {
+  if (d->store_dtype == LBM355_F64) return launch_nonfinite<S, double>(d, pdf, out, s);
+  if (d->store_dtype == LBM355_F32) return launch_nonfinite<S, float>(d, pdf, out, s);
+  if (d->store_dtype == LBM355_F16) return launch_nonfinite<S, _Float16>(d, pdf, out, s);
+  return fail(LBM355_ERR_ARG, "bad dtype");
+}
+
 template <class S>
 int reduce_dtype(const lbm355_sweep_desc* d, const void* pdf, double* out, hipStream_t s) {
   if (d->store_dtype == LBM355_F64) return launch_reduce<S, double>(d, pdf, out, s);
@@ -699,6 +750,17 @@ int lbm355_reduce_conserved(const lbm355_sweep_desc* d, const void* pdf, double*
   if (d->stencil == LBM355_D2Q9) return reduce_dtype<D2Q9>(d, pdf, out, s);
   if (d->stencil == LBM355_D3Q19) return reduce_dtype<D3Q19>(d, pdf, out, s);
   if (d->stencil == LBM355_D3Q27) return reduce_dtype<D3Q27>(d, pdf, out, s);
+  return fail(LBM355_ERR_ARG, "unknown stencil");
+}
+
+int lbm355_count_nonfinite(const lbm355_sweep_desc* d, const void* pdf, uint64_t* out, void* stream) {
+  if (!d || !pdf || !out) return fail(LBM355_ERR_ARG, "NULL argument");
+  if (d->nx <= 0 || d->ny <= 0 || d->nz <= 0) return fail(LBM355_ERR_ARG, "bad interior size");
+  hipStream_t s = (hipStream_t)stream;
+  unsigned long long* o = reinterpret_cast<unsigned long long*>(out);
+  if (d->stencil == LBM355_D2Q9) return nonfinite_dtype<D2Q9>(d, pdf, o, s);
+  if (d->stencil == LBM355_D3Q19) return nonfinite_dtype<D3Q19>(d, pdf, o, s);
+  if (d->stencil == LBM355_D3Q27) return nonfinite_dtype<D3Q27>(d, pdf, o, s);
   return fail(LBM355_ERR_ARG, "unknown stencil");
 }
 

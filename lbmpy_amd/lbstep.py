@@ -271,6 +271,34 @@ class LatticeBoltzmannStep:
         self.post_run()
         self.time_steps_run += time_steps
 
+    # ------------------------------------------------------------------ failure detection, checkpoints
+    def nonfinite_count(self):
+        """Number of interior populations that are NaN or infinite (counted on the GPU); 0 for a healthy run."""
+        return self._kernels[0].nonfinite_count(self._src, **self.kernel_params)
+
+    def check_finite(self):
+        """Raise FloatingPointError when the populations hold NaN or infinite values."""
+        n = self.nonfinite_count()
+        if n:
+            raise FloatingPointError(f"{n} populations are NaN or infinite after {self.time_steps_run} time steps")
+
+    def save_checkpoint(self, path):
+        """Write the run's state: populations, steps run, which AA kernel runs next and the kernels' launch counts
+        (lbmpy_amd.checkpoint)."""
+        from .checkpoint import save_pdf_state
+        torch.cuda.synchronize()
+        save_pdf_state(path, self._src, time_step=self.time_steps_run, stencil_name=self.method.stencil.name,
+                       pattern=self._pattern, kernels=self._kernels, next_odd=self._aa_next_odd)
+
+    def load_checkpoint(self, path):
+        """Resume from a state written by save_checkpoint: same domain, stencil, storage type and streaming
+        pattern; the boundary handling and kernel parameters are this object's own."""
+        from .checkpoint import load_pdf_state
+        self.time_steps_run, self._aa_next_odd = load_pdf_state(path, self._src, stencil_name=self.method.stencil.name,
+                                                                pattern=self._pattern, kernels=self._kernels)
+        torch.cuda.synchronize()
+        self.post_run()
+
     def benchmark_run(self, time_steps, number_of_cells=None):
         if number_of_cells is None:
             number_of_cells = self.number_of_cells

@@ -389,6 +389,18 @@ class SweepKernel:
                                               ctypes.c_void_p(out.data_ptr()), nat.current_stream_handle()))
         return out.cpu().numpy()
 
+    def nonfinite_count(self, pdf, **bindings):
+        """Number of interior populations of `pdf` that are NaN or infinite, counted on the GPU
+        (lbm355_count_nonfinite); 0 for a healthy field.  Synchronises the current stream."""
+        self._check_tensor(pdf)
+        binds = {s: float(bindings.get(s, 1.0)) for s in self.free_symbols}
+        d = self.descriptor(pdf, binds)
+        out = torch.zeros(1, dtype=torch.int64, device=pdf.device)
+        lib = nat.load()
+        nat.check(lib.lbm355_count_nonfinite(ctypes.byref(d), ctypes.c_void_p(pdf.data_ptr()),
+                                             ctypes.c_void_p(out.data_ptr()), nat.current_stream_handle()))
+        return int(out.item())
+
     def _check_tensor(self, t):
         if not isinstance(t, torch.Tensor) or not t.is_cuda:
             raise TypeError("PDF fields must be torch tensors on the GPU")
