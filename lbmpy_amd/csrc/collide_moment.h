@@ -96,8 +96,10 @@ __device__ __forceinline__ void collide_moment(T (&f)[S::Q], const SweepArgs<T>&
     dl[i] = feq - f[i];
   }
 
-  // base rates on the symmetric / antisymmetric parts, plus affine force term
-  const T re = tab[L::RATES + 0] * T(0.5), ro = tab[L::RATES + 1] * T(0.5);
+  // base rates on the symmetric / antisymmetric parts, plus affine force term:
+  //   r_e (dl_i + dl_k) / 2 + r_o (dl_i - dl_k) / 2 = s dl_i + t dl_k,  s = (r_e + r_o) / 2, t = (r_e - r_o) / 2
+  const T re = tab[L::RATES + 0], ro = tab[L::RATES + 1];
+  const T sr = T(0.5) * (re + ro), tr = T(0.5) * (re - ro);
   T post[Q];
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
@@ -106,9 +108,9 @@ __device__ __forceinline__ void collide_moment(T (&f)[S::Q], const SweepArgs<T>&
 #pragma unroll
     for (int d = 0; d < D; ++d) g += tab[L::G + 3 * i + d] * u[d];
     if (k == i) {
-      post[i] = f[i] + (re + re) * dl[i] + g;
+      post[i] = f[i] + re * dl[i] + g;
     } else {
-      post[i] = f[i] + re * (dl[i] + dl[k]) + ro * (dl[i] - dl[k]) + g;
+      post[i] = f[i] + sr * dl[i] + tr * dl[k] + g;
     }
   }
   // individually relaxed moments
