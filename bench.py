@@ -512,6 +512,60 @@ def run_patterns(args):
     print(json.dumps(line), flush=True)
 
 
+def run_families(args):
+    """Every collision family at 256^3 (FP32 storage and arithmetic, AA streaming, periodic, rest state): D3Q19 and
+    D3Q27 SRT, TRT and MRT, D3Q27 central-moment and cumulant, each given by its shear rate (lbmpy's defaults for the
+    other rates).  MLUPS and bandwidth per family from HIP events around each launch on the launch stream."""
+    import torch
+    from lbmpy_amd import (LBMConfig, LBStencil, Stencil, Method, create_lb_function, CreateKernelConfig, Target,
+                           Timestep)
+    from lbmpy_amd.sweep import create_pdf_array
+    n = args.n
+    cells = n ** 3
+    cfg = CreateKernelConfig(target=Target.GPU, data_type="float32", ghost_layers=0)
+    cases = [("D3Q19 SRT", Stencil.D3Q19, dict(method=Method.SRT, relaxation_rate=1.8)),
+             ("D3Q19 TRT", Stencil.D3Q19, dict(method=Method.TRT, relaxation_rates=[1.8, 1.1])),
+             ("D3Q19 MRT", Stencil.D3Q19, dict(method=Method.MRT, relaxation_rate=1.8)),
+             ("D3Q27 SRT", Stencil.D3Q27, dict(method=Method.SRT, relaxation_rate=1.8)),
+             ("D3Q27 TRT", Stencil.D3Q27, dict(method=Method.TRT, relaxation_rates=[1.8, 1.1])),
+             ("D3Q27 MRT", Stencil.D3Q27, dict(method=Method.MRT, relaxation_rate=1.8)),
+             ("D3Q27 central moment", Stencil.D3Q27, dict(method=Method.CENTRAL_MOMENT, relaxation_rate=1.8,
+                                                          compressible=True)),
+             ("D3Q27 cumulant", Stencil.D3Q27, dict(method=Method.CUMULANT, relaxation_rate=1.8, compressible=True))]
+    out = {}
+    for name, st, kw in cases:
+        stencil = LBStencil(st)
+        ks = [create_lb_function(lbm_config=LBMConfig(stencil=stencil, streaming_pattern="aa", timestep=t, **kw),
+                                 config=cfg) for t in (Timestep.ODD, Timestep.EVEN)]
+        for k in ks:
+            k.set_faces(["periodic"] * 3)
+        a = create_pdf_array((n, n, n), stencil.Q, ghost_layers=0, dtype="float32")
+        warm, timed, _ = aa_schedule(args.warmup, args.steps, 0)
+        for t in warm:
+            ks[t % 2](src=a)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+        torch.cuda.synchronize()
+        ev[0].record()
+        for i, t in enumerate(timed):
+            ks[t % 2](src=a)
+            ev[i + 1].record()
+        torch.cuda.synchronize()
+        per = [ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)]
+        launch_ms = sum(per) / len(per)
+        b = 2 * stencil.Q * 4
+        gbs = b * cells / (launch_ms / 1e3) / 1e9
+        out[name] = {"mlups": round(cells / (launch_ms / 1e3) / 1e6, 1), "launch_ms": round(launch_ms, 4),
+                     "bytes_per_cell": b, "achieved_gbs": round(gbs, 1), "roofline_frac": round(gbs / HBM_PEAK_GBS, 4)}
+        print(f"{name}: {out[name]}", flush=True)
+        del a, ks
+        torch.cuda.empty_cache()
+    line = {"metric": "stream-collide MLUPS per collision family (FP32, AA, 256^3 periodic)",
+            "workload": f"periodic {n}^3, FP32 zero-centred, rest state, {args.steps} timed launches after "
+                        f"{args.warmup} warm-up launches; shear rate 1.8, lbmpy's defaults for the other rates",
+            "unit": "MLUPS", "families": out}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -525,10 +579,12 @@ def main():
     ap.add_argument("--forms", default="float64,float16",
                     help="other storage forms run beside the headline one ('' for none)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="tgv", choices=["tgv", "channel", "ensemble", "closure", "patterns"],
+    ap.add_argument("--workload", default="tgv",
+                    choices=["tgv", "channel", "ensemble", "closure", "patterns", "families"],
                     help="tgv: the headline D3Q27 cumulant TGV; channel: D3Q19 MRT channel in three storage forms; "
                          "ensemble: 256 batched D3Q19 TRT channels; closure: learned closure on decaying turbulence; "
-                         "patterns: the D3Q27 cumulant sweep with every streaming pattern and correction")
+                         "patterns: the D3Q27 cumulant sweep with every streaming pattern and correction; "
+                         "families: every collision family at 256^3 (FP32, AA)")
     ap.add_argument("--closure-coarse", type=int, default=128)
     ap.add_argument("--closure-steps", type=int, default=1200)
     ap.add_argument("--members", type=int, default=256)
@@ -559,6 +615,9 @@ def main():
         return
     if args.workload == "patterns":
         run_patterns(args)
+        return
+    if args.workload == "families":
+        run_families(args)
         return
 
     import torch

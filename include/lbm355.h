@@ -95,11 +95,17 @@ extern "C" {
                                                 without computing them.  The host sets it after checking the rates;
                                                 set with other rates, the sweep still relaxes them with rate 1.
                                                 Members of one batch must agree on it. */
-#define LBM355_FLAG_SHEAR_PROJECTOR 4096 /* moment family, 3-D stencils: the table's individually relaxed moments are
-                                            exactly the shear (traceless second-order) moments with one rate, and
-                                            its SHEAR block holds their projector's closed form; the sweep applies
-                                            that instead of n_active rank-one corrections.  The host sets it after
-                                            checking the identity. */
+#define LBM355_FLAG_SHEAR_PROJECTOR 4096 /* moment family (3-D stencils) and central-moment family (D3Q27): the
+                                            table's individually relaxed moments are the shear (traceless
+                                            second-order) moments with one rate (and, for the central-moment
+                                            family, the conserved moments with rate 0), and its SHEAR entries hold
+                                            the closed form of their projector; the sweep applies that instead of
+                                            n_active rank-one corrections.  The host sets it after checking the
+                                            identity. */
+#define LBM355_FLAG_GAUSSIAN_EQUILIBRIUM 8192 /* central-moment family, D3Q27: the equilibrium's central moments are
+                                                 rho gamma_e (the full continuous Maxwellian; the table's GAUSS
+                                                 entry), evaluated as such instead of the table's polynomial.  The
+                                                 host sets it after checking the table. */
 #define LBM355_FLAG_ADAPT_VELOCITY 256 /* boundary links, UBB: drive the wall at its equilibrium input velocity
                                           u_w - F / (2 rho_ref), rho_ref the link's density (the UBB's own, or
                                           the fluid cell's for compressible methods) or 1 for incompressible ones

@@ -25,7 +25,7 @@ FACE_GHOST, FACE_PERIODIC, FACE_NOSLIP = 0, 1, 2
 FLAG_ZERO_CENTERED, FLAG_COMPRESSIBLE, FLAG_EQ_SHIFT, FLAG_FORCE, FLAG_SMAGORINSKY = 1, 2, 4, 8, 16
 FLAG_LEARNED_CLOSURE, FLAG_GALILEAN, FLAG_FOURTH_ORDER, FLAG_ADAPT_VELOCITY = 32, 64, 128, 256
 FLAG_STOCHASTIC_ROUNDING, FLAG_GENERAL_CUMULANT, FLAG_HIGHER_ORDERS_RATE1 = 512, 1024, 2048
-FLAG_SHEAR_PROJECTOR = 4096
+FLAG_SHEAR_PROJECTOR, FLAG_GAUSSIAN_EQUILIBRIUM = 4096, 8192
 
 DTYPE_CODES = {"float64": F64, "float32": F32, "float16": F16}
 STENCIL_NAMES = {"D2Q9": D2Q9, "D3Q19": D3Q19, "D3Q27": D3Q27}

@@ -29,7 +29,9 @@ struct CentralTab {
       L0 = 568 + 1458, LU = L0 + 27,           // force: L_i = L0_i + LU_i . u  (population space)
       FMODE = LU + 81,                         // 0 none, 1 Simple/Luo, 2 Guo
       MINV = FMODE + 1,                        // [27 dirs][27 monomials] (D3Q19 backward)
-      SIZE = MINV + 729;
+      GAUSS = MINV + 729,                      // 1: kappa_eq = rho gamma (LBM355_FLAG_GAUSSIAN_EQUILIBRIUM)
+      SHEAR_ON = GAUSS + 1, SHEAR = GAUSS + 2, // 1, s: the closed-form rank-one sum (LBM355_FLAG_SHEAR_PROJECTOR)
+      SIZE = GAUSS + 3;
 };
 
 template <class S, class T>
@@ -159,7 +161,8 @@ __device__ __forceinline__ void collide_central(T (&f)[S::Q], const SweepArgs<T>
 
   forward_central<S, T>(m, u);
 
-  // dk = (kappa_eq - gamma) - beta - kt
+  // dk = (kappa_eq - gamma) - beta - kt: from the table's polynomial in (drho, u), or, for the full continuous
+  // Maxwellian (host-checked), kappa_eq = rho gamma, i.e. kappa_eq - gamma = drho gamma
   T dk[27];
 #define LBM_DK(EX, EY, EZ)                                                                        \
   {                                                                                               \
@@ -168,18 +171,53 @@ __device__ __forceinline__ void collide_central(T (&f)[S::Q], const SweepArgs<T>
     _Pragma("unroll") for (int t = 0; t < 10; ++t) eq += (tab[L::EQ + 20 * e + t] + tab[L::EQ + 20 * e + 10 + t] * drho) * um[t]; \
     dk[e] = eq - beta_e<T, EX, EY, EZ>(u, u2) - m[EX][EY][EZ];                                    \
   }
-  LBM_DK(0, 0, 0) LBM_DK(0, 0, 1) LBM_DK(0, 0, 2) LBM_DK(0, 1, 0) LBM_DK(0, 1, 1) LBM_DK(0, 1, 2)
-  LBM_DK(0, 2, 0) LBM_DK(0, 2, 1) LBM_DK(0, 2, 2) LBM_DK(1, 0, 0) LBM_DK(1, 0, 1) LBM_DK(1, 0, 2)
-  LBM_DK(1, 1, 0) LBM_DK(1, 1, 1) LBM_DK(1, 1, 2) LBM_DK(1, 2, 0) LBM_DK(1, 2, 1) LBM_DK(1, 2, 2)
-  LBM_DK(2, 0, 0) LBM_DK(2, 0, 1) LBM_DK(2, 0, 2) LBM_DK(2, 1, 0) LBM_DK(2, 1, 1) LBM_DK(2, 1, 2)
-  LBM_DK(2, 2, 0) LBM_DK(2, 2, 1) LBM_DK(2, 2, 2)
+#define LBM_DKG(EX, EY, EZ)                                                                       \
+  {                                                                                               \
+    constexpr int e = 9 * EX + 3 * EY + EZ;                                                       \
+    constexpr double g = gam_v<EX> * gam_v<EY> * gam_v<EZ>;                                       \
+    if constexpr (g != 0.0)                                                                       \
+      dk[e] = drho * T(g) - beta_e<T, EX, EY, EZ>(u, u2) - m[EX][EY][EZ];                         \
+    else                                                                                          \
+      dk[e] = -beta_e<T, EX, EY, EZ>(u, u2) - m[EX][EY][EZ];                                      \
+  }
+#define LBM_DK_ALL(M)                                                                                  \
+  M(0, 0, 0) M(0, 0, 1) M(0, 0, 2) M(0, 1, 0) M(0, 1, 1) M(0, 1, 2) M(0, 2, 0) M(0, 2, 1) M(0, 2, 2)   \
+  M(1, 0, 0) M(1, 0, 1) M(1, 0, 2) M(1, 1, 0) M(1, 1, 1) M(1, 1, 2) M(1, 2, 0) M(1, 2, 1) M(1, 2, 2)   \
+  M(2, 0, 0) M(2, 0, 1) M(2, 0, 2) M(2, 1, 0) M(2, 1, 1) M(2, 1, 2) M(2, 2, 0) M(2, 2, 1) M(2, 2, 2)
+  if (Q == 27 && (a.flags & LBM355_FLAG_GAUSSIAN_EQUILIBRIUM)) {
+    LBM_DK_ALL(LBM_DKG)
+  } else {
+    LBM_DK_ALL(LBM_DK)
+  }
+#undef LBM_DK_ALL
+#undef LBM_DKG
 #undef LBM_DK
 
   const T r0 = EQUILIBRIUM ? T(1) : tab[L::R0];
+  // the host-checked closed form of the rank-one sum: the conserved monomials (000, 100, 010, 001) relax with
+  // rate 0 and the deviatoric second-order ones (the shear moments) with r0 + s
+  const bool closed = !EQUILIBRIUM && Q == 27 && (a.flags & LBM355_FLAG_SHEAR_PROJECTOR);
+  const T s_sh = closed ? tab[L::SHEAR] : T(0);
+  auto closed_sum = [&](const T (&x)[27], T scale, T (&acc)[27]) {
+    // acc += scale * (-r0 x_conserved + s dev2(x))
+    acc[0] -= scale * r0 * x[0];
+    acc[9] -= scale * r0 * x[9];
+    acc[3] -= scale * r0 * x[3];
+    acc[1] -= scale * r0 * x[1];
+    const T tr = (x[18] + x[6] + x[2]) * T(1.0 / 3.0);
+    const T ss = scale * s_sh;
+    acc[18] += ss * (x[18] - tr);
+    acc[6] += ss * (x[6] - tr);
+    acc[2] += ss * (x[2] - tr);
+    acc[12] += ss * x[12];
+    acc[10] += ss * x[10];
+    acc[4] += ss * x[4];
+  };
   T dkp[27];
 #pragma unroll
   for (int e = 0; e < 27; ++e) dkp[e] = r0 * dk[e];
-  for (int k = 0; k < (EQUILIBRIUM ? 0 : a.nactive); ++k) {
+  if (closed) closed_sum(dk, T(1), dkp);
+  for (int k = 0; k < ((EQUILIBRIUM || closed) ? 0 : a.nactive); ++k) {
     const TablePtr<T> prow = tab + L::PROW + 27 * k;
     const TablePtr<T> pinv = tab + L::PINV + 27 * k;
     T sk = 0;
@@ -205,7 +243,13 @@ __device__ __forceinline__ void collide_central(T (&f)[S::Q], const SweepArgs<T>
       T kl[3][3][3];
       load_grid<S, T>(fl, kl);
       forward_central<S, T>(kl, u);
-      for (int k = 0; k < a.nactive; ++k) {
+      if (closed) {
+        T kf[27];
+#pragma unroll
+        for (int e = 0; e < 27; ++e) kf[e] = kl[e / 9][(e / 3) % 3][e % 3];
+        closed_sum(kf, T(-0.5), dkp);
+      }
+      for (int k = 0; k < (closed ? 0 : a.nactive); ++k) {
         const TablePtr<T> prow = tab + L::PROW + 27 * k;
         const TablePtr<T> pinv = tab + L::PINV + 27 * k;
         T sk = 0;

@@ -459,9 +459,10 @@ def central_table(method, bindings, dtype=np.float64):
             monos.add(tuple(e))
     if len(monos) != Q:
         raise UnsupportedMethod("central-moment polynomials that need de-aliasing are not supported")
-    tab = np.zeros(540 + 1 + 27 + 1458 + 27 + 81 + 1 + 729)
+    tab = np.zeros(_CENTRAL_SIZE)
     EQ, R0, COEF, PROW, PINV, L0, LU, FMODE, MINV = 0, 540, 541, 568, 568 + 729, 568 + 1458, 568 + 1458 + 27, \
         568 + 1458 + 108, 568 + 1458 + 109
+    GAUSS, SHEAR_ON, SHEAR = MINV + 729, MINV + 730, MINV + 731
     # kappa_eq - gamma as a polynomial in (drho, u): rho -> 1 + drho
     full = eq
     rho = Poly.const(EQ_VARS, 1) + Poly.var(EQ_VARS, "drho")
@@ -516,7 +517,60 @@ def central_table(method, bindings, dtype=np.float64):
         for i in range(Q):
             for j, c in enumerate(cols):
                 tab[MINV + 27 * i + c] = Mi[i, j]
+    # closed forms the kernel may use instead of the general terms (LBM355_FLAG_GAUSSIAN_EQUILIBRIUM,
+    # LBM355_FLAG_SHEAR_PROJECTOR), each checked here against the table it replaces
+    gam = np.array([float(_gamma(e)) for e in _CANON])
+    eqb = tab[EQ:EQ + 540].reshape(27, 20)
+    want = np.zeros((27, 20))
+    want[:, 10] = gam
+    present = np.array([(e in monos) and not (D == 2 and e[2] != 0) for e in _CANON])
+    want[~present] = 0.0
+    # D3Q27 only: there every monomial central moment is one of the method's
+    tab[GAUSS] = 1.0 if Q == 27 and np.array_equal(eqb, want) else 0.0
+    s = None if Q != 27 else _central_shear_scale(tab[COEF:COEF + 27], tab[PROW:PROW + 729].reshape(27, 27),
+                             tab[PINV:PINV + 729].reshape(27, 27), n, r0, D)
+    if s is not None:
+        tab[SHEAR_ON], tab[SHEAR] = 1.0, s
     return tab.astype(dtype), n
+
+
+def central_closed_forms(tab):
+    """(Gaussian equilibrium, closed-form rank-one sum) markers of a central-moment table (central_table)."""
+    if tab.size != _CENTRAL_SIZE:
+        return False, False
+    base = _CENTRAL_SIZE - 3
+    return bool(tab[base] == 1.0), bool(tab[base + 1] == 1.0)
+
+
+_CENTRAL_SIZE = 540 + 1 + 27 + 1458 + 27 + 81 + 1 + 729 + 3
+
+
+def _central_shear_scale(coef, prow, pinv, n, r0, D):
+    """s when the central-moment table's rank-one sum, sum_k coef_k pinv_k prow_k^T (27 x 27, monomial
+    coordinates), equals -r0 on the conserved monomials (000, 100, 010, 001: relaxed with rate 0) plus s times the
+    deviatoric projector on the second-order monomials (the shear moments relaxed with r0 + s); None otherwise
+    (3-D only; checked to 1e-12)."""
+    if D != 3 or n == 0:
+        return None
+    R = np.zeros((27, 27))
+    for k in range(n):
+        R += coef[k] * np.outer(pinv[k], prow[k])
+    idx = {e: i for i, e in enumerate(_CANON)}
+    C = np.zeros((27, 27))
+    for e in ((0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1)):
+        C[idx[e], idx[e]] = -r0
+    Dv = np.zeros((27, 27))
+    diag = [idx[(2, 0, 0)], idx[(0, 2, 0)], idx[(0, 0, 2)]]
+    for a in diag:
+        for b in diag:
+            Dv[a, b] = (1.0 if a == b else 0.0) - 1.0 / 3.0
+    for e in ((1, 1, 0), (1, 0, 1), (0, 1, 1)):
+        Dv[idx[e], idx[e]] = 1.0
+    rest = R - C
+    s = float((rest * Dv).sum() / (Dv * Dv).sum())
+    if np.abs(rest - s * Dv).max() > 1e-12 * max(1.0, np.abs(R).max()):
+        return None
+    return s
 
 
 def fixed_density_table(method):

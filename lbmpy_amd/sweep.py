@@ -17,7 +17,7 @@ import torch
 
 from . import _native as nat
 from .kernel_tables import method_table, family_of, STENCIL_CODES, UnsupportedMethod, bind_rate, \
-    cumulant_is_default_set, moment_shear_projector, \
+    cumulant_is_default_set, moment_shear_projector, central_closed_forms, \
     cumulant_corrections
 from .relaxationrates import get_shear_relaxation_rate
 from .enums import SubgridScaleModel, Target
@@ -208,13 +208,20 @@ class SweepKernel:
         """The closed-form kernel paths this table admits: LBM355_FLAG_HIGHER_ORDERS_RATE1 for the default D3Q27
         cumulant set whose rates 10..26 (orders 3 to 6) are all 1, without corrections;
         LBM355_FLAG_SHEAR_PROJECTOR for a moment-family table whose individually relaxed moments are the shear
-        moments (moment_shear_projector)."""
+        moments (moment_shear_projector); for a central-moment table, the flags its checked closed forms allow
+        (central_closed_forms)."""
         flags = 0
         if (self.family == nat.FAMILY_CUMULANT and not self.general_cumulant and self.corrections is None
                 and bool(np.all(np.asarray(arr[10:27]) == 1.0))):
             flags |= nat.FLAG_HIGHER_ORDERS_RATE1
         if self.family == nat.FAMILY_MOMENT and moment_shear_projector(np.asarray(arr), self.method.stencil.Q):
             flags |= nat.FLAG_SHEAR_PROJECTOR
+        if self.family == nat.FAMILY_CENTRAL:
+            gauss, shear = central_closed_forms(np.asarray(arr))
+            if gauss:
+                flags |= nat.FLAG_GAUSSIAN_EQUILIBRIUM
+            if shear:
+                flags |= nat.FLAG_SHEAR_PROJECTOR
         return flags
 
     def shear_rate(self, bindings):
