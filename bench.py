@@ -529,8 +529,12 @@ def run_families(args):
              ("D3Q27 SRT", Stencil.D3Q27, dict(method=Method.SRT, relaxation_rate=1.8)),
              ("D3Q27 TRT", Stencil.D3Q27, dict(method=Method.TRT, relaxation_rates=[1.8, 1.1])),
              ("D3Q27 MRT", Stencil.D3Q27, dict(method=Method.MRT, relaxation_rate=1.8)),
+             ("D3Q19 central moment", Stencil.D3Q19, dict(method=Method.CENTRAL_MOMENT, relaxation_rate=1.8,
+                                                          compressible=True)),
              ("D3Q27 central moment", Stencil.D3Q27, dict(method=Method.CENTRAL_MOMENT, relaxation_rate=1.8,
                                                           compressible=True)),
+             ("D3Q27 central moment, incompressible", Stencil.D3Q27,
+              dict(method=Method.CENTRAL_MOMENT, relaxation_rate=1.8, compressible=False)),
              ("D3Q27 cumulant", Stencil.D3Q27, dict(method=Method.CUMULANT, relaxation_rate=1.8, compressible=True))]
     out = {}
     for name, st, kw in cases:

@@ -95,14 +95,14 @@ extern "C" {
                                                 without computing them.  The host sets it after checking the rates;
                                                 set with other rates, the sweep still relaxes them with rate 1.
                                                 Members of one batch must agree on it. */
-#define LBM355_FLAG_SHEAR_PROJECTOR 4096 /* moment family (3-D stencils) and central-moment family (D3Q27): the
+#define LBM355_FLAG_SHEAR_PROJECTOR 4096 /* moment and central-moment families, 3-D stencils: the
                                             table's individually relaxed moments are the shear (traceless
                                             second-order) moments with one rate (and, for the central-moment
                                             family, the conserved moments with rate 0), and its SHEAR entries hold
                                             the closed form of their projector; the sweep applies that instead of
                                             n_active rank-one corrections.  The host sets it after checking the
                                             identity. */
-#define LBM355_FLAG_GAUSSIAN_EQUILIBRIUM 8192 /* central-moment family, D3Q27: the equilibrium's central moments are
+#define LBM355_FLAG_GAUSSIAN_EQUILIBRIUM 8192 /* central-moment family, 3-D stencils: the equilibrium's central moments are
                                                  rho gamma_e (the full continuous Maxwellian; the table's GAUSS
                                                  entry), evaluated as such instead of the table's polynomial.  The
                                                  host sets it after checking the table. */

@@ -184,7 +184,7 @@ __device__ __forceinline__ void collide_central(T (&f)[S::Q], const SweepArgs<T>
   M(0, 0, 0) M(0, 0, 1) M(0, 0, 2) M(0, 1, 0) M(0, 1, 1) M(0, 1, 2) M(0, 2, 0) M(0, 2, 1) M(0, 2, 2)   \
   M(1, 0, 0) M(1, 0, 1) M(1, 0, 2) M(1, 1, 0) M(1, 1, 1) M(1, 1, 2) M(1, 2, 0) M(1, 2, 1) M(1, 2, 2)   \
   M(2, 0, 0) M(2, 0, 1) M(2, 0, 2) M(2, 1, 0) M(2, 1, 1) M(2, 1, 2) M(2, 2, 0) M(2, 2, 1) M(2, 2, 2)
-  if (Q == 27 && (a.flags & LBM355_FLAG_GAUSSIAN_EQUILIBRIUM)) {
+  if (D == 3 && (a.flags & LBM355_FLAG_GAUSSIAN_EQUILIBRIUM)) {
     LBM_DK_ALL(LBM_DKG)
   } else {
     LBM_DK_ALL(LBM_DK)
@@ -196,7 +196,7 @@ __device__ __forceinline__ void collide_central(T (&f)[S::Q], const SweepArgs<T>
   const T r0 = EQUILIBRIUM ? T(1) : tab[L::R0];
   // the host-checked closed form of the rank-one sum: the conserved monomials (000, 100, 010, 001) relax with
   // rate 0 and the deviatoric second-order ones (the shear moments) with r0 + s
-  const bool closed = !EQUILIBRIUM && Q == 27 && (a.flags & LBM355_FLAG_SHEAR_PROJECTOR);
+  const bool closed = !EQUILIBRIUM && D == 3 && (a.flags & LBM355_FLAG_SHEAR_PROJECTOR);
   const T s_sh = closed ? tab[L::SHEAR] : T(0);
   auto closed_sum = [&](const T (&x)[27], T scale, T (&acc)[27]) {
     // acc += scale * (-r0 x_conserved + s dev2(x))

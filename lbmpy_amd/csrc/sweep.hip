@@ -634,13 +634,13 @@ int lbm355_sweep(const lbm355_sweep_desc* d, const void* pdf_src, void* pdf_dst,
     return fail(LBM355_ERR_UNSUPPORTED, "LBM355_FLAG_HIGHER_ORDERS_RATE1 is for the default cumulant set without "
                                         "the corrections");
   if ((d->flags & LBM355_FLAG_SHEAR_PROJECTOR) &&
-      !((d->family == LBM355_FAMILY_MOMENT && d->stencil != LBM355_D2Q9) ||
-        (d->family == LBM355_FAMILY_CENTRAL && d->stencil == LBM355_D3Q27)))
-    return fail(LBM355_ERR_UNSUPPORTED, "LBM355_FLAG_SHEAR_PROJECTOR is for the moment family on 3-D stencils and "
-                                        "the D3Q27 central-moment family");
+      !((d->family == LBM355_FAMILY_MOMENT || d->family == LBM355_FAMILY_CENTRAL) && d->stencil != LBM355_D2Q9))
+    return fail(LBM355_ERR_UNSUPPORTED, "LBM355_FLAG_SHEAR_PROJECTOR is for the moment and central-moment families on "
+                                        "3-D stencils");
   if ((d->flags & LBM355_FLAG_GAUSSIAN_EQUILIBRIUM) &&
-      !(d->family == LBM355_FAMILY_CENTRAL && d->stencil == LBM355_D3Q27))
-    return fail(LBM355_ERR_UNSUPPORTED, "LBM355_FLAG_GAUSSIAN_EQUILIBRIUM is for the D3Q27 central-moment family");
+      !(d->family == LBM355_FAMILY_CENTRAL && d->stencil != LBM355_D2Q9))
+    return fail(LBM355_ERR_UNSUPPORTED, "LBM355_FLAG_GAUSSIAN_EQUILIBRIUM is for the central-moment family on 3-D "
+                                        "stencils");
   if ((d->flags & LBM355_FLAG_STOCHASTIC_ROUNDING) && (d->store_dtype != LBM355_F16 || d->compute_dtype != LBM355_F32))
     return fail(LBM355_ERR_ARG, "stochastic rounding is for FP16 storage with FP32 arithmetic");
   if ((d->omega_out || d->eddy_viscosity_out) && !(d->flags & LBM355_FLAG_SMAGORINSKY))

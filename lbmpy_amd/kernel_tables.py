@@ -525,9 +525,12 @@ def central_table(method, bindings, dtype=np.float64):
     want[:, 10] = gam
     present = np.array([(e in monos) and not (D == 2 and e[2] != 0) for e in _CANON])
     want[~present] = 0.0
-    # D3Q27 only: there every monomial central moment is one of the method's
-    tab[GAUSS] = 1.0 if Q == 27 and np.array_equal(eqb, want) else 0.0
-    s = None if Q != 27 else _central_shear_scale(tab[COEF:COEF + 27], tab[PROW:PROW + 729].reshape(27, 27),
+    # 3-D stencils whose monomials form a downward-closed set (every monomial below a used one is used: D3Q27,
+    # D3Q19), so that the raw moments the backward transform uses depend on used central moments only
+    closed_set = D == 3 and all(tuple(a - (i == k) for i, a in enumerate(e)) in monos
+                                for e in monos for k in range(3) if e[k] > 0)
+    tab[GAUSS] = 1.0 if closed_set and np.array_equal(eqb, want) else 0.0
+    s = None if not closed_set else _central_shear_scale(tab[COEF:COEF + 27], tab[PROW:PROW + 729].reshape(27, 27),
                              tab[PINV:PINV + 729].reshape(27, 27), n, r0, D)
     if s is not None:
         tab[SHEAR_ON], tab[SHEAR] = 1.0, s

@@ -68,12 +68,29 @@ def test_closed_forms_match_general_form(pattern, forcing, data_type, tol):
     assert np.abs(a - b).max() <= tol
 
 
+@pytest.mark.parametrize("pattern", ["pull", "aa"])
+def test_d3q19_closed_forms_against_oracle_fp64(pattern):
+    kw = dict(stencil=LBStencil(Stencil.D3Q19), method=Method.CENTRAL_MOMENT, relaxation_rate=1.8, compressible=True,
+              streaming_pattern=pattern, force_model=ForceModel.GUO, force=(1e-5, 0, -1e-5))
+    m = create_lb_method(LBMConfig(**kw))
+    shape = (8, 7, 6)
+    faces = {0: "periodic", 1: "noslip", 2: "periodic"}
+    f0 = seeded_state(m, shape, seed=33, du=0.03)
+    ref, _ = oracle_run(m, shape, faces, pattern, f0, 10)
+    ks = gpu_kernels(kw)
+    from lbmpy_amd.sweep import create_pdf_array
+    assert all(k.descriptor(create_pdf_array((8, 8, 8), 19, 1, "float64"), {}).flags & CLOSED == CLOSED for k in ks)
+    print(f"D3Q19 central closed forms vs oracle: {pattern}", flush=True)
+    got = gpu_run(ks, m, shape, faces, f0, 10)
+    assert np.abs(got - ref).max() <= 1e-12
+
+
 def test_other_central_methods_keep_the_general_form():
     rates = [0.0] * 4 + [1.8] * 5 + [1.3] + [1.0] * 17          # a bulk rate of its own
     (k,) = gpu_kernels(dict(stencil=D3Q27, method=Method.CENTRAL_MOMENT, relaxation_rates=rates, compressible=True))
     assert _flags(k) & nat.FLAG_GAUSSIAN_EQUILIBRIUM
     assert not _flags(k) & nat.FLAG_SHEAR_PROJECTOR
-    (k,) = gpu_kernels(dict(stencil=LBStencil(Stencil.D3Q19), method=Method.CENTRAL_MOMENT, relaxation_rate=1.8,
+    (k,) = gpu_kernels(dict(stencil=LBStencil(Stencil.D2Q9), method=Method.CENTRAL_MOMENT, relaxation_rate=1.8,
                             compressible=True))
     from lbmpy_amd.sweep import create_pdf_array
-    assert not k.descriptor(create_pdf_array((8, 8, 8), 19, 1, "float64"), {}).flags & CLOSED
+    assert not k.descriptor(create_pdf_array((8, 8), 9, 1, "float64"), {}).flags & CLOSED
