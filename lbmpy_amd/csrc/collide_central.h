@@ -14,8 +14,10 @@
 //   force  Simple/Luo: T^-1 P^-1 F_K = L (the population force vector itself);
 //          Guo:        (I - D/2) in polynomial space -> (1 - r0/2) L - 1/2 sum_k c_k T^-1 Pinv[:,k] (P[k,:] . T L)
 //   df     = T(u)^-1 (relax + force part);  f* = f + df.
-// Stencils D2Q9, D3Q27 use the chimera in both directions; D3Q19 uses the chimera forward
-// (absent directions are zeros) and a 19 x 27 matrix M^-1 after the inverse shift.
+// Every stencil uses the chimera in both directions on the 3 x 3 x 3 grid of velocities, absent directions
+// holding zeros.  For D3Q19 the raw moments of the eight monomials with all three exponents non-zero vanish on its
+// directions, so a D3Q19 method's moments are the other 19; setting those eight raw moments to zero before the
+// inverse chimera makes the grid's corners zero and leaves exactly the populations the 19 x 19 inverse gives.
 #pragma once
 #include "lbm_common.h"
 #include "collide_cumulant.h"
@@ -28,8 +30,7 @@ struct CentralTab {
       PROW = 568, PINV = 568 + 729,            // [27 active][27 monomials]
       L0 = 568 + 1458, LU = L0 + 27,           // force: L_i = L0_i + LU_i . u  (population space)
       FMODE = LU + 81,                         // 0 none, 1 Simple/Luo, 2 Guo
-      MINV = FMODE + 1,                        // [27 dirs][27 monomials] (D3Q19 backward)
-      GAUSS = MINV + 729,                      // 1: kappa_eq = rho gamma (LBM355_FLAG_GAUSSIAN_EQUILIBRIUM)
+      GAUSS = FMODE + 1,                       // 1: kappa_eq = rho gamma (LBM355_FLAG_GAUSSIAN_EQUILIBRIUM)
       SHEAR_ON = GAUSS + 1, SHEAR = GAUSS + 2, // 1, s: the closed-form rank-one sum (LBM355_FLAG_SHEAR_PROJECTOR)
       SIZE = GAUSS + 3;
 };
@@ -79,8 +80,7 @@ __device__ __forceinline__ void forward_central(T (&m)[3][3][3], const T (&u)[3]
 
 // central moments about u -> populations (the inverse of forward_central), result on the grid
 template <class S, class T>
-__device__ __forceinline__ void backward_central(T (&m)[3][3][3], const T (&u)[3], TablePtr<T> __restrict__ minv,
-                                                 T (&out)[S::Q]) {
+__device__ __forceinline__ void backward_central(T (&m)[3][3][3], const T (&u)[3], T (&out)[S::Q]) {
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -93,16 +93,14 @@ __device__ __forceinline__ void backward_central(T (&m)[3][3][3], const T (&u)[3
   for (int j = 0; j < 3; ++j)
 #pragma unroll
     for (int k = 0; k < 3; ++k) shift_bwd(m[0][j][k], m[1][j][k], m[2][j][k], u[0]);
-  if (S::Q == 19) {
-    // raw moments of the 19-moment set -> populations by the tabulated inverse moment matrix
+  if constexpr (S::Q == 19) {
+    // the raw moments of x y z, x^2 y z, ..., x^2 y^2 z^2 are zero for D3Q19 populations
 #pragma unroll
-    for (int i = 0; i < S::Q; ++i) {
-      T acc = 0;
+    for (int a = 1; a < 3; ++a)
 #pragma unroll
-      for (int e = 0; e < 27; ++e) acc += minv[27 * i + e] * m[e / 9][(e / 3) % 3][e % 3];
-      out[i] = acc;
-    }
-    return;
+      for (int b = 1; b < 3; ++b)
+#pragma unroll
+        for (int c = 1; c < 3; ++c) m[a][b][c] = T(0);
   }
 #pragma unroll
   for (int j = 0; j < 3; ++j)
@@ -265,7 +263,7 @@ __device__ __forceinline__ void collide_central(T (&f)[S::Q], const SweepArgs<T>
 #pragma unroll
   for (int e = 0; e < 27; ++e) m[e / 9][(e / 3) % 3][e % 3] = (D == 2 && e % 3 != 0) ? T(0) : dkp[e];
   T df[Q];
-  backward_central<S, T>(m, u, tab + L::MINV, df);
+  backward_central<S, T>(m, u, df);
   const T fscale = fmode == 2 ? T(1) - T(0.5) * r0 : T(1);
 #pragma unroll
   for (int i = 0; i < Q; ++i) f[i] += df[i] + (fmode != 0 ? fscale * fl[i] : T(0));

@@ -460,9 +460,9 @@ def central_table(method, bindings, dtype=np.float64):
     if len(monos) != Q:
         raise UnsupportedMethod("central-moment polynomials that need de-aliasing are not supported")
     tab = np.zeros(_CENTRAL_SIZE)
-    EQ, R0, COEF, PROW, PINV, L0, LU, FMODE, MINV = 0, 540, 541, 568, 568 + 729, 568 + 1458, 568 + 1458 + 27, \
-        568 + 1458 + 108, 568 + 1458 + 109
-    GAUSS, SHEAR_ON, SHEAR = MINV + 729, MINV + 730, MINV + 731
+    EQ, R0, COEF, PROW, PINV, L0, LU, FMODE = 0, 540, 541, 568, 568 + 729, 568 + 1458, 568 + 1458 + 27, \
+        568 + 1458 + 108
+    GAUSS, SHEAR_ON, SHEAR = FMODE + 1, FMODE + 2, FMODE + 3
     # kappa_eq - gamma as a polynomial in (drho, u): rho -> 1 + drho
     full = eq
     rho = Poly.const(EQ_VARS, 1) + Poly.var(EQ_VARS, "drho")
@@ -511,12 +511,9 @@ def central_table(method, bindings, dtype=np.float64):
             for a in range(D):
                 tab[LU + 3 * i + a] = float(tu[i][a])
         tab[FMODE] = 2.0 if type(fm).__name__ == "Guo" else 1.0
-    if Q == 19:
-        Mm = moment_matrix([_CANON[c] for c in cols], st)
-        Mi = np.array([[float(v) for v in r] for r in mat_inv(Mm)])      # (Q dirs, Q monomials)
-        for i in range(Q):
-            for j, c in enumerate(cols):
-                tab[MINV + 27 * i + c] = Mi[i, j]
+    if Q == 19 and any(all(x > 0 for x in e) for e in monos):
+        # the kernel's inverse transform sets the raw moments of these monomials to zero (they vanish on D3Q19)
+        raise UnsupportedMethod("a D3Q19 central-moment set cannot hold monomials with three non-zero exponents")
     # closed forms the kernel may use instead of the general terms (LBM355_FLAG_GAUSSIAN_EQUILIBRIUM,
     # LBM355_FLAG_SHEAR_PROJECTOR), each checked here against the table it replaces
     gam = np.array([float(_gamma(e)) for e in _CANON])
@@ -545,7 +542,7 @@ def central_closed_forms(tab):
     return bool(tab[base] == 1.0), bool(tab[base + 1] == 1.0)
 
 
-_CENTRAL_SIZE = 540 + 1 + 27 + 1458 + 27 + 81 + 1 + 729 + 3
+_CENTRAL_SIZE = 540 + 1 + 27 + 1458 + 27 + 81 + 1 + 3
 
 
 def _central_shear_scale(coef, prow, pinv, n, r0, D):
