@@ -102,10 +102,12 @@ extern "C" {
                                             the closed form of their projector; the sweep applies that instead of
                                             n_active rank-one corrections.  The host sets it after checking the
                                             identity. */
-#define LBM355_FLAG_GAUSSIAN_EQUILIBRIUM 8192 /* central-moment family, 3-D stencils: the equilibrium's central moments are
-                                                 rho gamma_e (the full continuous Maxwellian; the table's GAUSS
-                                                 entry), evaluated as such instead of the table's polynomial.  The
-                                                 host sets it after checking the table. */
+#define LBM355_FLAG_GAUSSIAN_EQUILIBRIUM 8192 /* central-moment family, 3-D stencils: the equilibrium's central
+                                                 moments are those of the full continuous Maxwellian, rho gamma_e
+                                                 (table entry GAUSS = 1), or of the incompressible one, gamma_e +
+                                                 drho times a rest Gaussian's (GAUSS = 2), evaluated in closed form
+                                                 instead of the table's polynomial.  The host sets it after checking
+                                                 the table. */
 #define LBM355_FLAG_ADAPT_VELOCITY 256 /* boundary links, UBB: drive the wall at its equilibrium input velocity
                                           u_w - F / (2 rho_ref), rho_ref the link's density (the UBB's own, or
                                           the fluid cell's for compressible methods) or 1 for incompressible ones

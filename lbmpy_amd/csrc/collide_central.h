@@ -30,7 +30,8 @@ struct CentralTab {
       PROW = 568, PINV = 568 + 729,            // [27 active][27 monomials]
       L0 = 568 + 1458, LU = L0 + 27,           // force: L_i = L0_i + LU_i . u  (population space)
       FMODE = LU + 81,                         // 0 none, 1 Simple/Luo, 2 Guo
-      GAUSS = FMODE + 1,                       // 1: kappa_eq = rho gamma (LBM355_FLAG_GAUSSIAN_EQUILIBRIUM)
+      GAUSS = FMODE + 1,                       // 1: kappa_eq = rho gamma, 2: the incompressible form
+                                               // (LBM355_FLAG_GAUSSIAN_EQUILIBRIUM)
       SHEAR_ON = GAUSS + 1, SHEAR = GAUSS + 2, // 1, s: the closed-form rank-one sum (LBM355_FLAG_SHEAR_PROJECTOR)
       SIZE = GAUSS + 3;
 };
@@ -123,6 +124,34 @@ __device__ __forceinline__ T beta_e(const T (&u)[3], const T (&u2)[3]) {
   return beta<EX, EY, EZ, T>(u, u2);
 }
 
+// Central moment about u of a rest Gaussian (c_s^2 = 1/3, centred at 0) of unit weight, truncated at second order
+// in u: prod_a g(e_a, u_a) with g(0) = 1, g(1) = -u, g(2) = 1/3 + u^2 -- the drho part of the incompressible
+// continuous equilibrium's central moments.
+template <class T, int EX, int EY, int EZ>
+__device__ __forceinline__ T rest_gauss(const T (&u)[3], const T (&u2)[3]) {
+  constexpr int e[3] = {EX, EY, EZ};
+  constexpr int n1 = (EX == 1) + (EY == 1) + (EZ == 1), n2 = (EX == 2) + (EY == 2) + (EZ == 2);
+  if constexpr (n1 >= 3) {
+    return T(0);
+  } else {
+    constexpr double third_n2 = n2 == 0 ? 1.0 : (n2 == 1 ? 1.0 / 3.0 : (n2 == 2 ? 1.0 / 9.0 : 1.0 / 27.0));
+    T lin = T(1);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      if (e[a] == 1) lin *= -u[a];
+    T quad = T(third_n2);
+    if constexpr (n1 == 0 && n2 > 0) {
+      constexpr double c = n2 == 1 ? 1.0 : (n2 == 2 ? 1.0 / 3.0 : 1.0 / 9.0);
+      T s = T(0);
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+        if (e[a] == 2) s += u2[a];
+      quad += T(c) * s;
+    }
+    return lin * quad;
+  }
+}
+
 // EQUILIBRIUM: relax every central moment with rate 1 and no force terms, which maps a state to the
 // method's equilibrium at its (rho, j) -- the macroscopic setter's use (macroscopic_value_kernels.py:29-45)
 template <class S, class T, bool EQUILIBRIUM = false>
@@ -159,8 +188,9 @@ __device__ __forceinline__ void collide_central(T (&f)[S::Q], const SweepArgs<T>
 
   forward_central<S, T>(m, u);
 
-  // dk = (kappa_eq - gamma) - beta - kt: from the table's polynomial in (drho, u), or, for the full continuous
-  // Maxwellian (host-checked), kappa_eq = rho gamma, i.e. kappa_eq - gamma = drho gamma
+  // dk = (kappa_eq - gamma) - beta - kt: from the table's polynomial in (drho, u), or in closed form
+  // (host-checked): for the full continuous Maxwellian kappa_eq = rho gamma, i.e. kappa_eq - gamma = drho gamma;
+  // for the incompressible one kappa_eq - gamma = drho rest_gauss(u)
   T dk[27];
 #define LBM_DK(EX, EY, EZ)                                                                        \
   {                                                                                               \
@@ -178,16 +208,26 @@ __device__ __forceinline__ void collide_central(T (&f)[S::Q], const SweepArgs<T>
     else                                                                                          \
       dk[e] = -beta_e<T, EX, EY, EZ>(u, u2) - m[EX][EY][EZ];                                      \
   }
+#define LBM_DKR(EX, EY, EZ)                                                                       \
+  {                                                                                               \
+    constexpr int e = 9 * EX + 3 * EY + EZ;                                                       \
+    dk[e] = drho * rest_gauss<T, EX, EY, EZ>(u, u2) - beta_e<T, EX, EY, EZ>(u, u2) - m[EX][EY][EZ]; \
+  }
 #define LBM_DK_ALL(M)                                                                                  \
   M(0, 0, 0) M(0, 0, 1) M(0, 0, 2) M(0, 1, 0) M(0, 1, 1) M(0, 1, 2) M(0, 2, 0) M(0, 2, 1) M(0, 2, 2)   \
   M(1, 0, 0) M(1, 0, 1) M(1, 0, 2) M(1, 1, 0) M(1, 1, 1) M(1, 1, 2) M(1, 2, 0) M(1, 2, 1) M(1, 2, 2)   \
   M(2, 0, 0) M(2, 0, 1) M(2, 0, 2) M(2, 1, 0) M(2, 1, 1) M(2, 1, 2) M(2, 2, 0) M(2, 2, 1) M(2, 2, 2)
   if (D == 3 && (a.flags & LBM355_FLAG_GAUSSIAN_EQUILIBRIUM)) {
-    LBM_DK_ALL(LBM_DKG)
+    if (tab[L::GAUSS] == T(2)) {
+      LBM_DK_ALL(LBM_DKR)
+    } else {
+      LBM_DK_ALL(LBM_DKG)
+    }
   } else {
     LBM_DK_ALL(LBM_DK)
   }
 #undef LBM_DK_ALL
+#undef LBM_DKR
 #undef LBM_DKG
 #undef LBM_DK
 

@@ -516,17 +516,13 @@ def central_table(method, bindings, dtype=np.float64):
         raise UnsupportedMethod("a D3Q19 central-moment set cannot hold monomials with three non-zero exponents")
     # closed forms the kernel may use instead of the general terms (LBM355_FLAG_GAUSSIAN_EQUILIBRIUM,
     # LBM355_FLAG_SHEAR_PROJECTOR), each checked here against the table it replaces
-    gam = np.array([float(_gamma(e)) for e in _CANON])
     eqb = tab[EQ:EQ + 540].reshape(27, 20)
-    want = np.zeros((27, 20))
-    want[:, 10] = gam
     present = np.array([(e in monos) and not (D == 2 and e[2] != 0) for e in _CANON])
-    want[~present] = 0.0
     # 3-D stencils whose monomials form a downward-closed set (every monomial below a used one is used: D3Q27,
     # D3Q19), so that the raw moments the backward transform uses depend on used central moments only
     closed_set = D == 3 and all(tuple(a - (i == k) for i, a in enumerate(e)) in monos
                                 for e in monos for k in range(3) if e[k] > 0)
-    tab[GAUSS] = 1.0 if closed_set and np.array_equal(eqb, want) else 0.0
+    tab[GAUSS] = _gaussian_mode(eqb, present) if closed_set else 0.0
     s = None if not closed_set else _central_shear_scale(tab[COEF:COEF + 27], tab[PROW:PROW + 729].reshape(27, 27),
                              tab[PINV:PINV + 729].reshape(27, 27), n, r0, D)
     if s is not None:
@@ -534,12 +530,52 @@ def central_table(method, bindings, dtype=np.float64):
     return tab.astype(dtype), n
 
 
+def _gaussian_central_eq(e, at_rest):
+    """kappa_eq - gamma of monomial e as the table's 20 coefficients ((1, drho) x _UMONO): for the full continuous
+    Maxwellian, drho gamma_e (its central moments are rho gamma_e); for the incompressible one (a rest Gaussian of
+    weight rho_0 = 1 centred at u plus one of weight drho centred at 0), drho prod_a g(e_a, u_a) with
+    g(0) = 1, g(1) = -u, g(2) = 1/3 + u^2, truncated at second order in u."""
+    row = np.zeros(20)
+    if not at_rest:
+        row[10] = float(_gamma(e))
+        return row
+    ones = [a for a in range(3) if e[a] == 1]
+    twos = [a for a in range(3) if e[a] == 2]
+    if len(ones) >= 3:
+        return row
+    lin = [0, 0, 0]
+    for a in ones:
+        lin[a] += 1
+    sign = -1.0 if len(ones) == 1 else 1.0
+    third = Fraction(1, 3)
+    terms = {tuple(lin): sign * float(third ** len(twos))}
+    if not ones:
+        for a in twos:
+            q = [0, 0, 0]
+            q[a] = 2
+            terms[tuple(q)] = float(third ** (len(twos) - 1))
+    for um, c in terms.items():
+        row[10 + _UMONO.index(um)] += c
+    return row
+
+
+def _gaussian_mode(eqb, present):
+    """1 when the central-moment equilibrium table is that of the full continuous Maxwellian, 2 when it is the
+    incompressible one (_gaussian_central_eq), 0 otherwise (LBM355_FLAG_GAUSSIAN_EQUILIBRIUM)."""
+    for mode, at_rest in ((1.0, False), (2.0, True)):
+        want = np.array([_gaussian_central_eq(e, at_rest) if present[i] else np.zeros(20)
+                         for i, e in enumerate(_CANON)])
+        if np.array_equal(eqb, want):
+            return mode
+    return 0.0
+
+
 def central_closed_forms(tab):
     """(Gaussian equilibrium, closed-form rank-one sum) markers of a central-moment table (central_table)."""
     if tab.size != _CENTRAL_SIZE:
         return False, False
     base = _CENTRAL_SIZE - 3
-    return bool(tab[base] == 1.0), bool(tab[base + 1] == 1.0)
+    return bool(tab[base] in (1.0, 2.0)), bool(tab[base + 1] == 1.0)
 
 
 _CENTRAL_SIZE = 540 + 1 + 27 + 1458 + 27 + 81 + 1 + 3

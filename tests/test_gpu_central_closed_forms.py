@@ -1,6 +1,7 @@
-"""The D3Q27 central-moment sweep with lbmpy's default method (the full continuous Maxwellian, given its shear
-rate) evaluates the equilibrium's central moments as rho gamma_e and the relaxation's rank-one sum in closed form
-(conserved moments at rate 0, shear moments through the deviatoric projector).  It must agree with the general
+"""The central-moment sweep with lbmpy's default methods (the continuous Maxwellian, compressible or incompressible,
+given the shear rate) evaluates the equilibrium's central moments in closed form (rho gamma_e, or gamma_e + drho
+times a rest Gaussian's) and the relaxation's rank-one sum too (conserved moments at rate 0, shear moments through
+the deviatoric projector).  It must agree with the general
 table-driven form and with the FP64 oracle, with and without the Guo force; other methods keep the general form."""
 import numpy as np
 import pytest
@@ -41,9 +42,7 @@ def test_closed_forms_against_oracle_fp64(pattern, forcing, compressible):
     f0 = seeded_state(m, shape, seed=31, du=0.03)
     ref, _ = oracle_run(m, shape, faces, pattern, f0, 10)
     ks = gpu_kernels(kw)
-    # the incompressible equilibrium's central moments are not rho gamma: only the relaxation's closed form applies
-    want = CLOSED if compressible else nat.FLAG_SHEAR_PROJECTOR
-    assert all(_flags(k) & CLOSED == want for k in ks)
+    assert all(_flags(k) & CLOSED == CLOSED for k in ks)
     print(f"central closed forms vs oracle: {pattern} {forcing} compressible={compressible}", flush=True)
     got = gpu_run(ks, m, shape, faces, f0, 10)
     assert np.abs(got - ref).max() <= 1e-12
@@ -51,8 +50,9 @@ def test_closed_forms_against_oracle_fp64(pattern, forcing, compressible):
 
 @pytest.mark.parametrize("data_type,tol", [("float64", 1e-15), ("float32", 2e-7)])
 @pytest.mark.parametrize("pattern,forcing", CASES)
-def test_closed_forms_match_general_form(pattern, forcing, data_type, tol):
-    kw = _case_kw(pattern, forcing)
+@pytest.mark.parametrize("compressible", [True, False])
+def test_closed_forms_match_general_form(pattern, forcing, data_type, tol, compressible):
+    kw = _case_kw(pattern, forcing, compressible)
     m = create_lb_method(LBMConfig(**kw))
     shape = (64, 5, 4)
     f0 = seeded_state(m, shape, seed=32, du=0.04, noise=3e-4)
