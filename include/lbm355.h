@@ -228,10 +228,11 @@ int lbm355_valu_probe(float* out, int64_t n_threads, int32_t iters, void* stream
 /* Copy of n_planes (19 or 27) planes of plane_elems elements each, plane q at q * plane_stride elements, with
  * one thread per element index loading that element of every plane and then storing them all: the sweep's
  * HBM access pattern (Q aligned streams read and Q written) without the collision.  src == dst is allowed
- * (in place, as an AA-even sweep accesses memory).  No lbmpy counterpart: it measures the ceiling of the
- * fzyx layout that the sweeps are held against. */
+ * (in place, as an AA-even sweep accesses memory).  waves_per_simd 1..8 caps the resident waves (through
+ * unused dynamic LDS, as the AA sweeps' occupancy caps do), 0 leaves the register-limited occupancy.  No lbmpy
+ * counterpart: it measures the ceiling of the fzyx layout that the sweeps are held against. */
 int lbm355_plane_copy(int32_t dtype, int32_t n_planes, const void* src, void* dst, int64_t plane_elems,
-                      int64_t plane_stride, int32_t nontemporal, void* stream);
+                      int64_t plane_stride, int32_t nontemporal, int32_t waves_per_simd, void* stream);
 
 /* ---- Boundary links (general) --------------------------------------------------------------------
  * One boundary condition applied on an index list of links between two sweeps, addressed the way

@@ -98,7 +98,8 @@ def load(path=None):
     lib.lbm355_get_macroscopic.restype = c_int32
     lib.lbm355_stream_copy.argtypes = [c_void_p, c_void_p, c_int64, c_int32, c_void_p]
     lib.lbm355_stream_copy.restype = c_int32
-    lib.lbm355_plane_copy.argtypes = [c_int32, c_int32, c_void_p, c_void_p, c_int64, c_int64, c_int32, c_void_p]
+    lib.lbm355_plane_copy.argtypes = [c_int32, c_int32, c_void_p, c_void_p, c_int64, c_int64, c_int32, c_int32,
+                                      c_void_p]
     lib.lbm355_plane_copy.restype = c_int32
     lib.lbm355_valu_probe.argtypes = [c_void_p, c_int64, c_int32, c_void_p]
     lib.lbm355_valu_probe.restype = c_int32

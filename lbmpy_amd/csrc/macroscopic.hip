@@ -11,6 +11,7 @@
 // Neighbour accesses wrap on periodic axes and use the ghost layer on ghost faces.
 // Both compute in FP64 regardless of the storage type.
 #include "lbm_common.h"
+#include "occupancy.h"
 #include "collide_moment.h"
 #include "collide_cumulant.h"
 #include "collide_central.h"
@@ -431,35 +432,42 @@ __global__ void __launch_bounds__(256) plane_copy_kernel(const T* src, T* dst, i
 }
 
 template <int P, typename T>
-int plane_copy_launch(const void* src, void* dst, int64_t n, int64_t stride, int32_t nt, hipStream_t s) {
+int plane_copy_launch(const void* src, void* dst, int64_t n, int64_t stride, int32_t nt, int32_t waves,
+                      hipStream_t s) {
   const int64_t blocks = (n + 255) / 256;
+  const size_t lds = lbm355::lds_for_wave_cap(waves, 256, 0);
   if (nt)
-    hipLaunchKernelGGL((plane_copy_kernel<P, T, 1>), dim3(blocks), dim3(256), 0, s, (const T*)src, (T*)dst, n, stride);
+    hipLaunchKernelGGL((plane_copy_kernel<P, T, 1>), dim3(blocks), dim3(256), lds, s, (const T*)src, (T*)dst, n,
+                       stride);
   else
-    hipLaunchKernelGGL((plane_copy_kernel<P, T, 0>), dim3(blocks), dim3(256), 0, s, (const T*)src, (T*)dst, n, stride);
+    hipLaunchKernelGGL((plane_copy_kernel<P, T, 0>), dim3(blocks), dim3(256), lds, s, (const T*)src, (T*)dst, n,
+                       stride);
   return hipGetLastError() == hipSuccess ? LBM355_OK : lbm355_set_error(LBM355_ERR_LAUNCH, "plane copy launch failed");
 }
 }  // namespace
 
 extern "C" int lbm355_plane_copy(int32_t dtype, int32_t n_planes, const void* src, void* dst, int64_t plane_elems,
-                                 int64_t plane_stride, int32_t nontemporal, void* stream) {
+                                 int64_t plane_stride, int32_t nontemporal, int32_t waves_per_simd, void* stream) {
   if (!src || !dst || plane_elems < 0 || plane_stride < plane_elems)
     return lbm355_set_error(LBM355_ERR_ARG, "bad plane-copy arguments (plane_stride must be >= plane_elems)");
+  if (waves_per_simd < 0 || waves_per_simd > 8)
+    return lbm355_set_error(LBM355_ERR_ARG, "plane copy: waves_per_simd must be 0 (no cap) to 8");
   if (n_planes != 19 && n_planes != 27)
     return lbm355_set_error(LBM355_ERR_UNSUPPORTED, "plane copy: n_planes must be 19 or 27");
   if (plane_elems == 0) return LBM355_OK;
   hipStream_t s = (hipStream_t)stream;
   const int nt = nontemporal ? 1 : 0;
+  const int w = waves_per_simd;
   switch (dtype) {
     case LBM355_F64:
-      return n_planes == 19 ? plane_copy_launch<19, double>(src, dst, plane_elems, plane_stride, nt, s)
-                            : plane_copy_launch<27, double>(src, dst, plane_elems, plane_stride, nt, s);
+      return n_planes == 19 ? plane_copy_launch<19, double>(src, dst, plane_elems, plane_stride, nt, w, s)
+                            : plane_copy_launch<27, double>(src, dst, plane_elems, plane_stride, nt, w, s);
     case LBM355_F32:
-      return n_planes == 19 ? plane_copy_launch<19, float>(src, dst, plane_elems, plane_stride, nt, s)
-                            : plane_copy_launch<27, float>(src, dst, plane_elems, plane_stride, nt, s);
+      return n_planes == 19 ? plane_copy_launch<19, float>(src, dst, plane_elems, plane_stride, nt, w, s)
+                            : plane_copy_launch<27, float>(src, dst, plane_elems, plane_stride, nt, w, s);
     case LBM355_F16:
-      return n_planes == 19 ? plane_copy_launch<19, _Float16>(src, dst, plane_elems, plane_stride, nt, s)
-                            : plane_copy_launch<27, _Float16>(src, dst, plane_elems, plane_stride, nt, s);
+      return n_planes == 19 ? plane_copy_launch<19, _Float16>(src, dst, plane_elems, plane_stride, nt, w, s)
+                            : plane_copy_launch<27, _Float16>(src, dst, plane_elems, plane_stride, nt, w, s);
     default:
       return lbm355_set_error(LBM355_ERR_ARG, "plane copy: unknown dtype");
   }
