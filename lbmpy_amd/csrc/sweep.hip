@@ -376,46 +376,9 @@ bool use_rowlds(const lbm355_sweep_desc* d, size_t* lds_bytes) {
   return true;
 }
 
-// Occupancy caps for the AA sweeps.  Several of them run faster with fewer waves resident per SIMD than their
-// register use allows: each wave streams 2Q population planes, and past a few waves per SIMD more of them add no
-// bandwidth.  The cap is imposed by requesting dynamic LDS that the kernel does not touch, sized so that only the
-// wanted number of workgroups fits in one CU's LDS.  Waves per SIMD, found by sweeping the request on MI355X at
-// 256^3 (periodic, HIP events per launch; the register-limited default in parentheses):
-//   D3Q27 cumulant      FP32  even 3 (6): 0.619 -> 0.589 ms   odd, LDS row kernel 2 (4): 0.635 -> 0.607 ms
-//                       FP64  even 2 (3): 1.337 -> 1.272 ms   odd, LDS row kernel 2:     1.344 -> 1.240 ms
-//                       FP16  even 4 (6): 0.334 -> 0.316 ms   odd 4:                     0.329 -> 0.316 ms
-//   D3Q19 SRT/TRT/MRT   FP32  even 3:     0.431 -> 0.415 ms   odd, LDS row kernel 3:     0.433 -> 0.420 ms
-//   D3Q27 SRT/TRT/MRT   FP32  even 3:     0.618 -> 0.596 ms   (odd: no consistent gain)
-//   D3Q19 central       FP32  odd, LDS row kernel 3: 0.435 -> 0.415 ms  (even: no gain)
-// D3Q27 central moments (2-3 waves by registers already) and D3Q19 FP16 storage gain nothing, and one wave per
-// SIMD is slower everywhere.  The even caps also pay with NoSlip faces and at 128^3, 384^3 and 512x256x256.  The
-// odd caps do not carry over: with NoSlip faces (a kernel with other register use) they cost 2-10%, and with
-// rows of 512 cells (one 512-thread LDS row block per CU at the cap) 15%; they apply only without fused walls
-// and with blocks of at most 256 threads.  Returns the waves per SIMD, 0 for no cap.
-int aa_wave_cap(const lbm355_sweep_desc* d, bool rowlds, bool walls, unsigned block_threads) {
-  if (d->pattern != LBM355_AA_EVEN && d->pattern != LBM355_AA_ODD) return 0;
-  if (d->flags & LBM355_FLAG_LEARNED_CLOSURE) return 0;
-  const bool even = d->pattern == LBM355_AA_EVEN;
-  if (!even && (walls || block_threads > 256)) return 0;
-  const bool f32 = d->store_dtype == LBM355_F32 && d->compute_dtype == LBM355_F32;
-  const bool f64 = d->store_dtype == LBM355_F64 && d->compute_dtype == LBM355_F64;
-  const bool f16 = d->store_dtype == LBM355_F16 && d->compute_dtype == LBM355_F32;
-  if (d->family == LBM355_FAMILY_CUMULANT && d->stencil == LBM355_D3Q27) {
-    if (f32) return even ? 3 : (rowlds ? 2 : 0);
-    if (f64) return even || rowlds ? 2 : 0;
-    if (f16) return 4;
-  } else if (d->family == LBM355_FAMILY_MOMENT && f32) {
-    if (d->stencil == LBM355_D3Q19) return even || rowlds ? 3 : 0;
-    if (d->stencil == LBM355_D3Q27) return even ? 3 : 0;
-  } else if (d->family == LBM355_FAMILY_CENTRAL && f32) {
-    if (d->stencil == LBM355_D3Q19 && rowlds) return 3;
-  }
-  return 0;
-}
-
-// Dynamic LDS to request for a sweep launch: the kernel's own (kernel_lds), or more where aa_wave_cap applies.
+// Dynamic LDS to request for a sweep launch: the kernel's own (kernel_lds), or more where a cap applies.
 size_t launch_lds(const lbm355_sweep_desc* d, bool rowlds, bool walls, size_t kernel_lds, unsigned block_threads) {
-  return lds_for_wave_cap(aa_wave_cap(d, rowlds, walls, block_threads), block_threads, kernel_lds);
+  return lds_for_wave_cap(sweep_wave_cap(d, rowlds, walls, block_threads), block_threads, kernel_lds);
 }
 
 template <class TC>

@@ -8,6 +8,7 @@
 // (fieldaccess.py:78-87: read slot i at x, write slot i at x + c_i) goes through the same table.
 #include <string>
 
+#include "occupancy.h"
 #include "sweep_eso_kernel.h"
 
 namespace lbm355 {
@@ -50,7 +51,7 @@ int lbm355_launch_sweep_eso(const lbm355_sweep_desc* d, const void* src, void* d
   if (!k) return lbm355_set_error(LBM355_ERR_UNSUPPORTED, "no Eso kernel for this stencil/family/dtype combination");
   const int bx = rows ? d->nx : (d->nx >= 256 ? 256 : ((d->nx + 63) / 64) * 64);
   const dim3 grid((d->nx + bx - 1) / bx, d->ny, d->nz), block(bx);
-  const size_t shm = rows ? lds : 0;
+  const size_t shm = lds_for_wave_cap(sweep_wave_cap(d, rows, false, bx), bx, rows ? lds : 0);
   if (d->compute_dtype == LBM355_F64) {
     SweepArgs<double> a = make_args<double>(d, src, dst, rho, u);
     hipLaunchKernelGGL((void (*)(SweepArgs<double>))k, grid, block, shm, s, a);
