@@ -509,14 +509,19 @@ class LatticeBoltzmannBoundaryHandling:
         import torch
         groups = self.link_groups
         if self._links_version != self.version:
-            self._links_version = self.version
             self._links_dev = []
+            c = np.array(self.method.stencil.stencil_entries)
             for obj, l, v in groups:
+                if isinstance(obj, (ExtrapolationOutflow, SimpleExtrapolationOutflow)) and \
+                        np.any(c[l[:, 3]] @ np.array(obj.normal_direction[:self.dim]) <= 0):
+                    raise ValueError(f"{obj.name}: the normal direction {obj.normal_direction} must point out of the "
+                                     "fluid, into the outflow cells")
                 data = None if v is None else torch.from_numpy(np.ascontiguousarray(v)).to(device)
                 if isinstance(obj, ExtrapolationOutflow):
                     data = torch.zeros((len(l), 2), dtype=torch.float64, device=device)  # set before first use
                 self._links_dev.append([obj, torch.from_numpy(np.ascontiguousarray(l)).to(device), data,
                                         not isinstance(obj, ExtrapolationOutflow)])
+            self._links_version = self.version
         return self._links_dev
 
     def apply(self, pdf, pattern_code):
@@ -537,14 +542,8 @@ class LatticeBoltzmannBoundaryHandling:
             raise TypeError("boundaries run on GPU tensors of float64, float32 or float16")
         lib = nat.load()
         faces = self._faces
-        c = np.array(m.stencil.stencil_entries)
         for group in groups:
             obj, links, vel, ready = group
-            if isinstance(obj, (ExtrapolationOutflow, SimpleExtrapolationOutflow)):
-                dirs = links[:, 3].cpu().numpy() if hasattr(links, "cpu") else links[:, 3]
-                if np.any(c[dirs] @ np.array(obj.normal_direction[:D]) <= 0):
-                    raise ValueError(f"{obj.name}: the normal direction {obj.normal_direction} must point out of the "
-                                     "fluid, into the outflow cells")
             d = nat.BoundaryDesc()
             d.stencil = nat.STENCIL_NAMES[m.stencil.name]
             d.pattern = int(pattern_code)
