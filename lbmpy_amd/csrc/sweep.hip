@@ -365,7 +365,9 @@ int xshift_dirs(int stencil) { return stencil == LBM355_D2Q9 ? 6 : (stencil == L
 // LDS row shifting applies when a block can span a whole periodic row
 // Not for FP16 storage: with 2-byte populations a wavefront access moves 128 bytes, and the barriers around the LDS
 // shift keep too few of them in flight.  The direct kernel, whose x-shifted loads are served partly from cache,
-// runs the D3Q27 cumulant AA-odd sweep at 256^3 in 0.441 ms against 0.487 ms on MI355X.
+// ran the D3Q27 cumulant AA-odd sweep at 256^3 in 0.441 ms against 0.487 ms on MI355X.  With both at their best
+// occupancy the LDS kernel's odd step is 3% faster (0.313 against 0.322 ms), but the even step after it 2% slower
+// (0.322 against 0.316 ms), and D3Q19 FP16 loses 10% with it: the direct kernel stays.
 bool use_rowlds(const lbm355_sweep_desc* d, size_t* lds_bytes) {
   if (d->store_dtype == LBM355_F16) return false;
   if (d->pattern == LBM355_AA_EVEN || table_pattern(d->pattern) || d->face[0] != LBM355_FACE_PERIODIC) return false;
