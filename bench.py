@@ -205,9 +205,11 @@ def aa_schedule(warmup, steps, total_steps):
     return range(0, warmup), range(warmup, warmup + steps), range(warmup + steps, total)
 
 
-def run_form(dtype, n, steps, warmup, total_steps, u0=0.05):
-    """Initialise the TGV, run `warmup` untimed steps, `steps` timed steps (one event pair per launch on
-    the launch stream), then untimed steps up to `total_steps`; returns timings and (rho, u)."""
+def run_form(dtype, n, steps, warmup, total_steps, u0=0.05, tune=True):
+    """Initialise the TGV, pick the two kernels' occupancy caps by timing them on a copy of the state
+    (lbmpy_amd.sweep.tune_occupancy, unless `tune` is false), run `warmup` untimed steps, `steps` timed steps (one
+    event pair per launch on the launch stream), then untimed steps up to `total_steps`; returns timings and
+    (rho, u)."""
     import torch
     from lbmpy_amd import (LBMConfig, LBStencil, Stencil, Method, create_lb_function, CreateKernelConfig, Target,
                            Timestep)
@@ -234,6 +236,10 @@ def run_form(dtype, n, steps, warmup, total_steps, u0=0.05):
     torch.cuda.synchronize()
 
     kernels = (odd, even)                                 # the first AA step is ODD
+    tuning = None
+    if tune:
+        from lbmpy_amd.sweep import tune_occupancy
+        tuning = tune_occupancy(kernels, pdf, candidates=(0, -1, 2, 3, 4))
     warm, timed, tail = aa_schedule(warmup, steps, total_steps)
     for t in warm:
         kernels[t % 2](src=pdf)
@@ -259,8 +265,11 @@ def run_form(dtype, n, steps, warmup, total_steps, u0=0.05):
     del pdf, storage
     torch.cuda.empty_cache()
     avg = (lambda v: sum(v) / len(v) if v else None)
+    caps = {"aa_odd": odd.waves_per_simd, "aa_even": even.waves_per_simd,
+            "tuning_ms": None if tuning is None else
+            {name: {str(c): round(t, 4) for c, t in tuning[i].items()} for i, name in enumerate(("aa_odd", "aa_even"))}}
     return dict(ms=ms, odd_ms=avg(odd_ms), even_ms=avg(even_ms), launch_ms=avg(per), omega=omega, rho=rho_f,
-                u=u_f, finite=finite, total_steps=tail.stop)
+                u=u_f, finite=finite, total_steps=tail.stop, caps=caps)
 
 
 def poiseuille_startup(y, t, force, nu, h, terms=4000):
@@ -589,6 +598,9 @@ def main():
     ap.add_argument("--forms", default="float64,float16",
                     help="other storage forms run beside the headline one ('' for none)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the library's occupancy caps instead of timing the headline kernels at several caps "
+                         "(on a copy of the state, before the warm-up) and keeping the fastest")
     ap.add_argument("--workload", default="tgv",
                     choices=["tgv", "channel", "ensemble", "closure", "patterns", "families"],
                     help="tgv: the headline D3Q27 cumulant TGV; channel: D3Q19 MRT channel in three storage forms; "
@@ -642,14 +654,14 @@ def main():
     results = {}
     ref = None
     for dt in forms:
-        r = run_form(dt, n, args.steps, args.warmup, total)
+        r = run_form(dt, n, args.steps, args.warmup, total, tune=not args.no_tune)
         b = 2 * 27 * size[dt]
         res = {"mlups": round(cells * args.steps / (r["ms"] / 1e3) / 1e6, 1),
                "ms_per_step": round(r["ms"] / args.steps, 4),
                "kernel_ms": {"aa_" + k: (None if r[k + "_ms"] is None else round(r[k + "_ms"], 4))
                              for k in ("odd", "even")},
                "achieved_gbs": round(b * cells / (r["launch_ms"] / 1e3) / 1e9, 1), "bytes_per_cell_step": b,
-               "finite": r["finite"]}
+               "finite": r["finite"], "waves_per_simd": r["caps"]}
         res["roofline_frac"] = round(res["achieved_gbs"] / HBM_PEAK_GBS, 4)
         if dt == "float64":
             ref = (r["rho"], r["u"])

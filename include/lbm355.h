@@ -149,7 +149,8 @@ typedef struct lbm355_sweep_desc {
   const void* velocity_in;  /* moment family: equilibrium velocity read instead of computed (lbmpy velocity_input,
                                creationfunctions.py:675-677); D components */
   uint32_t rounding_seed;   /* LBM355_FLAG_STOCHASTIC_ROUNDING: seed of this launch's rounding (a new one per launch) */
-  int32_t reserved;         /* 0 */
+  int32_t waves_per_simd;   /* occupancy cap of this launch: 0 = the library's table for the kernel, -1 = none,
+                               1..8 = at most that many waves per SIMD (results do not depend on it) */
 } lbm355_sweep_desc;
 
 /* One stream-collide time step over the interior.

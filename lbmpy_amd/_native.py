@@ -55,7 +55,7 @@ class SweepDesc(ctypes.Structure):
                 ("force", c_double * 3), ("smagorinsky_constant", c_double), ("omega_shear", c_double),
                 ("tables", c_void_p), ("closure", c_void_p),
                 ("omega_out", c_void_p), ("eddy_viscosity_out", c_void_p), ("density_in", c_void_p),
-                ("velocity_in", c_void_p), ("rounding_seed", ctypes.c_uint32), ("reserved", c_int32)]
+                ("velocity_in", c_void_p), ("rounding_seed", ctypes.c_uint32), ("waves_per_simd", c_int32)]
 
 
 EXPORTED = ("lbm355_sweep", "lbm355_table_elements", "lbm355_noslip_links", "lbm355_reduce_conserved",

@@ -446,6 +446,8 @@ int validate(const lbm355_sweep_desc* d) {
       return fail(LBM355_ERR_UNSUPPORTED, "the learned closure runs with pull or AA streaming");
   }
   if (!d->tables) return fail(LBM355_ERR_ARG, "method table pointer is NULL");
+  if (d->waves_per_simd < -1 || d->waves_per_simd > 8)
+    return fail(LBM355_ERR_ARG, "waves_per_simd must be -1 (no cap), 0 (the library's table) or 1..8");
   if ((d->family == LBM355_FAMILY_MOMENT || d->family == LBM355_FAMILY_CENTRAL) &&
       (d->n_active < 0 || d->n_active > Q))
     return fail(LBM355_ERR_ARG, "n_active out of range");

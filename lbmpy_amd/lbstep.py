@@ -292,8 +292,8 @@ class LatticeBoltzmannStep:
                          repr(getattr(o, "_density", None)), repr(getattr(o, "density", None)),
                          repr(getattr(o, "normal_direction", None))) for o, l, v, _ in groups)
         params = tuple(sorted((str(k), repr(v)) for k, v in self.kernel_params.items()))
-        kernels = tuple((id(k), id(getattr(k, "closure", None)), tuple(getattr(k, "faces", None) or ()))
-                        for k in self._kernels)
+        kernels = tuple((id(k), id(getattr(k, "closure", None)), tuple(getattr(k, "faces", None) or ()),
+                         getattr(k, "waves_per_simd", 0)) for k in self._kernels)
         return (self._src.data_ptr(), self._tmp.data_ptr() if self._tmp is not None else 0, self._aa_next_odd,
                 faces, bh.version, bparams, params, kernels)
 
@@ -338,6 +338,18 @@ class LatticeBoltzmannStep:
             left -= 2 * n
         for _ in range(left):
             self.time_step()
+
+    def tune_occupancy(self, candidates=(0, -1, 1, 2, 3, 4, 5, 6), reps=8):
+        """Pick each sweep kernel's occupancy cap by timing it on a copy of the populations
+        (lbmpy_amd.sweep.tune_occupancy); the run's state does not advance.  Returns the timings."""
+        from .sweep import tune_occupancy
+        self.pre_run()
+        self._graph = None
+        even_odd = self._kernels
+        kernels = list(even_odd) if self._two_arrays else \
+            ([even_odd[1], even_odd[0]] if self._aa_next_odd else list(even_odd))
+        return tune_occupancy(kernels, self._src, self._tmp if self._two_arrays else None, candidates=candidates,
+                              reps=reps, **self.kernel_params)
 
     def run(self, time_steps):
         self.pre_run()

@@ -143,6 +143,9 @@ class SweepKernel:
             raise NotImplementedError("the Smagorinsky model acts on the shear rates of the default polynomial "
                                       "cumulant set")
         self.closure = None
+        #: occupancy cap of this kernel's launches: 0 = the library's table, -1 = none, 1..8 = waves per SIMD
+        #: (results do not depend on it; tune_occupancy picks the fastest on the GPU at hand)
+        self.waves_per_simd = 0
         self._shear_rate = None          # (bindings key, bound shear rate): the lookup is symbolic and slow
         self._tables = {}
         self.corrections = cumulant_corrections(lc)
@@ -290,6 +293,7 @@ class SweepKernel:
         d.flags = flags
         d.n_active = nact
         d.tables = tab.data_ptr()
+        d.waves_per_simd = int(self.waves_per_simd)
         return d
 
     # ------------------------------------------------------------------ call
@@ -420,6 +424,65 @@ class SweepKernel:
 
     def __repr__(self):
         return f"SweepKernel({self.update_rule!r})"
+
+
+def tune_occupancy(kernels, src, dst=None, candidates=(0, -1, 1, 2, 3, 4, 5, 6), reps=8, **params):
+    """Time every kernel of a time loop at each occupancy cap in `candidates` (``SweepKernel.waves_per_simd``: 0 the
+    library's table, -1 none, n at most n waves per SIMD) and leave each kernel at its fastest; returns
+    {kernel index: {cap: ms per launch}}.
+
+    `kernels` are the loop's kernels in launch order: one pull / push kernel (which swaps `src` and `dst`), or the
+    ODD and EVEN kernels of an in-place pattern (which alternate on `src`).  The loop runs on copies of the arrays,
+    so the caller's state does not advance; `params` are the kernels' free-symbol values.  The best cap of a kernel
+    can differ between MI355X boxes: the caps of the library's table were measured on some and are one or two
+    waves off on others."""
+    import torch
+    a = src.clone()
+    b = dst.clone() if dst is not None else None
+    # stochastic rounding draws a new seed per launch: the tuning launches must not shift the caller's sequence
+    launches = [getattr(k, "rounding_launches", None) for k in kernels]
+
+    def launch(j, k):
+        names = k.lbm_config.field_name, k.lbm_config.temporary_field_name
+        if b is not None:
+            k(**{names[0]: a if j % 2 == 0 else b, names[1]: b if j % 2 == 0 else a}, **params)
+        else:
+            k(**{names[0]: a}, **params)
+
+    def rounds(count, timed=None, ev=None):
+        for r in range(count):
+            for j, kk in enumerate(kernels):
+                if j == timed:
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record()
+                    launch(len(kernels) * r + j, kk)
+                    e1.record()
+                    ev.append((e0, e1))
+                else:
+                    launch(len(kernels) * r + j, kk)
+
+    rounds(4)                                   # the GPU's clocks settle before the first candidate is timed
+    result = {}
+    for i, k in enumerate(kernels):
+        times = {}
+        keep = k.waves_per_simd
+        # two passes over the candidates; each keeps its better pass, which damps the drift between them
+        for _ in range(2):
+            for cap in candidates:
+                k.waves_per_simd = cap
+                rounds(1)
+                ev = []
+                rounds(reps, i, ev)
+                torch.cuda.synchronize()
+                t = sum(e0.elapsed_time(e1) for e0, e1 in ev) / len(ev)
+                times[cap] = min(t, times.get(cap, t))
+        k.waves_per_simd = min(times, key=times.get) if times else keep
+        result[i] = times
+    for k, n in zip(kernels, launches):
+        if n is not None:
+            k.rounding_launches = n
+    del a, b
+    return result
 
 
 def _mix32(x):
