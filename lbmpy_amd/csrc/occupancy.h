@@ -72,9 +72,9 @@ constexpr WaveCaps kWaveCaps[] = {
 // never staged) and do not carry over: with fused NoSlip faces (kernels of other register use) the AA-odd caps
 // cost 2-10%, and with 512-cell rows, where a 512-thread row block is alone on its CU at the cap, 15%.  So they
 // apply only there.  The learned closure gains nothing from a cap.
-// A launch's own waves_per_simd (1..8, or -1 for none) overrides the table: the best cap of the AA-odd kernels differs
-// between MI355X boxes (D3Q27 cumulant FP32 at 256^3: 0.607 ms at 2 waves and 0.664 at 3 on one, 0.658 and 0.609 on
-// another), and lbmpy_amd.sweep.tune_occupancy picks it by timing.
+// A launch's own waves_per_simd (1..8, or -1 for none) overrides the table: the best cap of the AA-odd kernels
+// differs from one MI355X to another (D3Q27 cumulant FP32 at 256^3: 0.607 ms at 2 waves and 0.664 at 3 on one,
+// 0.658 and 0.609 on another), and lbmpy_amd.sweep.tune_occupancy picks it by timing.
 inline int sweep_wave_cap(const lbm355_sweep_desc* d, bool staged, bool walls, unsigned block_threads) {
   if (d->waves_per_simd != 0) return d->waves_per_simd > 0 ? d->waves_per_simd : 0;
   if (d->flags & LBM355_FLAG_LEARNED_CLOSURE) return 0;

@@ -435,7 +435,7 @@ def tune_occupancy(kernels, src, dst=None, candidates=(0, -1, 1, 2, 3, 4, 5, 6),
     ODD and EVEN kernels of an in-place pattern (which alternate on `src`).  The loop runs on copies of the arrays
     (GPU memory for one more set of them), so the caller's state does not advance; `params` are the kernels'
     free-symbol values.  The best cap of a kernel
-    can differ between MI355X boxes: the caps of the library's table were measured on some and are one or two
+    can differ from one MI355X to another: the caps of the library's table were measured on some and are one or two
     waves off on others."""
     import torch
     a = src.clone()
