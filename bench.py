@@ -315,6 +315,9 @@ def run_channel(args):
         src = create_pdf_array((nx, ny, nz), 19, ghost_layers=0, dtype=dt)
         dst = create_pdf_array((nx, ny, nz), 19, ghost_layers=0, dtype=dt)
         mass0 = k.conserved(src)[0]
+        if not args.no_tune:
+            from lbmpy_amd.sweep import tune_occupancy
+            tune_occupancy([k], src, dst, candidates=(0, -1, 2, 3, 4))
         for _ in range(min(args.warmup, steps)):
             k(src=src, dst=dst)
             src, dst = dst, src
@@ -343,7 +346,8 @@ def run_channel(args):
                "achieved_gbs": round(2 * 19 * src.element_size() * nx * ny * nz * timed / (ms / 1e3) / 1e9, 1),
                "profile_err_vs_analytic_max_rel": float(np.abs(prof - u_an).max() / umax),
                "mass_drift_rel": float((mass1 - mass0) / (nx * ny * nz)),
-               "u_centre": float(prof[ny // 2]), "u_centre_analytic": float(u_an[ny // 2])}
+               "u_centre": float(prof[ny // 2]), "u_centre_analytic": float(u_an[ny // 2]),
+               "waves_per_simd": k.waves_per_simd}
         if dt == "float64":
             ref_profile = prof
         else:
