@@ -150,7 +150,8 @@ typedef struct lbm355_sweep_desc {
                                creationfunctions.py:675-677); D components */
   uint32_t rounding_seed;   /* LBM355_FLAG_STOCHASTIC_ROUNDING: seed of this launch's rounding (a new one per launch) */
   int32_t waves_per_simd;   /* occupancy cap of this launch: 0 = the library's table for the kernel, -1 = none,
-                               1..8 = at most that many waves per SIMD (results do not depend on it) */
+                               1..8 = at most that many waves per SIMD (results do not depend on it; the batched
+                               sweep, whose launch shape is its own argument, ignores it) */
 } lbm355_sweep_desc;
 
 /* One stream-collide time step over the interior.
