@@ -534,11 +534,12 @@ def run_patterns(args):
 def run_families(args):
     """Every collision family at 256^3 (FP32 storage and arithmetic, AA streaming, periodic, rest state): D3Q19 and
     D3Q27 SRT, TRT and MRT, D3Q27 central-moment and cumulant, each given by its shear rate (lbmpy's defaults for the
-    other rates).  MLUPS and bandwidth per family from HIP events around each launch on the launch stream."""
+    other rates).  Each family's two kernels are tuned for occupancy first (unless --no-tune).  MLUPS and bandwidth
+    per family from HIP events around each launch on the launch stream."""
     import torch
     from lbmpy_amd import (LBMConfig, LBStencil, Stencil, Method, create_lb_function, CreateKernelConfig, Target,
                            Timestep)
-    from lbmpy_amd.sweep import create_pdf_array
+    from lbmpy_amd.sweep import create_pdf_array, tune_occupancy
     n = args.n
     cells = n ** 3
     cfg = CreateKernelConfig(target=Target.GPU, data_type="float32", ghost_layers=0)
@@ -563,6 +564,8 @@ def run_families(args):
         for k in ks:
             k.set_faces(["periodic"] * 3)
         a = create_pdf_array((n, n, n), stencil.Q, ghost_layers=0, dtype="float32")
+        if not args.no_tune:
+            tune_occupancy(ks, a, candidates=(0, -1, 2, 3, 4))
         warm, timed, _ = aa_schedule(args.warmup, args.steps, 0)
         for t in warm:
             ks[t % 2](src=a)
@@ -578,7 +581,8 @@ def run_families(args):
         b = 2 * stencil.Q * 4
         gbs = b * cells / (launch_ms / 1e3) / 1e9
         out[name] = {"mlups": round(cells / (launch_ms / 1e3) / 1e6, 1), "launch_ms": round(launch_ms, 4),
-                     "bytes_per_cell": b, "achieved_gbs": round(gbs, 1), "roofline_frac": round(gbs / HBM_PEAK_GBS, 4)}
+                     "bytes_per_cell": b, "achieved_gbs": round(gbs, 1), "roofline_frac": round(gbs / HBM_PEAK_GBS, 4),
+                     "waves_per_simd": {"aa_odd": ks[0].waves_per_simd, "aa_even": ks[1].waves_per_simd}}
         print(f"{name}: {out[name]}", flush=True)
         del a, ks
         torch.cuda.empty_cache()
